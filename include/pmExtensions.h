@@ -1,0 +1,84 @@
+/*
+ * unicorn-mi355x — runtime controls with no counterpart in the reference API.
+ *
+ * The reference fixes its GPU stub behaviour at compile time (pmInternalDefinitions.h:117-224)
+ * and through environment variables read by its test harness (commonAPI.cpp:580-625). This
+ * build exposes the HIP stub's execution modes, its HBM budget and its counters here so that
+ * drivers and tests can compare them on the same tasks.
+ *
+ * HIP stub modes:
+ *   PM_HIP_MODE_PLAIN     per-subtask stream, launch and staging, as the reference stub:
+ *                         subscribed records are gathered on the host into pinned memory,
+ *                         copied H2D into a COMPACT device buffer, the GPU callback runs, and
+ *                         written records go D2H and are scattered back (pmExecutionStub.cpp
+ *                         :3216-3988). Read-only buffers are cached by exact subscription.
+ *   PM_HIP_MODE_RESIDENT  address spaces stay mirrored in HBM across tasks; SUBSCRIPTION_OPTIMAL
+ *                         resolves to a zero-copy NATURAL view of the mirror (or a device-side
+ *                         gather when COMPACT is forced); one launch per subtask.
+ *   PM_HIP_MODE_BATCHED   RESIDENT, and the launches of many subtasks (of one or several tasks)
+ *                         are recorded into a device work queue and executed by one persistent
+ *                         kernel per kernel family, optionally replayed from a hipGraph.
+ */
+#ifndef UNICORN_PM_EXTENSIONS_H
+#define UNICORN_PM_EXTENSIONS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include "pmPublicDefinitions.h"
+#endif
+
+#define PM_HIP_MODE_PLAIN 0
+#define PM_HIP_MODE_RESIDENT 1
+#define PM_HIP_MODE_BATCHED 2
+
+#ifdef __cplusplus
+namespace pm
+{
+#endif
+
+typedef struct pmHipStubStats
+{
+    uint64_t subtasks;          /* GPU subtasks executed */
+    uint64_t batches;           /* work-queue flushes (BATCHED) */
+    uint64_t h2d_bytes;         /* host-to-device staging bytes */
+    uint64_t d2h_bytes;         /* device-to-host staging bytes */
+    uint64_t d2d_bytes;         /* device gather/scatter bytes (COMPACT views of mirrors) */
+    uint64_t cache_hits;        /* PLAIN: read-only device buffers reused */
+    uint64_t evictions;         /* HBM mirrors evicted to honour the budget */
+} pmHipStubStats;
+
+#ifdef __cplusplus
+/* Select the HIP stub mode for tasks submitted afterwards. */
+pmStatus pmSetHipStubMode(int pMode);
+int pmGetHipStubMode();
+/* Force the view used for SUBSCRIPTION_OPTIMAL on the HIP stub in RESIDENT/BATCHED modes:
+ * SUBSCRIPTION_NATURAL (default, zero-copy) or SUBSCRIPTION_COMPACT (device gather/scatter). */
+pmStatus pmSetHipOptimalView(pmSubscriptionVisibilityType pType);
+/* Use hipGraph replay for batch flushes (default on). */
+pmStatus pmSetHipGraphs(bool pEnable);
+/* Maximum subtasks in flight on the HIP stub (PLAIN/RESIDENT) and per batch (BATCHED). */
+pmStatus pmSetHipPipelineDepth(int pDepth);
+pmStatus pmSetHipBatchLimit(int pSubtasks);
+/* Bytes of HBM the runtime may hold for mirrors and subtask buffers. */
+pmStatus pmSetHbmBudget(size_t pBytes);
+size_t pmGetHbmBudget();
+size_t pmGetHbmResident();
+/* Number of CPU stub threads (0 = one per host core). Takes effect at pmInitialize. */
+pmStatus pmSetCpuStubCount(int pCount);
+pmStatus pmGetHipStubStats(pmHipStubStats* pStats);
+pmStatus pmResetHipStubStats();
+/* Make the HBM mirror of an address space current and keep it (prefetch). */
+pmStatus pmPrefetchToDevice(pmMemHandle pMemHandle);
+/* Device pointer of an address space's HBM mirror (prefetching it if needed). The caller may
+ * read it on the GPU; writes through it must be followed by pmMarkDeviceWritten. */
+pmStatus pmGetDeviceMemPtr(pmMemHandle pMemHandle, void** pDevicePtr);
+pmStatus pmMarkDeviceWritten(pmMemHandle pMemHandle);
+/* Whether a HIP device is available to the runtime. */
+bool pmHasHipDevice();
+
+} /* namespace pm */
+#endif
+
+#endif

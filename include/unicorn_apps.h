@@ -1,0 +1,65 @@
+/*
+ * unicorn-mi355x — C-ABI drivers of the four benchmark apps (libunicorn_apps.so).
+ *
+ * The apps themselves are written against the pm:: API (pmPublicDefinitions.h) exactly as the
+ * reference's testSuite apps are: data-distribution callbacks subscribe memory, CPU and GPU
+ * subtask callbacks compute. These drivers replace the reference's per-app main()/commonStart
+ * harness (testSuite/common/source/code/src/commonAPI.cpp:142-298) with plain C entry points
+ * that tests and benchmarks call through ctypes: create the address spaces, load inputs, submit
+ * the task(s) on a chosen device set, fetch outputs.
+ *
+ * All functions return 0 on success; on failure a non-zero pmStatus (or -1 for a driver-level
+ * error) and ua_last_error() describes it. Handles are opaque.
+ */
+#ifndef UNICORN_APPS_H
+#define UNICORN_APPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* device sets, as the reference harness's local CPU / GPU / CPU+GPU modes */
+enum { UA_DEVICE_CPU = 0, UA_DEVICE_GPU = 1, UA_DEVICE_BOTH = 2 };
+
+typedef struct ua_stats {
+    uint64_t subtasks, batches, h2d_bytes, d2h_bytes, d2d_bytes, cache_hits, evictions;
+    uint64_t plain_launches, persistent_launches, queue_items;
+} ua_stats;
+
+int ua_init(int cpu_stubs);                   /* pmInitialize; cpu_stubs 0 = one per core */
+int ua_finalize(void);
+int ua_has_gpu(void);
+int ua_set_hip_mode(int mode);                /* PM_HIP_MODE_PLAIN / RESIDENT / BATCHED */
+int ua_set_hip_compact(int compact);          /* force COMPACT views in RESIDENT/BATCHED */
+int ua_set_hip_graphs(int on);
+int ua_set_hip_depth(int depth);
+int ua_set_batch_limit(int subtasks);
+int ua_set_hbm_budget(uint64_t bytes);
+int ua_get_stats(ua_stats* s);
+int ua_reset_stats(void);
+const char* ua_last_error(void);
+
+/* ---- imageFiltering (reference testSuite/imageFiltering) ----
+ * Input: width*height*3 bytes, rows stored bottom-up (the reference's inverted layout,
+ * imageFiltering.cpp:98-134); output: width*height*3 bytes top-down, channels swapped. */
+typedef struct ua_if_config {
+    int32_t width, height, radius;
+    int32_t mode;          /* 0 = reference char arithmetic (mod 256), 1 = shift + clamp */
+    int32_t shift;         /* right shift of mode 1 */
+    int32_t tile;          /* subtask tile side (reference TILE_DIM 2048) */
+    int8_t weights[32 * 32];   /* (2r+1)^2 values, row-major */
+} ua_if_config;
+void* ua_if_create(const ua_if_config* cfg);
+int ua_if_set_input(void* h, const uint8_t* image);
+int ua_if_run(void* h, int device, double* seconds);
+int ua_if_get_output(void* h, uint8_t* out);
+int ua_if_destroy(void* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

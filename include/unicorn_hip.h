@@ -1,0 +1,190 @@
+/*
+ * unicorn-mi355x — C-ABI of the HIP layer (libunicorn_hip.so).
+ *
+ * This is the only library that calls HIP. The pm:: runtime (pmStubHIP) and the apps' GPU
+ * callbacks link to it; Python tests reach it through ctypes. It replaces, for one MI355X:
+ *   - the reference's dlsym table over libcudart and its device helpers
+ *     (source/code/src/pmCudaInterface.cu:69-378),
+ *   - the generic H2D / launch / D2H sequence of one GPU subtask (pmCudaInterface.cu:474-550),
+ *   - the CUDA/cuBLAS calls inside the apps' GPU callbacks
+ *     (the testSuite apps' *CUDA.cu files, cited per launcher below).
+ *
+ * Conventions: plain pointers and sizes only; every function returns 0 on success and a
+ * non-zero uh_status otherwise (uh_last_error() describes the last failure on the calling
+ * thread). Streams are opaque `void*` (a hipStream_t). A stream created by
+ * uh_batch_create() is a *batch handle*: kernel launchers called with it record a work item
+ * in a device work queue instead of launching, and uh_batch_flush() runs all recorded items of
+ * a family in one persistent launch. Launchers never synchronise.
+ */
+#ifndef UNICORN_HIP_H
+#define UNICORN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum uh_status {
+    UH_OK = 0,
+    UH_ERR_HIP = 1,          /* a HIP runtime call failed */
+    UH_ERR_ARG = 2,          /* invalid argument (shape, stride, pointer) */
+    UH_ERR_OOM = 3,          /* device or pinned allocation failed */
+    UH_ERR_NODEV = 4,        /* no HIP device */
+    UH_ERR_BATCH = 5         /* operation not supported on a batch handle */
+};
+
+const char* uh_last_error(void);
+
+/* ---- device (reference pmCudaInterface.cu:227-334: probe, bind, memory queries) ---- */
+int uh_device_count(int* count);
+int uh_set_device(int device);
+int uh_device_props(int device, char* name, size_t name_len, size_t* total_mem, int* cu_count,
+                    int* clock_khz, int* l2_bytes, int* lds_per_block);
+int uh_mem_info(size_t* free_bytes, size_t* total_bytes);
+int uh_device_sync(void);
+
+/* ---- memory ---- */
+int uh_malloc(void** ptr, size_t bytes);
+int uh_free(void* ptr);
+int uh_host_alloc(void** ptr, size_t bytes);          /* pinned host memory (hipHostMalloc) */
+int uh_host_free(void* ptr);
+int uh_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int uh_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int uh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Strided 2-D copy; kind 0 = H2D, 1 = D2H, 2 = D2D. */
+int uh_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+                 size_t rows, int kind, void* stream);
+/* Fill n 32-bit words with a value using a kernel (never a memset node: graph-safe). */
+int uh_fill_u32(void* dst, uint32_t value, size_t n_words, void* stream);
+
+/* ---- streams, events, graphs ---- */
+int uh_stream_create(void** stream);
+int uh_stream_destroy(void* stream);
+int uh_stream_sync(void* stream);
+int uh_stream_query(void* stream);                    /* 0 = idle, 1 = busy */
+int uh_stream_wait_event(void* stream, void* event);
+int uh_event_create(void** event);
+int uh_event_destroy(void* event);
+int uh_event_record(void* event, void* stream);
+int uh_event_sync(void* event);
+int uh_event_query(void* event);                      /* 0 = complete, 1 = pending */
+int uh_event_elapsed_ms(float* ms, void* start, void* stop);
+int uh_graph_begin_capture(void* stream);
+int uh_graph_end_capture(void* stream, void** graph_exec);
+int uh_graph_launch(void* graph_exec, void* stream);
+int uh_graph_destroy(void* graph_exec);
+
+/* Launch a __global__ kernel of the pm kernel-kind callback type
+ * (reference pmCudaInterface.cu:532-543). args = array of pointers to each argument. */
+int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, int bz,
+                     size_t shared_mem, void* stream, void** args);
+
+/* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
+ *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
+ * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to
+ * dst + dst_off[i]. The three arrays are device pointers to uint64. */
+int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const uint64_t* dst_off,
+                    const uint64_t* len, uint32_t n_records, uint64_t max_len, void* stream);
+
+/* ---- built-in reduction: dst[i] += src[i] (REDUCE_ADD over floats; reference
+ *      pmReducer.cpp:593-605) ---- */
+int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream);
+
+/* ---- imageFiltering (reference imageFilteringCUDA.cu:86-342) ----
+ * One subtask: output tile rows [row0, row0+rows) x cols [col0, col0+cols) of an H x W RGB
+ * image. The input view holds the halo'd region in inverted (bottom-up) row order: view row v
+ * is image row (view_last_row - v), view column u is image column (view_first_col + u).
+ * out(i,j,c) = sum_{dy,dx in [-r,r]} w[dy+r][dx+r] * in(clamp(i+dy), clamp(j+dx), 2-c)
+ * mode 0 (REF): accumulated modulo 256, as the reference's char accumulators.
+ * mode 1 (CFG): int32 sum, arithmetic right shift by `shift`, clamped to [0,255].
+ * weights: (2r+1)^2 int8 values, row-major, read from host memory at call time. */
+typedef struct uh_imagefilter_args {
+    const uint8_t* in;  uint64_t in_stride;     /* bytes between view rows */
+    uint8_t* out;       uint64_t out_stride;    /* bytes between output rows */
+    int32_t row0, col0, rows, cols;             /* output tile in image coordinates */
+    int32_t view_last_row, view_first_col;      /* input view origin in image coordinates */
+    int32_t height, width;                      /* image size */
+    int32_t radius, mode, shift, pad;
+    int8_t weights[32 * 32];                    /* row-major (2r+1)x(2r+1), r <= 15 */
+} uh_imagefilter_args;
+int uh_imagefilter(const uh_imagefilter_args* args, void* stream);
+
+/* ---- pageRank pull SpMV over an HBM-resident in-link CSR (replaces
+ *      pageRankCUDA.cu:46-164: push with float atomics over a padded web dump) ----
+ * For destination pages t in [row0, row0+rows):
+ *   y[t] = sum_{e in [row_ptr[t], row_ptr[t+1])} w[col[e]]
+ *   w_next[t] = (deg[t] > 0) ? (damping * y[t]) / deg[t] : 0
+ * where w[src] = (damping * x[src]) / deg[src] was produced by the previous iteration
+ * (the reference's increment, pageRank.cpp:262,367). y and w_next are indexed by global page id. */
+typedef struct uh_pagerank_args {
+    const uint32_t* row_ptr; const uint32_t* col; const uint32_t* deg;
+    const float* w; float* y; float* w_next;
+    uint32_t row0, rows;
+    float damping; uint32_t pad;
+} uh_pagerank_args;
+int uh_pagerank_pull(const uh_pagerank_args* args, void* stream);
+/* w[i] = (damping * x0) / deg[i] (or 0 for dangling pages) for the first iteration. */
+int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, float x0, float damping,
+                       void* stream);
+/* Push formulation, as the reference callback: y[t-1] += (damping*x[src])/deg over a padded
+ * record slice (count, t1..t_count, 0-pad to max_out) for pages [0, pages); float atomics. */
+int uh_pagerank_push_ell(const uint32_t* records, uint32_t pages, uint32_t max_out,
+                         const float* x, float x0, int first_iteration, float damping,
+                         float* y, void* stream);
+
+/* ---- dense GEMM, row-major: C = alpha * A(MxK) * B(KxN) + beta * C ----
+ * precision 0 = split-BF16 3-term MFMA (hi*hi + hi*lo + lo*hi, fp32 accumulate),
+ * precision 1 = FP32 MFMA (v_mfma_f32_*_f32), precision 2 = FP32 VALU reference kernel.
+ * Column-major callers pass the transposed problem (C^T = B^T A^T).
+ * Replaces cublasSgemm in matrixMultiplyBlasCUDA.cu:54-77 and luDecompositionCUDA.cu:190. */
+typedef struct uh_gemm_args {
+    const float* A; const float* B; float* C;
+    uint64_t lda, ldb, ldc;
+    int32_t M, N, K, precision;
+    float alpha, beta;
+} uh_gemm_args;
+int uh_gemm(const uh_gemm_args* args, void* stream);
+/* Tile configuration chosen by the autotuner for a shape class (see uh_gemm_tune). */
+int uh_gemm_set_tile(int M, int N, int K, int precision, int tile_id);
+int uh_gemm_tile_count(int* n);
+
+/* ---- LU (column-major, no pivoting, reference convention: L lower with the pivots on its
+ *      diagonal, U unit upper; luDecomposition.cpp:116-125) ---- */
+/* Factor one n x n diagonal block in place (reference luDecompositionCUDA.cu:66-97). */
+int uh_lu_panel(float* a, uint64_t lda, int n, void* stream);
+/* X := L^-1 X, L lower non-unit (n x n), X n x m (reference luDecompositionCUDA.cu:127). */
+int uh_trsm_lln(const float* l, uint64_t ldl, float* x, uint64_t ldx, int n, int m, void* stream);
+/* X := X U^-1, U upper unit (n x n), X m x n (reference luDecompositionCUDA.cu:147). */
+int uh_trsm_run(const float* u, uint64_t ldu, float* x, uint64_t ldx, int m, int n, void* stream);
+
+/* ---- iterative refinement helpers (FP64 residual, FP32 triangular solves) ---- */
+/* r = b - A x in FP64; A column-major n x n (fp32), x (fp64) and b (fp64) n x nrhs
+ * column-major; r written as fp64 and also as fp32 (rf) for the correction solve. */
+int uh_residual_f64(const float* a, uint64_t lda, int n, const double* x, const double* b,
+                    int nrhs, double* r, float* rf, void* stream);
+/* Solve (L U) z = rhs in place for nrhs columns (fp32), with the factors of uh_lu_panel. */
+int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, int nrhs, void* stream);
+/* x += z (z fp32, x fp64), and norms: out[0..nrhs) = max|r|, out[nrhs..2nrhs) = max|x|. */
+int uh_refine_update(double* x, const float* z, int n, int nrhs, void* stream);
+int uh_colmax_abs_f64(const double* v, int n, int ncols, double* out, void* stream);
+
+/* ---- batching: device work queue + persistent kernels ---- */
+int uh_batch_create(void** batch_handle, void* stream);
+int uh_batch_destroy(void* batch_handle);
+int uh_is_batch(void* stream);
+/* Number of work items recorded since the last flush. */
+int uh_batch_pending(void* batch_handle, int* n);
+/* Upload the recorded work items and launch one persistent kernel per kernel family on the
+ * batch's stream. With use_graph != 0 the upload+launch sequence is captured once per batch
+ * shape and replayed as a hipGraph. */
+int uh_batch_flush(void* batch_handle, int use_graph);
+/* Launch counters (kernels launched through this library since load). */
+int uh_launch_counters(uint64_t* plain_launches, uint64_t* persistent_launches, uint64_t* items);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,185 @@
+"""unicorn_amd — MI355X-native device subtask path of the Unicorn (pm::) task library.
+
+The product is native: the pm:: runtime (libpmlib.so), the gfx950 HIP kernels behind a C-ABI
+(libunicorn_hip.so) and the four benchmark apps with C-ABI drivers (libunicorn_apps.so).
+This module only loads those libraries with ctypes and offers thin Python handles over the app
+drivers for tests and benchmarks. There is no Python or CPU fallback for any GPU path: if a
+library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+PM_HIP_MODE_PLAIN = 0
+PM_HIP_MODE_RESIDENT = 1
+PM_HIP_MODE_BATCHED = 2
+DEVICE_CPU, DEVICE_GPU, DEVICE_BOTH = 0, 1, 2
+
+_libs: dict[str, C.CDLL] = {}
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def _load(name: str) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = LIB_DIR / f"lib{name}.so"
+    if not path.exists():
+        raise NativeLibraryMissing(f"{path} is not built; run `python -m unicorn_amd.build`")
+    lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+    _libs[name] = lib
+    return lib
+
+
+def hip_lib() -> C.CDLL:
+    return _load("unicorn_hip")
+
+
+def pm_lib() -> C.CDLL:
+    hip_lib()
+    return _load("pmlib")
+
+
+def apps_lib() -> C.CDLL:
+    pm_lib()
+    lib = _load("unicorn_apps")
+    _declare_apps(lib)
+    return lib
+
+
+class ua_stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "subtasks", "batches", "h2d_bytes", "d2h_bytes", "d2d_bytes", "cache_hits", "evictions",
+        "plain_launches", "persistent_launches", "queue_items")]
+
+
+class ua_if_config(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("radius", C.c_int32),
+                ("mode", C.c_int32), ("shift", C.c_int32), ("tile", C.c_int32),
+                ("weights", C.c_int8 * 1024)]
+
+
+_declared = False
+
+
+def _declare_apps(lib: C.CDLL) -> None:
+    global _declared
+    if _declared:
+        return
+    vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "ua_init": (i32, [i32]), "ua_finalize": (i32, []), "ua_has_gpu": (i32, []),
+        "ua_set_hip_mode": (i32, [i32]), "ua_set_hip_compact": (i32, [i32]),
+        "ua_set_hip_graphs": (i32, [i32]), "ua_set_hip_depth": (i32, [i32]),
+        "ua_set_batch_limit": (i32, [i32]), "ua_set_hbm_budget": (i32, [C.c_uint64]),
+        "ua_get_stats": (i32, [C.POINTER(ua_stats)]), "ua_reset_stats": (i32, []),
+        "ua_last_error": (C.c_char_p, []),
+        "ua_if_create": (vp, [C.POINTER(ua_if_config)]), "ua_if_set_input": (i32, [vp, vp]),
+        "ua_if_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_if_get_output": (i32, [vp, vp]),
+        "ua_if_destroy": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _declared = True
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = apps_lib().ua_last_error()
+        raise RuntimeError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")
+
+
+_runtime_up = False
+
+
+def init(cpu_stubs: int = 0) -> None:
+    """Start the pm:: runtime (idempotent)."""
+    global _runtime_up
+    if _runtime_up:
+        return
+    _check(apps_lib().ua_init(cpu_stubs), "pmInitialize")
+    _runtime_up = True
+
+
+def has_gpu() -> bool:
+    init()
+    return bool(apps_lib().ua_has_gpu())
+
+
+def set_hip_mode(mode: int, compact: bool = False, graphs: bool = True) -> None:
+    init()
+    lib = apps_lib()
+    _check(lib.ua_set_hip_mode(mode), "pmSetHipStubMode")
+    _check(lib.ua_set_hip_compact(1 if compact else 0), "pmSetHipOptimalView")
+    _check(lib.ua_set_hip_graphs(1 if graphs else 0), "pmSetHipGraphs")
+
+
+def stats() -> dict:
+    s = ua_stats()
+    _check(apps_lib().ua_get_stats(C.byref(s)), "stats")
+    return {n: int(getattr(s, n)) for n, _ in ua_stats._fields_}
+
+
+def reset_stats() -> None:
+    _check(apps_lib().ua_reset_stats(), "reset stats")
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class ImageFilter:
+    """imageFiltering task over an H x W RGB image (rows stored bottom-up, as the reference)."""
+
+    def __init__(self, width: int, height: int, weights: np.ndarray, radius: int, mode: int = 0,
+                 shift: int = 4, tile: int = 2048):
+        init()
+        cfg = ua_if_config()
+        cfg.width, cfg.height, cfg.radius, cfg.mode, cfg.shift, cfg.tile = width, height, radius, mode, shift, tile
+        w = np.asarray(weights, dtype=np.int8).reshape(-1)
+        if w.size != (2 * radius + 1) ** 2:
+            raise ValueError("weights must hold (2r+1)^2 values")
+        for i, v in enumerate(w):
+            cfg.weights[i] = int(v)
+        self.shape = (height, width, 3)
+        self._h = apps_lib().ua_if_create(C.byref(cfg))
+        if not self._h:
+            _check(-1, "ua_if_create")
+
+    def set_input(self, image: np.ndarray) -> None:
+        image = np.ascontiguousarray(image, dtype=np.uint8)
+        if image.shape != self.shape:
+            raise ValueError(f"image shape {image.shape} != {self.shape}")
+        _check(apps_lib().ua_if_set_input(self._h, _ptr(image)), "ua_if_set_input")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_if_run(self._h, device, C.byref(t)), "imageFilter task")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint8)
+        _check(apps_lib().ua_if_get_output(self._h, _ptr(out)), "ua_if_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_if_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,74 @@
+// Process-level C-ABI of the app drivers: runtime start/stop, HIP stub controls, counters.
+#include "app_common.h"
+#include "unicorn_hip.h"
+
+#include <cstdlib>
+
+namespace apps {
+
+static std::mutex g_err_mtx;
+static std::string g_err;
+
+void set_error(const std::string& s) {
+    std::lock_guard<std::mutex> g(g_err_mtx);
+    g_err = s;
+}
+
+}  // namespace apps
+
+using namespace pm;
+
+extern "C" {
+
+int ua_init(int cpu_stubs) {
+    if (cpu_stubs > 0) pmSetCpuStubCount(cpu_stubs);
+    pmStatus s = pmInitialize();
+    if (s != pmSuccess) apps::set_error(pmGetLastError());
+    return (int)s;
+}
+
+int ua_finalize(void) { return (int)pmFinalize(); }
+
+int ua_has_gpu(void) { return pmHasHipDevice() ? 1 : 0; }
+
+int ua_set_hip_mode(int mode) { return (int)pmSetHipStubMode(mode); }
+
+int ua_set_hip_compact(int compact) {
+    return (int)pmSetHipOptimalView(compact ? SUBSCRIPTION_COMPACT : SUBSCRIPTION_NATURAL);
+}
+
+int ua_set_hip_graphs(int on) { return (int)pmSetHipGraphs(on != 0); }
+int ua_set_hip_depth(int depth) { return (int)pmSetHipPipelineDepth(depth); }
+int ua_set_batch_limit(int n) { return (int)pmSetHipBatchLimit(n); }
+int ua_set_hbm_budget(uint64_t bytes) { return (int)pmSetHbmBudget((size_t)bytes); }
+
+int ua_get_stats(ua_stats* s) {
+    pmHipStubStats h;
+    pmStatus st = pmGetHipStubStats(&h);
+    if (st != pmSuccess) return (int)st;
+    s->subtasks = h.subtasks;
+    s->batches = h.batches;
+    s->h2d_bytes = h.h2d_bytes;
+    s->d2h_bytes = h.d2h_bytes;
+    s->d2d_bytes = h.d2d_bytes;
+    s->cache_hits = h.cache_hits;
+    s->evictions = h.evictions;
+    s->plain_launches = s->persistent_launches = s->queue_items = 0;
+    if (pmHasHipDevice()) uh_launch_counters(&s->plain_launches, &s->persistent_launches, &s->queue_items);
+    return 0;
+}
+
+int ua_reset_stats(void) { return (int)pmResetHipStubStats(); }
+
+const char* ua_last_error(void) {
+    static thread_local std::string copy;
+    {
+        std::lock_guard<std::mutex> g(apps::g_err_mtx);
+        copy = apps::g_err;
+    }
+    const char* pe = pmGetLastError();
+    if (pe && *pe) copy += (copy.empty() ? "" : " | ") + std::string(pe);
+    return copy.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,454 @@
+// Device management, memory, streams/events/graphs, staging kernels and the batch work queue
+// of libunicorn_hip.so. Replaces the reference's CUDA runtime wrapper
+// (source/code/src/pmCudaInterface.cu:69-378) for one MI355X.
+#include "uh_internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <mutex>
+
+namespace uh {
+
+static thread_local char t_err[512] = "";
+std::atomic<uint64_t> g_plain_launches{0};
+std::atomic<uint64_t> g_persistent_launches{0};
+std::atomic<uint64_t> g_items{0};
+
+int fail(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_err, sizeof(t_err), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int check_hip(hipError_t err, const char* what) {
+    if (err == hipErrorOutOfMemory || err == hipErrorMemoryAllocation)
+        return fail(UH_ERR_OOM, "%s: %s", what, hipGetErrorString(err));
+    return fail(UH_ERR_HIP, "%s: %s", what, hipGetErrorString(err));
+}
+
+int cu_count() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 256;
+        cus = p.multiProcessorCount;
+    }
+    return cus;
+}
+
+Batch* as_batch(void* stream) {
+    uintptr_t v = (uintptr_t)stream;
+    if (v & 1u) return (Batch*)(v & ~(uintptr_t)1);
+    return nullptr;
+}
+
+void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles) {
+    const uint8_t* p = (const uint8_t*)desc;
+    b->items[f].insert(b->items[f].end(), p, p + bytes);
+    b->tiles[f].push_back(tiles);
+}
+
+// ---- staging kernels ----
+
+// One workgroup per record; 16-byte vector copies when both ends are aligned.
+__global__ void __launch_bounds__(256) k_copy_records(uint8_t* dst, const uint8_t* src,
+                                                       const uint64_t* src_off,
+                                                       const uint64_t* dst_off,
+                                                       const uint64_t* len) {
+    const uint32_t r = blockIdx.x;
+    const uint8_t* s = src + src_off[r];
+    uint8_t* d = dst + dst_off[r];
+    const uint64_t n = len[r];
+    const uint64_t chunk = (uint64_t)blockIdx.y * 65536u;
+    if (chunk >= n) return;
+    const uint64_t end = (chunk + 65536u < n) ? chunk + 65536u : n;
+    if ((((uintptr_t)s | (uintptr_t)d | chunk) & 15u) == 0) {
+        const uint64_t nv = (end - chunk) >> 4;
+        const uint4* sv = (const uint4*)(s + chunk);
+        uint4* dv = (uint4*)(d + chunk);
+        for (uint64_t i = threadIdx.x; i < nv; i += blockDim.x) dv[i] = sv[i];
+        for (uint64_t i = chunk + (nv << 4) + threadIdx.x; i < end; i += blockDim.x) d[i] = s[i];
+    } else {
+        for (uint64_t i = chunk + threadIdx.x; i < end; i += blockDim.x) d[i] = s[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* dst, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_add_f32(float* dst, const float* src, size_t n) {
+    size_t n4 = n >> 2;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    float4* d4 = (float4*)dst;
+    const float4* s4 = (const float4*)src;
+    for (size_t k = i; k < n4; k += stride) {
+        float4 a = d4[k], b = s4[k];
+        a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+        d4[k] = a;
+    }
+    for (size_t k = (n4 << 2) + i; k < n; k += stride) dst[k] += src[k];
+}
+
+}  // namespace uh
+
+using namespace uh;
+
+extern "C" {
+
+const char* uh_last_error(void) { return t_err; }
+
+int uh_device_count(int* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; return fail(UH_ERR_NODEV, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
+    *count = n;
+    return n > 0 ? UH_OK : fail(UH_ERR_NODEV, "no HIP device");
+}
+
+int uh_set_device(int device) { UH_HIP(hipSetDevice(device)); return UH_OK; }
+
+int uh_device_props(int device, char* name, size_t name_len, size_t* total_mem, int* cu,
+                    int* clock_khz, int* l2_bytes, int* lds_per_block) {
+    hipDeviceProp_t p;
+    UH_HIP(hipGetDeviceProperties(&p, device));
+    if (name && name_len) {
+        snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (cu) *cu = p.multiProcessorCount;
+    if (clock_khz) *clock_khz = p.clockRate;
+    if (l2_bytes) *l2_bytes = p.l2CacheSize;
+    if (lds_per_block) *lds_per_block = (int)p.sharedMemPerBlock;
+    return UH_OK;
+}
+
+int uh_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    UH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return UH_OK;
+}
+
+int uh_device_sync(void) { UH_HIP(hipDeviceSynchronize()); return UH_OK; }
+
+int uh_malloc(void** ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (!bytes) return UH_OK;
+    UH_HIP(hipMalloc(ptr, bytes));
+    return UH_OK;
+}
+
+int uh_free(void* ptr) { if (ptr) UH_HIP(hipFree(ptr)); return UH_OK; }
+
+int uh_host_alloc(void** ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (!bytes) return UH_OK;
+    UH_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return UH_OK;
+}
+
+int uh_host_free(void* ptr) { if (ptr) UH_HIP(hipHostFree(ptr)); return UH_OK; }
+
+static int copy_kind(const char* what, void* dst, const void* src, size_t bytes, void* stream,
+                     hipMemcpyKind kind) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "%s on a batch handle", what);
+    if (!bytes) return UH_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, (hipStream_t)stream);
+    if (e != hipSuccess) return check_hip(e, what);
+    return UH_OK;
+}
+
+int uh_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return copy_kind("hipMemcpyAsync(H2D)", dst, src, bytes, stream, hipMemcpyHostToDevice);
+}
+int uh_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    return copy_kind("hipMemcpyAsync(D2H)", dst, src, bytes, stream, hipMemcpyDeviceToHost);
+}
+int uh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return copy_kind("hipMemcpyAsync(D2D)", dst, src, bytes, stream, hipMemcpyDeviceToDevice);
+}
+
+int uh_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                 size_t rows, int kind, void* stream) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_memcpy_2d on a batch handle");
+    if (!width || !rows) return UH_OK;
+    if (width > dpitch || width > spitch) return fail(UH_ERR_ARG, "uh_memcpy_2d: width > pitch");
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    UH_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, k, (hipStream_t)stream));
+    return UH_OK;
+}
+
+int uh_fill_u32(void* dst, uint32_t value, size_t n, void* stream) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_fill_u32 on a batch handle");
+    if (!n) return UH_OK;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (uint32_t*)dst, value, n);
+    UH_LAUNCH_CHECK("k_fill_u32");
+    g_plain_launches++;
+    return UH_OK;
+}
+
+int uh_stream_create(void** stream) {
+    hipStream_t s;
+    UH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return UH_OK;
+}
+int uh_stream_destroy(void* stream) { if (stream) UH_HIP(hipStreamDestroy((hipStream_t)stream)); return UH_OK; }
+int uh_stream_sync(void* stream) {
+    Batch* b = as_batch(stream);
+    UH_HIP(hipStreamSynchronize(b ? b->stream : (hipStream_t)stream));
+    return UH_OK;
+}
+int uh_stream_query(void* stream) {
+    Batch* b = as_batch(stream);
+    hipError_t e = hipStreamQuery(b ? b->stream : (hipStream_t)stream);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return check_hip(e, "hipStreamQuery") ? -1 : -1;
+}
+int uh_stream_wait_event(void* stream, void* event) {
+    Batch* b = as_batch(stream);
+    UH_HIP(hipStreamWaitEvent(b ? b->stream : (hipStream_t)stream, (hipEvent_t)event, 0));
+    return UH_OK;
+}
+int uh_event_create(void** event) {
+    hipEvent_t e;
+    UH_HIP(hipEventCreate(&e));
+    *event = e;
+    return UH_OK;
+}
+int uh_event_destroy(void* event) { if (event) UH_HIP(hipEventDestroy((hipEvent_t)event)); return UH_OK; }
+int uh_event_record(void* event, void* stream) {
+    Batch* b = as_batch(stream);
+    UH_HIP(hipEventRecord((hipEvent_t)event, b ? b->stream : (hipStream_t)stream));
+    return UH_OK;
+}
+int uh_event_sync(void* event) { UH_HIP(hipEventSynchronize((hipEvent_t)event)); return UH_OK; }
+int uh_event_query(void* event) {
+    hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    check_hip(e, "hipEventQuery");
+    return -1;
+}
+int uh_event_elapsed_ms(float* ms, void* start, void* stop) {
+    UH_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return UH_OK;
+}
+
+int uh_graph_begin_capture(void* stream) {
+    UH_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return UH_OK;
+}
+int uh_graph_end_capture(void* stream, void** graph_exec) {
+    hipGraph_t g;
+    UH_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ge;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+    *graph_exec = ge;
+    return UH_OK;
+}
+int uh_graph_launch(void* graph_exec, void* stream) {
+    UH_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return UH_OK;
+}
+int uh_graph_destroy(void* graph_exec) {
+    if (graph_exec) UH_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return UH_OK;
+}
+
+int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, int bz,
+                     size_t shared_mem, void* stream, void** args) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "kernel-kind callbacks cannot be batched");
+    if (gx <= 0 || gy <= 0 || gz <= 0 || bx <= 0 || by <= 0 || bz <= 0 || bx * by * bz > 1024)
+        return fail(UH_ERR_ARG, "uh_launch_kernel: bad geometry");
+    UH_HIP(hipLaunchKernel(func, dim3(gx, gy, gz), dim3(bx, by, bz), args, shared_mem,
+                           (hipStream_t)stream));
+    g_plain_launches++;
+    return UH_OK;
+}
+
+int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const uint64_t* dst_off,
+                    const uint64_t* len, uint32_t n_records, uint64_t max_len, void* stream) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_copy_records on a batch handle");
+    if (!n_records || !max_len) return UH_OK;
+    uint64_t chunks = (max_len + 65535) / 65536;
+    if (chunks > 65535 || n_records > 0x7fffffffu) return fail(UH_ERR_ARG, "uh_copy_records: too large");
+    hipLaunchKernelGGL(k_copy_records, dim3(n_records, (unsigned)chunks), dim3(256), 0,
+                       (hipStream_t)stream, (uint8_t*)dst, (const uint8_t*)src, src_off, dst_off, len);
+    UH_LAUNCH_CHECK("k_copy_records");
+    g_plain_launches++;
+    return UH_OK;
+}
+
+int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream) {
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_reduce_add_f32 on a batch handle");
+    if (!n) return UH_OK;
+    if (((uintptr_t)dst | (uintptr_t)src) & 15u) return fail(UH_ERR_ARG, "uh_reduce_add_f32: unaligned");
+    size_t blocks = (n / 4 + 255) / 256;
+    size_t cap = (size_t)cu_count() * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       dst, src, n);
+    UH_LAUNCH_CHECK("k_reduce_add_f32");
+    g_plain_launches++;
+    return UH_OK;
+}
+
+// ---- batching ----
+
+int uh_batch_create(void** handle, void* stream) {
+    if (as_batch(stream)) return fail(UH_ERR_ARG, "uh_batch_create: stream is a batch");
+    Batch* b = new Batch();
+    b->stream = (hipStream_t)stream;
+    b->slot_bytes = 0;
+    *handle = (void*)((uintptr_t)b | 1u);
+    return UH_OK;
+}
+
+int uh_batch_destroy(void* handle) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_destroy: not a batch");
+    hipStreamSynchronize(b->stream);
+    for (int i = 0; i < Batch::RING; ++i) {
+        for (int m = 0; m < (1 << FAM_COUNT); ++m)
+            if (b->graphs[i][m]) hipGraphExecDestroy(b->graphs[i][m]);
+        if (b->host_slot[i]) hipHostFree(b->host_slot[i]);
+        if (b->dev_slot[i]) hipFree(b->dev_slot[i]);
+        if (b->slot_free[i]) hipEventDestroy(b->slot_free[i]);
+    }
+    delete b;
+    return UH_OK;
+}
+
+int uh_is_batch(void* stream) { return as_batch(stream) ? 1 : 0; }
+
+int uh_batch_pending(void* handle, int* n) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_pending: not a batch");
+    size_t t = 0;
+    for (int f = 0; f < FAM_COUNT; ++f) t += b->tiles[f].size();
+    *n = (int)t;
+    return UH_OK;
+}
+
+static const size_t kItemBytes[FAM_COUNT] = {
+    sizeof(uh_imagefilter_args), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem)};
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Layout of one upload: per family a QueueHeader block at a fixed offset of the slot
+// (family f's block begins at f * family_stride) so that captured graphs can be replayed
+// for any batch contents that fit the slot.
+int uh_batch_flush(void* handle, int use_graph) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_flush: not a batch");
+    size_t need_family[FAM_COUNT];
+    size_t family_stride = 0;
+    int mask = 0;
+    for (int f = 0; f < FAM_COUNT; ++f) {
+        size_t n = b->tiles[f].size();
+        need_family[f] = align_up(sizeof(QueueHeader) + n * kItemBytes[f] + (n + 1) * 4, 256);
+        if (need_family[f] > family_stride) family_stride = need_family[f];
+        if (n) mask |= 1 << f;
+    }
+    if (!mask) return UH_OK;
+    // Slots are sized generously and fixed per batch so graphs stay valid.
+    size_t min_stride = 64 * 1024;
+    if (family_stride < min_stride) family_stride = min_stride;
+    size_t slot = family_stride * FAM_COUNT;
+    if (slot > b->slot_bytes) {
+        hipStreamSynchronize(b->stream);
+        for (int i = 0; i < Batch::RING; ++i) {
+            for (int m = 0; m < (1 << FAM_COUNT); ++m)
+                if (b->graphs[i][m]) { hipGraphExecDestroy(b->graphs[i][m]); b->graphs[i][m] = nullptr; }
+            if (b->host_slot[i]) hipHostFree(b->host_slot[i]);
+            if (b->dev_slot[i]) hipFree(b->dev_slot[i]);
+            b->host_slot[i] = nullptr; b->dev_slot[i] = nullptr;
+            UH_HIP(hipHostMalloc((void**)&b->host_slot[i], slot, hipHostMallocDefault));
+            UH_HIP(hipMalloc((void**)&b->dev_slot[i], slot));
+            if (!b->slot_free[i]) UH_HIP(hipEventCreateWithFlags(&b->slot_free[i], hipEventDisableTiming));
+        }
+        b->slot_bytes = slot;
+    }
+    family_stride = b->slot_bytes / FAM_COUNT;
+    int pos = b->ring_pos;
+    b->ring_pos = (pos + 1) % Batch::RING;
+    UH_HIP(hipEventSynchronize(b->slot_free[pos]));   // previous use of this slot has finished
+    uint8_t* host = b->host_slot[pos];
+    uint8_t* dev = b->dev_slot[pos];
+    for (int f = 0; f < FAM_COUNT; ++f) {
+        uint8_t* blk = host + f * family_stride;
+        QueueHeader* h = (QueueHeader*)blk;
+        uint32_t n = (uint32_t)b->tiles[f].size();
+        h->next_tile = 0;
+        h->n_items = n;
+        h->items_offset = (uint32_t)sizeof(QueueHeader);
+        h->prefix_offset = (uint32_t)align_up(sizeof(QueueHeader) + n * kItemBytes[f], 4);
+        h->done = 0;
+        if (n) memcpy(blk + h->items_offset, b->items[f].data(), n * kItemBytes[f]);
+        uint32_t* prefix = (uint32_t*)(blk + h->prefix_offset);
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i < n; ++i) { prefix[i] = acc; acc += b->tiles[f][i]; }
+        prefix[n] = acc;
+        h->total_tiles = acc;
+        g_items += n;
+    }
+    auto enqueue = [&](hipStream_t s) -> int {
+        // Only the bytes in use are copied unless a graph is being captured (graphs copy the
+        // whole slot so one capture serves every batch shape).
+        size_t bytes = use_graph ? b->slot_bytes : (size_t)(FAM_COUNT - 1) * family_stride + need_family[FAM_COUNT - 1];
+        UH_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+        int rc = UH_OK;
+        if ((mask >> FAM_FILTER) & 1) rc |= filter_persistent(dev + FAM_FILTER * family_stride, s);
+        if ((mask >> FAM_GEMM) & 1) rc |= gemm_persistent(dev + FAM_GEMM * family_stride, s);
+        if ((mask >> FAM_PAGERANK) & 1) rc |= pagerank_persistent(dev + FAM_PAGERANK * family_stride, s);
+        if ((mask >> FAM_TRSM) & 1) rc |= trsm_persistent(dev + FAM_TRSM * family_stride, s);
+        return rc;
+    };
+    int rc;
+    if (use_graph) {
+        hipGraphExec_t& ge = b->graphs[pos][mask];
+        if (!ge) {
+            UH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(b->stream);
+            hipGraph_t g;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+        }
+        UH_HIP(hipGraphLaunch(ge, b->stream));
+        rc = UH_OK;
+    } else {
+        rc = enqueue(b->stream);
+    }
+    UH_HIP(hipEventRecord(b->slot_free[pos], b->stream));
+    for (int f = 0; f < FAM_COUNT; ++f) { b->items[f].clear(); b->tiles[f].clear(); }
+    return rc;
+}
+
+int uh_launch_counters(uint64_t* plain, uint64_t* persistent, uint64_t* items) {
+    if (plain) *plain = g_plain_launches.load();
+    if (persistent) *persistent = g_persistent_launches.load();
+    if (items) *items = g_items.load();
+    return UH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// pageRank kernels for gfx950 (replace pageRank_cuda / zeroInit,
+// testSuite/pageRank/source/code/src/pageRankCUDA.cu:46-164).
+//
+// The reference pushes contributions with float atomics from every source page into a
+// full-length output vector per subtask, over a padded record format copied from pageable host
+// memory each iteration. Here the graph is resident in HBM as an in-link CSR (rows = destination
+// pages, sources ascending) and each destination row is reduced in a fixed order by a group of
+// PR_G lanes (strided partial sums, then a butterfly over the group): no atomics, deterministic,
+// one pass over the edges per iteration. The epilogue also produces the next iteration's
+// per-source increment w = (damping * y) / deg, so the division of the reference's
+// pageRank.cpp:262,367 happens once per page instead of once per edge.
+#include "uh_internal.h"
+
+namespace uh {
+
+static constexpr int PR_THREADS = 256;
+static constexpr int PR_G = 4;                           // lanes per destination row
+static constexpr int PR_ROWS_PER_TILE = PR_THREADS / PR_G;
+
+__device__ __forceinline__ void pagerank_rows(const uh_pagerank_args& a, uint32_t tile) {
+    const uint32_t lrow = threadIdx.x / PR_G;
+    const uint32_t sub = threadIdx.x % PR_G;
+    const uint32_t r = tile * PR_ROWS_PER_TILE + lrow;
+    const bool active = r < a.rows;
+    const uint32_t t = a.row0 + r;
+    float s = 0.f;
+    if (active) {
+        const uint32_t b = a.row_ptr[t], e = a.row_ptr[t + 1];
+        for (uint32_t k = b + sub; k < e; k += PR_G) s += a.w[a.col[k]];
+    }
+    // butterfly over the PR_G lanes of the row (fixed order -> deterministic)
+#pragma unroll
+    for (int off = PR_G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PR_G);
+    if (active && sub == 0) {
+        a.y[t] = s;
+        const uint32_t d = a.deg[t];
+        a.w_next[t] = d ? (a.damping * s) / (float)d : 0.f;
+    }
+}
+
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_pull(uh_pagerank_args a) {
+    pagerank_rows(a, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_persistent(const uint8_t* queue) {
+    __shared__ uint32_t s_tile;
+    QueueHeader* h = (QueueHeader*)queue;
+    const uh_pagerank_args* items = (const uh_pagerank_args*)(queue + h->items_offset);
+    const uint32_t* prefix = (const uint32_t*)(queue + h->prefix_offset);
+    const uint32_t total = h->total_tiles, n_items = h->n_items;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&h->next_tile, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        __syncthreads();
+        if (t >= total) break;
+        uint32_t item, local;
+        queue_locate(prefix, n_items, t, item, local);
+        pagerank_rows(items[item], local);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n,
+                                                         float x0, float damping) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) {
+        uint32_t d = deg[i];
+        w[i] = d ? (damping * x0) / (float)d : 0.f;
+    }
+}
+
+// Reference-style push over padded records (count, t_1..t_count, 0-padding to max_out); targets
+// are 1-based page ids (pageRankCUDA.cu:46-65). Float atomics: summation order is not fixed.
+__global__ void __launch_bounds__(256) k_pagerank_push_ell(const uint32_t* rec, uint32_t pages,
+                                                           uint32_t max_out, const float* x, float x0,
+                                                           int first, float damping, float* y) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= pages) return;
+    const uint32_t* r = rec + (uint64_t)p * (max_out + 1);
+    const uint32_t n = r[0];
+    const float incr = (damping * (first ? x0 : x[p])) / (float)n;
+    for (uint32_t k = 0; k < n; ++k) atomicAdd(&y[r[1 + k] - 1], incr);
+}
+
+uint32_t pagerank_tiles(const uh_pagerank_args& a) {
+    return (a.rows + PR_ROWS_PER_TILE - 1) / PR_ROWS_PER_TILE;
+}
+
+int pagerank_persistent(const uint8_t* queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_pagerank_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
+    UH_LAUNCH_CHECK("k_pagerank_persistent");
+    g_persistent_launches++;
+    return UH_OK;
+}
+
+}  // namespace uh
+
+using namespace uh;
+
+extern "C" int uh_pagerank_pull(const uh_pagerank_args* a, void* stream) {
+    if (!a || !a->row_ptr || !a->w || !a->y || !a->w_next || !a->deg)
+        return fail(UH_ERR_ARG, "uh_pagerank_pull: null pointer");
+    if (a->rows == 0) return UH_OK;
+    if (!a->col) return fail(UH_ERR_ARG, "uh_pagerank_pull: null col");
+    const uint32_t tiles = pagerank_tiles(*a);
+    if (Batch* b = as_batch(stream)) {
+        record_item(b, FAM_PAGERANK, a, sizeof(*a), tiles);
+        return UH_OK;
+    }
+    hipLaunchKernelGGL(k_pagerank_pull, dim3(tiles), dim3(PR_THREADS), 0, (hipStream_t)stream, *a);
+    UH_LAUNCH_CHECK("k_pagerank_pull");
+    g_plain_launches++;
+    return UH_OK;
+}
+
+extern "C" int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, float x0, float damping,
+                                  void* stream) {
+    if (!n) return UH_OK;
+    if (!w || !deg) return fail(UH_ERR_ARG, "uh_pagerank_init_w: null pointer");
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_pagerank_init_w on a batch handle");
+    hipLaunchKernelGGL(k_pagerank_init_w, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       w, deg, n, x0, damping);
+    UH_LAUNCH_CHECK("k_pagerank_init_w");
+    g_plain_launches++;
+    return UH_OK;
+}
+
+extern "C" int uh_pagerank_push_ell(const uint32_t* records, uint32_t pages, uint32_t max_out,
+                                    const float* x, float x0, int first_iteration, float damping,
+                                    float* y, void* stream) {
+    if (!pages) return UH_OK;
+    if (!records || !y || (!first_iteration && !x)) return fail(UH_ERR_ARG, "uh_pagerank_push_ell: null pointer");
+    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_pagerank_push_ell on a batch handle");
+    hipLaunchKernelGGL(k_pagerank_push_ell, dim3((pages + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       records, pages, max_out, x, x0, first_iteration, damping, y);
+    UH_LAUNCH_CHECK("k_pagerank_push_ell");
+    g_plain_launches++;
+    return UH_OK;
+}

@@ -1,0 +1,103 @@
+// Internal declarations shared by the HIP sources of libunicorn_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <atomic>
+#include <mutex>
+
+#include "unicorn_hip.h"
+
+namespace uh {
+
+// Records the failure text for uh_last_error() and returns the status.
+int fail(int status, const char* fmt, ...);
+int check_hip(hipError_t err, const char* what);
+
+#define UH_HIP(call)                                            \
+    do {                                                        \
+        hipError_t uh_e_ = (call);                              \
+        if (uh_e_ != hipSuccess) return uh::check_hip(uh_e_, #call); \
+    } while (0)
+
+#define UH_LAUNCH_CHECK(name)                                   \
+    do {                                                        \
+        hipError_t uh_e_ = hipGetLastError();                   \
+        if (uh_e_ != hipSuccess) return uh::check_hip(uh_e_, name); \
+    } while (0)
+
+extern std::atomic<uint64_t> g_plain_launches;
+extern std::atomic<uint64_t> g_persistent_launches;
+extern std::atomic<uint64_t> g_items;
+
+int cu_count();
+
+// Kernel families that a batch can hold. Each family records fixed-size descriptors.
+enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_COUNT = 4 };
+
+// Work item of the TRSM family (both triangular kinds of the LU stage 2).
+struct TrsmItem {
+    const float* tri; float* x;
+    uint64_t ldt, ldx;
+    int32_t n, m, kind, pad;     // kind 0 = L^-1 X (lower non-unit), 1 = X U^-1 (upper unit)
+};
+
+// Device work-queue block of one family, as uploaded for one flush:
+//   QueueHeader | item descriptors (n_items * item_size) | tile prefix (n_items + 1 uint32)
+struct QueueHeader {
+    uint32_t next_tile;       // work-stealing counter, zeroed by the upload itself
+    uint32_t n_items;
+    uint32_t total_tiles;
+    uint32_t items_offset;    // bytes from the header start
+    uint32_t prefix_offset;   // bytes from the header start
+    uint32_t done;            // workgroups finished (for the last-one-out reset)
+    uint32_t pad[2];
+};
+
+struct Batch {
+    hipStream_t stream = nullptr;
+    std::vector<uint8_t> items[FAM_COUNT];     // raw descriptors per family
+    std::vector<uint32_t> tiles[FAM_COUNT];    // tile count of each recorded item
+    // upload ring: pinned host slot + device slot per ring entry
+    static const int RING = 4;
+    uint8_t* host_slot[RING] = {};
+    uint8_t* dev_slot[RING] = {};
+    hipEvent_t slot_free[RING] = {};
+    size_t slot_bytes = 0;
+    int ring_pos = 0;
+    // graphs captured per (ring slot, family mask)
+    hipGraphExec_t graphs[RING][1 << FAM_COUNT] = {};
+};
+
+Batch* as_batch(void* stream);   // nullptr if the stream is not a batch handle
+void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles);
+
+// Per-family persistent launchers used by uh_batch_flush (queue = device pointer to the
+// family's QueueHeader).
+int filter_persistent(const uint8_t* queue, hipStream_t s);
+int gemm_persistent(const uint8_t* queue, hipStream_t s);
+int pagerank_persistent(const uint8_t* queue, hipStream_t s);
+int trsm_persistent(const uint8_t* queue, hipStream_t s);
+
+// Tile counts of one item of each family (host side, for the queue prefix).
+uint32_t filter_tiles(const uh_imagefilter_args& a);
+uint32_t gemm_tiles(const uh_gemm_args& a);
+uint32_t pagerank_tiles(const uh_pagerank_args& a);
+uint32_t trsm_tiles(const TrsmItem& a);
+
+// Device helper: map a global tile index of a work queue to (item, local tile) by a binary
+// search over the prefix array held in global memory.
+__device__ inline void queue_locate(const uint32_t* prefix, uint32_t n_items, uint32_t tile,
+                                    uint32_t& item, uint32_t& local) {
+    uint32_t lo = 0, hi = n_items;           // invariant: prefix[lo] <= tile < prefix[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= tile) lo = mid; else hi = mid;
+    }
+    item = lo;
+    local = tile - prefix[lo];
+}
+
+}  // namespace uh

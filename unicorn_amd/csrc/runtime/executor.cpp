@@ -1,0 +1,1032 @@
+// Tasks, per-device queues and the execution stubs of the single-node runtime.
+//
+// Reference structure (source/code/src): pmScheduler pushes subtask ranges to a priority queue
+// per stub (pmExecutionStub.cpp:142-242, pmSafePriorityQueue.cpp), CPU stubs run subtask_cpu
+// (pmExecutionStub.cpp:2638-2780), and the CUDA stub runs the per-subtask staging pipeline of
+// pmExecutionStub.cpp:2167-2470 / 3216-4070. Here:
+//   - every stub pulls subtask ids from the task at the head of one shared priority order
+//     (priority, then submission order), which is the reference's PULL model without peers to
+//     steal from on one node;
+//   - CPU stubs: one thread per host core, one subtask at a time;
+//   - the HIP stub: one thread for device 0 with three modes (pmExtensions.h): PLAIN follows the
+//     reference stub (pinned gather, H2D into a COMPACT buffer, callback, D2H, scatter; a cache
+//     of read-only buffers keyed by exact subscription), RESIDENT runs callbacks on views of HBM
+//     mirrors, BATCHED additionally replaces per-subtask launches by a device work queue drained
+//     by persistent kernels, and gathers subtasks of all active tasks into each batch.
+#include "pm_internal.h"
+#include "unicorn_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <functional>
+#include <unordered_map>
+
+namespace pm {
+namespace rt {
+
+Runtime* Runtime::inst_ = nullptr;
+
+double now_secs() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+[[noreturn]] void raise(pmStatus s, const std::string& what) { throw Error{s, what}; }
+
+pmTaskInfo Task::info() {
+    pmTaskInfo ti;
+    ti.taskHandle = (pmTaskHandle)this;
+    ti.taskConf = conf.empty() ? nullptr : conf.data();
+    ti.taskConfLength = (uint)conf.size();
+    ti.taskId = details.taskId;
+    ti.subtaskCount = subtask_count;
+    ti.priority = details.priority;
+    ti.originatingHost = 0;
+    return ti;
+}
+
+bool Task::writable(uint i) const {
+    pmMemType m = mem[i].memType;
+    return m == WRITE_ONLY || m == READ_WRITE || m == WRITE_ONLY_LAZY || m == READ_WRITE_LAZY;
+}
+bool Task::read_only(uint i) const { return !writable(i); }
+bool Task::write_only(uint i) const {
+    pmMemType m = mem[i].memType;
+    return m == WRITE_ONLY || m == WRITE_ONLY_LAZY;
+}
+bool Task::is_reduced(uint i) const {
+    return callbacks->cb.dataReduction != nullptr && writable(i);
+}
+pmSubscriptionVisibilityType Task::visibility(uint i, bool gpu) const {
+    pmSubscriptionVisibilityType v = mem[i].subscriptionVisibilityType;
+    if (v == SUBSCRIPTION_OPTIMAL) {
+        if (!gpu) return SUBSCRIPTION_NATURAL;
+        Runtime* r = Runtime::get();
+        if (!gpu_resident) return SUBSCRIPTION_COMPACT;   // staged like the reference stub
+        return (pmSubscriptionVisibilityType)r->hip_view.load();
+    }
+    return v;
+}
+
+Runtime* Runtime::get() { return inst_; }
+
+static int env_int(const char* name, int dflt) {
+    const char* v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+void Runtime::init() {
+    if (inst_) return;
+    Runtime* r = new Runtime();
+    inst_ = r;
+    int n = 0;
+    if (!getenv("PMLIB_DISABLE_HIP") && uh_device_count(&n) == 0 && n > 0) {
+        r->gpu_count = 1;
+        uh_set_device(0);
+        pmDeviceInfo* d = new pmDeviceInfo();
+        d->deviceHandle = d;
+        size_t total = 0;
+        int cus = 0;
+        uh_device_props(0, d->name, MAX_NAME_STR_LEN, &total, &cus, nullptr, nullptr, nullptr);
+        snprintf(d->description, MAX_DESC_STR_LEN, "HIP device 0, %d CUs, %zu MiB HBM", cus, total >> 20);
+        d->deviceType = GPU_HIP;
+        d->deviceIdOnHost = 0;
+        d->deviceIdInCluster = 0;
+        r->gpu_device = d;
+        size_t freeb = 0, totb = 0;
+        uh_mem_info(&freeb, &totb);
+        size_t budget = (size_t)((double)freeb * 0.85);
+        int mb = env_int("PMLIB_HBM_BUDGET_MB", 0);
+        if (mb > 0) budget = (size_t)mb << 20;
+        Memory::get().set_budget(budget);
+        for (int i = 0; i < 16; ++i) {
+            uh_stream_create(&r->gpu_streams[i]);
+            uh_event_create(&r->gpu_events[i]);
+        }
+        uh_stream_create(&r->batch_stream);
+        uh_stream_create(&r->copy_stream);
+        uh_batch_create(&r->batch, r->batch_stream);
+    }
+    r->hip_mode = env_int("PMLIB_HIP_MODE", r->hip_mode.load());
+    r->use_graphs = env_int("PMLIB_HIP_GRAPHS", 1);
+    int ncpu = r->cpu_workers.load();
+    if (!ncpu) ncpu = env_int("PMLIB_MAX_CPU_PER_HOST", (int)std::thread::hardware_concurrency());
+    if (ncpu <= 0) ncpu = 1;
+    for (int i = 0; i < ncpu; ++i) {
+        pmDeviceInfo* d = new pmDeviceInfo();
+        d->deviceHandle = d;
+        snprintf(d->name, MAX_NAME_STR_LEN, "CPU core %d", i);
+        snprintf(d->description, MAX_DESC_STR_LEN, "host CPU stub %d", i);
+        d->deviceType = CPU;
+        d->deviceIdOnHost = (uint)(i + r->gpu_count);
+        d->deviceIdInCluster = d->deviceIdOnHost;
+        r->cpu_devices.push_back(d);
+    }
+    for (int i = 0; i < ncpu; ++i) r->cpu_threads.emplace_back(&Runtime::cpu_loop, r, i);
+    if (r->gpu_count) r->gpu_thread = std::thread(&Runtime::gpu_loop, r);
+}
+
+void Runtime::finalize() {
+    Runtime* r = inst_;
+    if (!r) return;
+    {
+        std::lock_guard<std::mutex> g(r->q_mtx);
+        r->stopping = true;
+    }
+    r->q_cv.notify_all();
+    for (auto& t : r->cpu_threads) t.join();
+    if (r->gpu_thread.joinable()) r->gpu_thread.join();
+    if (r->gpu_count) {
+        uh_device_sync();
+        uh_batch_destroy(r->batch);
+        for (int i = 0; i < 16; ++i) {
+            uh_stream_destroy(r->gpu_streams[i]);
+            uh_event_destroy(r->gpu_events[i]);
+        }
+        uh_stream_destroy(r->batch_stream);
+        uh_stream_destroy(r->copy_stream);
+    }
+    for (auto* d : r->cpu_devices) delete d;
+    delete r->gpu_device;
+    inst_ = nullptr;
+    delete r;
+}
+
+void Runtime::submit(Task* t) {
+    t->submit_time = now_secs();
+    {
+        std::lock_guard<std::mutex> g(q_mtx);
+        t->seq = ++seq;
+        if (t->subtask_count == 0) {
+            t->state = TaskState::Done;
+            t->start_time = t->end_time = t->submit_time;
+            return;
+        }
+        // keep `active` ordered by (priority, seq): lower priority value first
+        auto it = std::find_if(active.begin(), active.end(), [&](Task* o) {
+            return o->details.priority > t->details.priority;
+        });
+        active.insert(it, t);
+    }
+    q_cv.notify_all();
+}
+
+void Runtime::wait(Task* t) {
+    std::unique_lock<std::mutex> g(t->mtx);
+    t->cv.wait(g, [&] { return t->state == TaskState::Done; });
+}
+
+Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
+    // caller holds q_mtx
+    double now = 0;
+    for (auto it = active.begin(); it != active.end(); ++it) {
+        Task* t = *it;
+        if (gpu ? !t->gpu_allowed : !t->cpu_allowed) continue;
+        if (t->deadline > 0) {
+            if (now == 0) now = now_secs();
+            if (now > t->deadline) {
+                // time-out: remaining subtasks are cancelled (reference TASK_TIME_OUT,
+                // pmTimedEventManager.cpp:92-95)
+                ulong f = t->next.exchange(t->subtask_count);
+                if (f < t->subtask_count) {
+                    int expected = pmSuccess;
+                    t->status.compare_exchange_strong(expected, (int)pmTimerFailure);
+                    active.erase(it);
+                    finish_subtasks(t, t->subtask_count - f);
+                    return pick(gpu, first, count, max_count);
+                }
+            }
+        }
+        ulong f = t->next.load();
+        if (f >= t->subtask_count) continue;
+        ulong n = std::min(max_count, t->subtask_count - f);
+        t->next = f + n;
+        if (t->next.load() >= t->subtask_count) active.erase(it);
+        first = f;
+        count = n;
+        if (t->state == TaskState::Queued) {
+            t->state = TaskState::Running;
+            t->start_time = now_secs();
+        }
+        return t;
+    }
+    return nullptr;
+}
+
+void Runtime::finish_subtasks(Task* t, ulong n) {
+    ulong d = t->done.fetch_add(n) + n;
+    if (d == t->subtask_count) complete(t);
+}
+
+// ---------------------------------------------------------------- reductions
+
+// Completed subtasks of a reduced task are folded pairwise in completion order into one
+// accumulated shadow (reference pmReducer::AddSubtask, pmReducer.cpp:191-217).
+static void reduce_into(Task* t, std::unique_ptr<Subtask> st) {
+    std::lock_guard<std::mutex> g(t->reduce_mtx);
+    if (!t->pending_reduce) {
+        t->pending_reduce = std::move(st);
+        return;
+    }
+    Subtask* acc = t->pending_reduce.get();
+    pmReductionOpType op;
+    pmReductionDataType dt;
+    bool builtin = is_builtin_reduction(t->callbacks->cb.dataReduction, &op, &dt);
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        if (!t->is_reduced(i) || !st->shadow[i]) continue;
+        if (acc->shadow_on_device[i] != st->shadow_on_device[i])
+            raise(pmFatalError, "mixed host/device reduction shadows");
+        if (acc->shadow_on_device[i]) {
+            if (!builtin || op != REDUCE_ADD || dt != REDUCE_FLOATS)
+                raise(pmFatalError, "device reduction supports REDUCE_ADD over floats");
+            if (uh_reduce_add_f32((float*)acc->shadow[i], (const float*)st->shadow[i],
+                                  acc->shadow_len[i] / 4, Runtime::get()->copy_stream) ||
+                uh_stream_sync(Runtime::get()->copy_stream))
+                raise(pmGraphicsCardError, uh_last_error());
+            Memory::get().device_free(st->shadow[i], st->shadow_len[i]);
+            st->shadow[i] = nullptr;
+        } else if (builtin) {
+            reduce_host(acc->shadow[i], st->shadow[i], acc->shadow_len[i], op, dt);
+        }
+    }
+    if (!builtin) {
+        // user-defined reduction: both subtasks are described by NATURAL views of their shadows
+        auto view = [&](Subtask* s) {
+            pmSubtaskInfo si;
+            si.subtaskId = s->id;
+            si.memCount = (uint)t->spaces.size();
+            for (uint i = 0; i < t->spaces.size(); ++i) {
+                if (t->is_reduced(i) && s->shadow[i]) {
+                    si.memInfo[i].ptr = si.memInfo[i].writePtr = s->shadow[i];
+                    si.memInfo[i].length = s->shadow_len[i];
+                }
+            }
+            return si;
+        };
+        pmDeviceInfo d1 = *(acc->device ? acc->device : Runtime::get()->cpu_devices[0]);
+        pmDeviceInfo d2 = *(st->device ? st->device : Runtime::get()->cpu_devices[0]);
+        pmStatus s = t->callbacks->cb.dataReduction(t->info(), d1, view(acc), d2, view(st.get()));
+        if (s != pmSuccess) raise(s, "dataReduction callback failed");
+    }
+    for (uint i = 0; i < t->spaces.size(); ++i)
+        if (st->shadow[i] && !st->shadow_on_device[i]) { free(st->shadow[i]); st->shadow[i] = nullptr; }
+}
+
+// Write the accumulated shadow of every reduced space back into the address space.
+static void commit_reduction(Task* t) {
+    std::lock_guard<std::mutex> g(t->reduce_mtx);
+    Subtask* acc = t->pending_reduce.get();
+    if (!acc) return;
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        if (!t->is_reduced(i) || !acc->shadow[i]) continue;
+        AddressSpace* as = t->spaces[i];
+        pmSubscriptionInfo u = acc->mem[i].unified();
+        if (acc->shadow_on_device[i]) {
+            Memory::get().ensure_device(as, Runtime::get()->copy_stream);
+            if (uh_memcpy_d2d(as->dev + u.offset, acc->shadow[i], acc->shadow_len[i], Runtime::get()->copy_stream) ||
+                uh_stream_sync(Runtime::get()->copy_stream))
+                raise(pmGraphicsCardError, uh_last_error());
+            Memory::get().mark_device_written(as);
+            Memory::get().device_free(acc->shadow[i], acc->shadow_len[i]);
+        } else {
+            Memory::get().sync_host(as);
+            memcpy(as->host + u.offset, acc->shadow[i], acc->shadow_len[i]);
+            Memory::get().mark_host_written(as);
+            free(acc->shadow[i]);
+        }
+        acc->shadow[i] = nullptr;
+    }
+    t->pending_reduce.reset();
+}
+
+void Runtime::complete(Task* t) {
+    try {
+        commit_reduction(t);
+    } catch (Error& e) {
+        int expected = pmSuccess;
+        t->status.compare_exchange_strong(expected, (int)e.status);
+        set_last_error(e.what);
+    }
+    for (AddressSpace* as : t->spaces) {
+        std::lock_guard<std::mutex> g(as->mtx);
+        as->users--;
+    }
+    if (t->callbacks->cb.taskCompletionCallback) t->callbacks->cb.taskCompletionCallback(t->info());
+    std::lock_guard<std::mutex> g(t->mtx);
+    t->end_time = now_secs();
+    t->state = TaskState::Done;
+    t->cv.notify_all();
+}
+
+static void fail_task(Task* t, const Error& e) {
+    int expected = pmSuccess;
+    t->status.compare_exchange_strong(expected, (int)e.status);
+    set_last_error(e.what);
+}
+
+// ---------------------------------------------------------------- CPU stubs
+
+void Runtime::cpu_loop(int idx) {
+    pmDeviceInfo* dev = cpu_devices[idx];
+    for (;;) {
+        Task* t;
+        ulong first = 0, count = 0;
+        {
+            std::unique_lock<std::mutex> g(q_mtx);
+            q_cv.wait(g, [&] {
+                if (stopping) return true;
+                for (Task* x : active)
+                    if (x->cpu_allowed && x->next.load() < x->subtask_count) return true;
+                return false;
+            });
+            if (stopping) return;
+            t = pick(false, first, count, 1);
+            if (!t) continue;
+        }
+        for (ulong s = first; s < first + count; ++s) {
+            std::unique_ptr<Subtask> st(new Subtask());
+            st->id = s;
+            try {
+                if (t->status.load() == pmSuccess) {
+                    if (t->callbacks->cb.deviceSelection &&
+                        !t->callbacks->cb.deviceSelection(t->info(), *dev))
+                        raise(pmNoCompatibleDevice, "device selection rejected the CPU stub");
+                    run_distribution(t, *st, dev);
+                    std::vector<std::vector<uint8_t>> compact;
+                    pmSubtaskInfo si = host_subtask_info(t, *st, compact);
+                    pmStatus rs = t->callbacks->cb.subtask_cpu(t->info(), *dev, si);
+                    if (rs != pmSuccess) raise(rs, "subtask_cpu callback failed");
+                    // COMPACT views: scatter written records back
+                    for (uint i = 0; i < t->spaces.size(); ++i) {
+                        if (!t->writable(i) || si.memInfo[i].visibilityType != SUBSCRIPTION_COMPACT) continue;
+                        CompactLayout cl = make_compact(st->mem[i]);
+                        for (auto& kv : st->mem[i].write.map()) {
+                            size_t co = cl.compact_offset(kv.first);
+                            memcpy(t->spaces[i]->host + kv.first, compact[i].data() + co, kv.second);
+                        }
+                    }
+                    if (t->callbacks->cb.dataReduction) reduce_into(t, std::move(st));
+                }
+            } catch (Error& e) {
+                fail_task(t, e);
+            }
+            finish_subtasks(t, 1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- HIP stub
+
+namespace {
+
+void hipchk(int rc, const char* what) {
+    if (rc) raise(pmGraphicsCardError, std::string(what) + ": " + uh_last_error());
+}
+
+// One GPU subtask in flight.
+struct GpuSlot {
+    Task* task = nullptr;
+    std::unique_ptr<Subtask> st;
+    void* stream = nullptr;
+    void* event = nullptr;
+    // device buffers owned by this subtask (freed at completion)
+    std::vector<std::pair<void*, size_t>> dev_bufs;
+    // PLAIN: written records to scatter from the pinned buffer at completion
+    struct Scatter { uint mem; size_t src_off, len, pinned_off; };
+    std::vector<Scatter> scatters;
+    uint8_t* pinned = nullptr;
+    size_t pinned_cap = 0;
+    // RESIDENT + COMPACT: written records to scatter device->mirror at completion
+    struct DevScatter { AddressSpace* as; void* buf; CompactLayout cl; };
+    std::vector<DevScatter> dev_scatters;
+    // PLAIN: read-only cache entries this subtask reads (pinned until it completes)
+    std::vector<std::pair<uint64_t, std::vector<size_t>>> cache_refs;
+    bool busy = false;
+};
+
+// Read-only device buffers of the PLAIN mode, keyed by (address space, exact records)
+// (reference pmCudaCacheKey, pmDataTypes.h:295-387; LRU eviction).
+struct CacheEntry {
+    void* buf = nullptr;
+    size_t bytes = 0;
+    uint64_t last_use = 0;
+    int users = 0;
+};
+
+struct GpuState {
+    GpuSlot slots[16];
+    std::map<std::pair<uint64_t, std::vector<size_t>>, CacheEntry> cache;
+    uint64_t clock = 0;
+};
+
+GpuState& gstate() {
+    static GpuState s;
+    return s;
+}
+
+void cache_purge(AddressSpace* as, bool all) {
+    auto& c = gstate().cache;
+    for (auto it = c.begin(); it != c.end();) {
+        if ((all || (as && it->first.first == as->uid)) && it->second.users == 0) {
+            Memory::get().device_free(it->second.buf, it->second.bytes);
+            it = c.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+void cache_shrink(size_t need) {
+    auto& c = gstate().cache;
+    while (Memory::get().resident() + need > Memory::get().budget()) {
+        auto victim = c.end();
+        for (auto it = c.begin(); it != c.end(); ++it)
+            if (it->second.users == 0 && (victim == c.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == c.end()) return;
+        Memory::get().device_free(victim->second.buf, victim->second.bytes);
+        c.erase(victim);
+    }
+}
+
+void* slot_device_alloc(GpuSlot& s, size_t bytes) {
+    if (!bytes) return nullptr;
+    cache_shrink(bytes);
+    void* p = Memory::get().device_alloc(bytes);
+    s.dev_bufs.push_back({p, bytes});
+    return p;
+}
+
+uint8_t* slot_pinned(GpuSlot& s, size_t bytes) {
+    if (bytes > s.pinned_cap) {
+        if (s.pinned) uh_host_free(s.pinned);
+        s.pinned = nullptr;
+        size_t cap = std::max(bytes, (size_t)1 << 20);
+        hipchk(uh_host_alloc((void**)&s.pinned, cap), "pinned staging allocation");
+        s.pinned_cap = cap;
+    }
+    return s.pinned;
+}
+
+void stat_add(uint64_t pmHipStubStats::*f, uint64_t v) {
+    Runtime* r = Runtime::get();
+    std::lock_guard<std::mutex> g(r->stats_mtx);
+    r->stats.*f += v;
+}
+
+void upload_scratch(Task* t, Subtask& st, GpuSlot& s, pmSubtaskInfo& si, void* stream) {
+    auto it = st.scratch.find(PRE_SUBTASK_TO_SUBTASK);
+    if (it == st.scratch.end()) it = st.scratch.find(PRE_SUBTASK_TO_POST_SUBTASK);
+    if (it == st.scratch.end() || it->second.empty()) return;
+    void* d = slot_device_alloc(s, it->second.size());
+    uint8_t* pin = slot_pinned(s, it->second.size());
+    memcpy(pin, it->second.data(), it->second.size());
+    hipchk(uh_memcpy_h2d(d, pin, it->second.size(), stream), "scratch upload");
+    hipchk(uh_stream_sync(stream), "scratch upload");
+    si.gpuContext.scratchBuffer = d;
+    (void)t;
+}
+
+// PLAIN mode: stage every address space of the subtask as the reference stub does.
+pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
+    Subtask& st = *s.st;
+    pmSubtaskInfo si;
+    si.subtaskId = st.id;
+    si.memCount = (uint)t->spaces.size();
+    si.splitInfo = st.split;
+    // gather everything to upload into one pinned buffer
+    struct Up { void* dst; size_t pin_off, len; };
+    std::vector<Up> ups;
+    std::vector<std::pair<size_t, std::pair<const uint8_t*, size_t>>> gathers;  // pin_off <- src,len
+    size_t pin_total = 0;
+    size_t scatter_total = 0;
+    struct PendingScatter { uint mem; size_t src_off, len; void* dev; };
+    std::vector<PendingScatter> pend;
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        AddressSpace* as = t->spaces[i];
+        MemSubscription& ms = st.mem[i];
+        pmMemInfo mi;
+        mi.visibilityType = t->visibility(i, true);
+        const bool natural = mi.visibilityType == SUBSCRIPTION_NATURAL;
+        // records to stage and their device positions
+        std::vector<size_t> rsrc, rdst, rlen;
+        size_t total = 0, base_off = 0;
+        CompactLayout cl;
+        if (natural) {
+            pmSubscriptionInfo u = t->read_only(i) ? ms.cread : ms.unified();
+            base_off = u.offset;
+            total = u.length;
+            for (auto& kv : ms.read.map()) { rsrc.push_back(kv.first); rdst.push_back(kv.first - base_off); rlen.push_back(kv.second); }
+        } else {
+            cl = make_compact(ms);
+            total = cl.total;
+            for (auto& kv : ms.read.map()) { rsrc.push_back(kv.first); rdst.push_back(cl.compact_offset(kv.first)); rlen.push_back(kv.second); }
+        }
+        if (!total) { si.memInfo[i] = mi; continue; }
+        uint8_t* dbuf = nullptr;
+        if (t->read_only(i) && t->details.gpuCacheEnabled) {
+            std::vector<size_t> key;
+            key.push_back(natural ? 1 : 0);
+            key.push_back((size_t)as->version.load());
+            for (auto& kv : ms.read.map()) { key.push_back(kv.first); key.push_back(kv.second); }
+            auto& cache = gstate().cache;
+            auto ck = std::make_pair(as->uid, key);
+            auto it = cache.find(ck);
+            if (it != cache.end()) {
+                it->second.last_use = ++gstate().clock;
+                it->second.users++;
+                dbuf = (uint8_t*)it->second.buf;
+                stat_add(&pmHipStubStats::cache_hits, 1);
+                rsrc.clear();
+            } else {
+                cache_shrink(total);
+                CacheEntry e;
+                e.buf = Memory::get().device_alloc(total);
+                e.bytes = total;
+                e.users = 1;
+                e.last_use = ++gstate().clock;
+                cache[ck] = e;
+                dbuf = (uint8_t*)e.buf;
+            }
+            s.cache_refs.push_back(ck);
+        } else {
+            dbuf = (uint8_t*)slot_device_alloc(s, total);
+        }
+        for (size_t r = 0; r < rsrc.size(); ++r) {
+            gathers.push_back({pin_total, {as->host + rsrc[r], rlen[r]}});
+            ups.push_back({dbuf + rdst[r], pin_total, rlen[r]});
+            pin_total += rlen[r];
+        }
+        if (t->writable(i)) {
+            for (auto& kv : ms.write.map()) {
+                size_t doff = natural ? kv.first - base_off : cl.compact_offset(kv.first);
+                pend.push_back({i, kv.first, kv.second, dbuf + doff});
+                scatter_total += kv.second;
+            }
+        }
+        mi.ptr = dbuf;
+        mi.length = total;
+        if (natural) {
+            if (t->read_only(i)) mi.readPtr = dbuf;
+            else if (t->write_only(i)) mi.writePtr = dbuf;
+            else {
+                if (ms.has_read) mi.readPtr = dbuf + (ms.cread.offset - base_off);
+                if (ms.has_write) mi.writePtr = dbuf + (ms.cwrite.offset - base_off);
+            }
+        } else {
+            if (cl.has_read) mi.readPtr = dbuf + cl.first_read;
+            if (cl.has_write) mi.writePtr = dbuf + cl.first_write;
+        }
+        if (t->is_reduced(i)) {
+            st.shadow[i] = nullptr;   // plain mode reduces on the host after D2H
+        }
+        si.memInfo[i] = mi;
+    }
+    // host gather into pinned memory, then H2D (pmExecutionStub.cpp:3655-3750, 3370-3615)
+    uint8_t* pin = slot_pinned(s, std::max(pin_total, scatter_total));
+    for (auto& g : gathers) memcpy(pin + g.first, g.second.first, g.second.second);
+    // one copy per run of records that is contiguous both in pinned memory and on the device
+    // (a COMPACT read-only view becomes a single H2D, as pmCudaInterface.cu:482-487)
+    for (size_t u = 0; u < ups.size();) {
+        size_t v = u + 1, len = ups[u].len;
+        while (v < ups.size() && ups[v].pin_off == ups[u].pin_off + len &&
+               (uint8_t*)ups[v].dst == (uint8_t*)ups[u].dst + len) {
+            len += ups[v].len;
+            ++v;
+        }
+        hipchk(uh_memcpy_h2d(ups[u].dst, pin + ups[u].pin_off, len, s.stream), "H2D staging");
+        u = v;
+    }
+    stat_add(&pmHipStubStats::h2d_bytes, pin_total);
+    // the D2H commands are issued after the callback (see run_plain)
+    s.scatters.clear();
+    size_t poff = 0;
+    for (auto& p : pend) {
+        s.scatters.push_back({p.mem, p.src_off, p.len, poff});
+        poff += p.len;
+    }
+    // stash device sources of the scatters in dev_scatters' place
+    s.dev_scatters.clear();
+    for (size_t k = 0; k < pend.size(); ++k) {
+        GpuSlot::DevScatter d;
+        d.as = t->spaces[pend[k].mem];
+        d.buf = pend[k].dev;
+        s.dev_scatters.push_back(d);
+    }
+    return si;
+}
+
+// RESIDENT / BATCHED: views of HBM mirrors. COMPACT views are gathered on the device.
+pmSubtaskInfo stage_resident(Task* t, GpuSlot& s, void* stream) {
+    Subtask& st = *s.st;
+    pmSubtaskInfo si;
+    si.subtaskId = st.id;
+    si.memCount = (uint)t->spaces.size();
+    si.splitInfo = st.split;
+    s.dev_scatters.clear();
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        AddressSpace* as = t->spaces[i];
+        MemSubscription& ms = st.mem[i];
+        pmMemInfo mi;
+        mi.visibilityType = t->visibility(i, true);
+        if (mi.visibilityType == SUBSCRIPTION_NATURAL) {
+            pmSubscriptionInfo u = t->read_only(i) ? ms.cread : ms.unified();
+            if (u.length) {
+                uint8_t* base = as->dev + u.offset;
+                if (t->is_reduced(i)) {
+                    void* sh = Memory::get().device_alloc(u.length);
+                    hipchk(uh_memcpy_d2d(sh, as->dev + u.offset, u.length, stream), "shadow init");
+                    st.shadow[i] = sh;
+                    st.shadow_len[i] = u.length;
+                    st.shadow_on_device[i] = true;
+                    base = (uint8_t*)sh;
+                }
+                mi.ptr = base;
+                mi.length = u.length;
+                if (t->read_only(i)) mi.readPtr = base;
+                else if (t->write_only(i)) mi.writePtr = base;
+                else {
+                    if (ms.has_read) mi.readPtr = base + (ms.cread.offset - u.offset);
+                    if (ms.has_write) mi.writePtr = base + (ms.cwrite.offset - u.offset);
+                }
+            }
+        } else {
+            if (t->is_reduced(i)) raise(pmFatalError, "COMPACT views of reduced spaces are not supported on the HIP stub");
+            CompactLayout cl = make_compact(ms);
+            if (cl.total) {
+                uint8_t* buf = (uint8_t*)slot_device_alloc(s, cl.total);
+                // record tables: src offsets, dst offsets, lengths
+                size_t n = cl.src_off.size();
+                std::vector<uint64_t> tab(3 * n);
+                for (size_t r = 0; r < n; ++r) { tab[r] = cl.src_off[r]; tab[n + r] = cl.dst_off[r]; tab[2 * n + r] = cl.len[r]; }
+                uint64_t* dtab = (uint64_t*)slot_device_alloc(s, tab.size() * 8);
+                uint8_t* pin = slot_pinned(s, tab.size() * 8);
+                memcpy(pin, tab.data(), tab.size() * 8);
+                hipchk(uh_memcpy_h2d(dtab, pin, tab.size() * 8, stream), "record table upload");
+                uint64_t maxlen = 0;
+                for (size_t r = 0; r < n; ++r) maxlen = std::max<uint64_t>(maxlen, cl.len[r]);
+                if (t->read_only(i) || !t->write_only(i)) {
+                    hipchk(uh_copy_records(buf, as->dev, dtab, dtab + n, dtab + 2 * n, (uint32_t)n, maxlen, stream), "device gather");
+                    stat_add(&pmHipStubStats::d2d_bytes, cl.total);
+                }
+                hipchk(uh_stream_sync(stream), "record table upload");   // pinned buffer reuse
+                mi.ptr = buf;
+                mi.length = cl.total;
+                if (cl.has_read) mi.readPtr = buf + cl.first_read;
+                if (cl.has_write) mi.writePtr = buf + cl.first_write;
+                if (t->writable(i)) {
+                    GpuSlot::DevScatter d;
+                    d.as = as;
+                    d.buf = buf;
+                    d.cl = cl;
+                    s.dev_scatters.push_back(d);
+                }
+            }
+        }
+        si.memInfo[i] = mi;
+    }
+    return si;
+}
+
+// After a RESIDENT/BATCHED subtask with COMPACT writable views: scatter written records back
+// into the mirror (device to device).
+void scatter_resident(Task* t, GpuSlot& s, void* stream) {
+    for (auto& d : s.dev_scatters) {
+        MemSubscription* ms = nullptr;
+        for (uint i = 0; i < t->spaces.size(); ++i) if (t->spaces[i] == d.as) ms = &s.st->mem[i];
+        if (!ms) continue;
+        for (auto& kv : ms->write.map()) {
+            size_t co = d.cl.compact_offset(kv.first);
+            hipchk(uh_memcpy_d2d(d.as->dev + kv.first, (uint8_t*)d.buf + co, kv.second, stream), "device scatter");
+            stat_add(&pmHipStubStats::d2d_bytes, kv.second);
+        }
+    }
+}
+
+}  // namespace
+
+// Launch a kernel-kind callback: copies of taskConf and pmDeviceInfo are placed in device
+// memory owned by the slot; arguments are passed by value as the kernel signature requires.
+static pmStatus launch_kernel_kind(Task* t, Subtask& st, pmSubtaskInfo& si, void* stream,
+                                   pmDeviceInfo* dev, GpuSlot& s) {
+    pmTaskInfo ti = t->info();
+    if (!t->conf.empty()) {
+        void* dconf = slot_device_alloc(s, t->conf.size());
+        uint8_t* pin = slot_pinned(s, t->conf.size() + sizeof(pmDeviceInfo));
+        memcpy(pin, t->conf.data(), t->conf.size());
+        hipchk(uh_memcpy_h2d(dconf, pin, t->conf.size(), stream), "taskConf upload");
+        ti.taskConf = dconf;
+    }
+    void* ddev = slot_device_alloc(s, sizeof(pmDeviceInfo));
+    uint8_t* pin = slot_pinned(s, t->conf.size() + sizeof(pmDeviceInfo));
+    memcpy(pin + t->conf.size(), dev, sizeof(pmDeviceInfo));
+    hipchk(uh_memcpy_h2d(ddev, pin + t->conf.size(), sizeof(pmDeviceInfo), stream), "device info upload");
+    pmStatus* dstatus = (pmStatus*)slot_device_alloc(s, sizeof(pmStatus));
+    hipchk(uh_stream_sync(stream), "kernel-kind argument upload");
+    pmDeviceInfo* ddevp = (pmDeviceInfo*)ddev;
+    void* args[] = {&ti, &ddevp, &si, &dstatus};
+    const pmGpuLaunchConf& lc = st.launch;
+    hipchk(uh_launch_kernel((const void*)t->callbacks->cb.subtask_gpu_hip, lc.blocksX, lc.blocksY, lc.blocksZ,
+                            lc.threadsX, lc.threadsY, lc.threadsZ, (size_t)lc.sharedMem, stream, args),
+           "kernel-kind launch");
+    return pmSuccess;
+}
+
+static pmStatus run_gpu_callback(Task* t, Subtask& st, pmSubtaskInfo& si, void* stream,
+                                 pmDeviceInfo* dev, GpuSlot& s) {
+    if (t->callbacks->cb.subtask_gpu_custom)
+        return t->callbacks->cb.subtask_gpu_custom(t->info(), *dev, si, stream);
+    if (!st.has_launch) raise(pmInvalidCallbacks, "subtask_gpu_hip needs pmSetGpuLaunchConf");
+    return launch_kernel_kind(t, st, si, stream, dev, s);
+}
+
+static void release_slot(GpuSlot& s) {
+    for (auto& b : s.dev_bufs) Memory::get().device_free(b.first, b.second);
+    s.dev_bufs.clear();
+    for (auto& k : s.cache_refs) {
+        auto it = gstate().cache.find(k);
+        if (it != gstate().cache.end() && it->second.users > 0) it->second.users--;
+    }
+    s.cache_refs.clear();
+    s.scatters.clear();
+    s.dev_scatters.clear();
+    s.busy = false;
+    s.task = nullptr;
+    s.st.reset();
+}
+
+// Completion of one PLAIN/RESIDENT slot (its event has fired).
+static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
+    Task* t = s.task;
+    try {
+        if (plain) {
+            // scatter written records from pinned memory into the host address space
+            // (reference CopyDataFromPinnedBuffers, pmExecutionStub.cpp:3752-3847); records of
+            // reduced spaces go to a host shadow of the subtask instead
+            for (auto& sc : s.scatters) {
+                if (t->is_reduced(sc.mem)) {
+                    pmSubscriptionInfo u = s.st->mem[sc.mem].unified();
+                    if (!s.st->shadow[sc.mem]) {
+                        s.st->shadow[sc.mem] = calloc(1, u.length);
+                        if (!s.st->shadow[sc.mem]) raise(pmMemoryError, "shadow allocation");
+                        s.st->shadow_len[sc.mem] = u.length;
+                        s.st->shadow_on_device[sc.mem] = false;
+                    }
+                    memcpy((uint8_t*)s.st->shadow[sc.mem] + (sc.src_off - u.offset), s.pinned + sc.pinned_off, sc.len);
+                } else {
+                    memcpy(t->spaces[sc.mem]->host + sc.src_off, s.pinned + sc.pinned_off, sc.len);
+                }
+            }
+        }
+        if (t->callbacks->cb.dataReduction && t->status.load() == pmSuccess) {
+            reduce_into(t, std::move(s.st));
+        }
+    } catch (Error& e) {
+        fail_task(t, e);
+    }
+    stat_add(&pmHipStubStats::subtasks, 1);
+    release_slot(s);
+    r->finish_subtasks(t, 1);
+}
+
+void Runtime::gpu_loop() {
+    uh_set_device(0);
+    GpuState& gs = gstate();
+    int inflight = 0;
+    int next_slot = 0;
+    // batches in flight: event + list of (task, subtask) slots
+    struct BatchRun {
+        void* event = nullptr;
+        std::vector<std::unique_ptr<GpuSlot>> slots;
+    };
+    std::deque<BatchRun> batches;
+    void* batch_events[4];
+    for (int i = 0; i < 4; ++i) uh_event_create(&batch_events[i]);
+    int batch_ev = 0;
+    auto ensure_task_ready = [&](Task* t) {
+        if (t->gpu_prepared) return;
+        if (t->gpu_resident) {
+            for (AddressSpace* as : t->spaces) Memory::get().ensure_device(as, copy_stream);
+            hipchk(uh_stream_sync(copy_stream), "mirror upload");
+        }
+        t->gpu_prepared = true;
+    };
+
+    auto retire_batches = [&](bool block) {
+        while (!batches.empty()) {
+            BatchRun& b = batches.front();
+            if (block) uh_event_sync(b.event);
+            else if (uh_event_query(b.event) != 0) return;
+            for (auto& sp : b.slots) {
+                GpuSlot& s = *sp;
+                Task* t = s.task;
+                try {
+                    if (!s.dev_scatters.empty()) {
+                        scatter_resident(t, s, batch_stream);
+                        hipchk(uh_stream_sync(batch_stream), "device scatter");
+                    }
+                } catch (Error& e) { fail_task(t, e); }
+                retire_slot(this, s, false);
+            }
+            batches.pop_front();
+        }
+    };
+
+    for (;;) {
+        // retire finished per-subtask slots
+        for (int i = 0; i < 16; ++i) {
+            GpuSlot& s = gs.slots[i];
+            if (s.busy && uh_event_query(s.event) == 0) {
+                bool plain = !s.task->gpu_resident || hip_mode.load() == PM_HIP_MODE_PLAIN;
+                if (!plain && !s.dev_scatters.empty()) {
+                    try { scatter_resident(s.task, s, s.stream); hipchk(uh_stream_sync(s.stream), "scatter"); }
+                    catch (Error& e) { fail_task(s.task, e); }
+                }
+                retire_slot(this, s, plain);
+                inflight--;
+            }
+        }
+        retire_batches(false);
+        const int mode = hip_mode.load();
+        const int depth = std::max(1, std::min(16, pipeline_depth.load()));
+        Task* t = nullptr;
+        ulong first = 0, count = 0;
+        {
+            std::unique_lock<std::mutex> g(q_mtx);
+            auto has_work = [&] {
+                for (Task* x : active)
+                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return true;
+                return false;
+            };
+            if (inflight == 0 && batches.empty()) {
+                q_cv.wait(g, [&] { return stopping || has_work(); });
+                if (stopping && !has_work()) break;
+            } else if (!has_work() || (mode != PM_HIP_MODE_BATCHED && inflight >= depth) ||
+                       (mode == PM_HIP_MODE_BATCHED && (int)batches.size() >= 2)) {
+                g.unlock();
+                // wait for the oldest work to finish
+                if (mode == PM_HIP_MODE_BATCHED && !batches.empty()) retire_batches(true);
+                else {
+                    for (int i = 0; i < 16; ++i)
+                        if (gs.slots[i].busy) { uh_event_sync(gs.slots[i].event); break; }
+                }
+                continue;
+            }
+            ulong max_take = mode == PM_HIP_MODE_BATCHED ? (ulong)batch_limit.load() : 1;
+            t = pick(true, first, count, max_take);
+            if (!t) continue;
+        }
+
+        if (mode == PM_HIP_MODE_BATCHED && t->gpu_resident) {
+            // Gather subtasks of this and other active tasks into one batch.
+            BatchRun run;
+            std::vector<std::pair<Task*, std::pair<ulong, ulong>>> claims;
+            claims.push_back({t, {first, count}});
+            ulong taken = count;
+            {
+                std::lock_guard<std::mutex> g(q_mtx);
+                while (taken < (ulong)batch_limit.load()) {
+                    ulong f2 = 0, c2 = 0;
+                    Task* t2 = nullptr;
+                    // only tasks that can be batched (GPU-resident)
+                    for (Task* x : active) {
+                        if (x->gpu_allowed && x->gpu_resident && x->next.load() < x->subtask_count) { t2 = x; break; }
+                    }
+                    if (!t2) break;
+                    Task* got = pick(true, f2, c2, (ulong)batch_limit.load() - taken);
+                    if (!got) break;
+                    claims.push_back({got, {f2, c2}});
+                    taken += c2;
+                }
+            }
+            for (auto& cl : claims) {
+                Task* tt = cl.first;
+                try { ensure_task_ready(tt); } catch (Error& e) { fail_task(tt, e); }
+                for (ulong sid = cl.second.first; sid < cl.second.first + cl.second.second; ++sid) {
+                    std::unique_ptr<GpuSlot> sp(new GpuSlot());
+                    sp->task = tt;
+                    sp->st.reset(new Subtask());
+                    sp->st->id = sid;
+                    sp->busy = true;
+                    if (tt->status.load() == pmSuccess) {
+                        try {
+                            if (tt->callbacks->cb.deviceSelection && !tt->callbacks->cb.deviceSelection(tt->info(), *gpu_device))
+                                raise(pmNoCompatibleDevice, "device selection rejected the HIP stub");
+                            run_distribution(tt, *sp->st, gpu_device);
+                            pmSubtaskInfo si = stage_resident(tt, *sp, batch_stream);
+                            if (sp->st->reserved_gpu)
+                                si.gpuContext.reservedGlobalMem = slot_device_alloc(*sp, sp->st->reserved_gpu);
+                            upload_scratch(tt, *sp->st, *sp, si, batch_stream);
+                            pmStatus rs = run_gpu_callback(tt, *sp->st, si, batch, gpu_device, *sp);
+                            if (rs != pmSuccess) raise(rs, "GPU callback failed");
+                        } catch (Error& e) { fail_task(tt, e); }
+                    }
+                    run.slots.push_back(std::move(sp));
+                }
+                // the stub held one pick per claim; slots release one each
+            }
+            try {
+                hipchk(uh_batch_flush(batch, use_graphs.load()), "batch flush");
+            } catch (Error& e) {
+                for (auto& cl : claims) fail_task(cl.first, e);
+            }
+            run.event = batch_events[batch_ev];
+            batch_ev = (batch_ev + 1) % 4;
+            hipchk(uh_event_record(run.event, batch_stream), "batch event");
+            {
+                std::lock_guard<std::mutex> g(stats_mtx);
+                stats.batches++;
+            }
+            for (auto& sp : run.slots)
+                for (uint i = 0; i < sp->task->spaces.size(); ++i)
+                    if (sp->task->writable(i) && !sp->task->is_reduced(i)) Memory::get().mark_device_written(sp->task->spaces[i]);
+            batches.push_back(std::move(run));
+            continue;
+        }
+
+        // PLAIN / RESIDENT: one slot per subtask
+        for (ulong sid = first; sid < first + count; ++sid) {
+            int si_idx = -1;
+            for (int k = 0; k < depth; ++k) {
+                int idx = (next_slot + k) % depth;
+                if (!gs.slots[idx].busy) { si_idx = idx; break; }
+            }
+            if (si_idx < 0) {
+                // all slots busy: wait for one
+                for (int k = 0; k < depth; ++k) {
+                    GpuSlot& s = gs.slots[k];
+                    if (s.busy) {
+                        uh_event_sync(s.event);
+                        bool plain = !s.task->gpu_resident || hip_mode.load() == PM_HIP_MODE_PLAIN;
+                        if (!plain && !s.dev_scatters.empty()) {
+                            try { scatter_resident(s.task, s, s.stream); hipchk(uh_stream_sync(s.stream), "scatter"); }
+                            catch (Error& e) { fail_task(s.task, e); }
+                        }
+                        retire_slot(this, s, plain);
+                        inflight--;
+                        si_idx = k;
+                        break;
+                    }
+                }
+            }
+            next_slot = (si_idx + 1) % depth;
+            GpuSlot& s = gs.slots[si_idx];
+            s.task = t;
+            s.st.reset(new Subtask());
+            s.st->id = sid;
+            s.stream = gpu_streams[si_idx];
+            s.event = gpu_events[si_idx];
+            s.busy = true;
+            const bool plain = !t->gpu_resident || mode == PM_HIP_MODE_PLAIN;
+            if (t->status.load() == pmSuccess) {
+                try {
+                    if (t->callbacks->cb.deviceSelection && !t->callbacks->cb.deviceSelection(t->info(), *gpu_device))
+                        raise(pmNoCompatibleDevice, "device selection rejected the HIP stub");
+                    if (!plain) ensure_task_ready(t);
+                    run_distribution(t, *s.st, gpu_device);
+                    pmSubtaskInfo info = plain ? stage_plain(t, s) : stage_resident(t, s, s.stream);
+                    if (s.st->reserved_gpu) info.gpuContext.reservedGlobalMem = slot_device_alloc(s, s.st->reserved_gpu);
+                    upload_scratch(t, *s.st, s, info, s.stream);
+                    pmStatus rs = run_gpu_callback(t, *s.st, info, s.stream, gpu_device, s);
+                    if (rs != pmSuccess) raise(rs, "GPU callback failed");
+                    if (plain) {
+                        // D2H of written records into the pinned buffer (pmCudaInterface.cu:510-527)
+                        uint64_t bytes = 0;
+                        for (size_t k = 0; k < s.scatters.size();) {
+                            size_t v = k + 1, len = s.scatters[k].len;
+                            while (v < s.scatters.size() &&
+                                   s.scatters[v].pinned_off == s.scatters[k].pinned_off + len &&
+                                   (uint8_t*)s.dev_scatters[v].buf == (uint8_t*)s.dev_scatters[k].buf + len) {
+                                len += s.scatters[v].len;
+                                ++v;
+                            }
+                            hipchk(uh_memcpy_d2h(s.pinned + s.scatters[k].pinned_off, s.dev_scatters[k].buf, len,
+                                                 s.stream), "D2H staging");
+                            bytes += len;
+                            k = v;
+                        }
+                        s.dev_scatters.clear();
+                        stat_add(&pmHipStubStats::d2h_bytes, bytes);
+                    } else {
+                        for (uint i = 0; i < t->spaces.size(); ++i)
+                            if (t->writable(i) && !t->is_reduced(i)) Memory::get().mark_device_written(t->spaces[i]);
+                    }
+                } catch (Error& e) {
+                    fail_task(t, e);
+                }
+            }
+            hipchk(uh_event_record(s.event, s.stream), "subtask event");
+            inflight++;
+        }
+    }
+    retire_batches(true);
+    for (int i = 0; i < 4; ++i) uh_event_destroy(batch_events[i]);
+    cache_purge(nullptr, true);
+}
+
+void purge_gpu_cache_for(AddressSpace* as) { cache_purge(as, false); }
+
+}  // namespace rt
+}  // namespace pm

@@ -1,0 +1,241 @@
+// Internal types of the single-node pm:: runtime (libpmlib.so).
+//
+// Layer map (reference counterparts in parentheses):
+//   api.cpp           public API, status mapping            (pmPublicDefinitions.cpp)
+//   subscription.cpp  subscription records and views        (pmSubscriptionManager.cpp:397-2031)
+//   memory.cpp        address spaces, HBM mirrors, budget   (pmAddressSpace, pmMemChunk, pmCache)
+//   executor.cpp      tasks, per-device queues, CPU stubs,  (pmTask, pmScheduler, pmExecutionStub,
+//                     the HIP stub and its three modes       pmDispatcherGPU, pmCudaInterface)
+//   reduce.cpp        built-in subtask reductions           (pmReducer.cpp:593-710)
+#pragma once
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pmPublicDefinitions.h"
+#include "pmPublicUtilities.h"
+#include "pmExtensions.h"
+
+namespace pm {
+namespace rt {
+
+// Thrown inside the runtime, mapped to pmStatus at the API boundary.
+struct Error {
+    pmStatus status;
+    std::string what;
+};
+[[noreturn]] void raise(pmStatus s, const std::string& what);
+void set_last_error(const std::string& s);
+
+double now_secs();
+
+// ---------------------------------------------------------------- intervals / subscriptions
+
+// Disjoint, non-adjacent byte intervals [offset, offset+length) keyed by offset.
+class IntervalSet {
+public:
+    void add(size_t offset, size_t length);
+    bool empty() const { return m_.empty(); }
+    const std::map<size_t, size_t>& map() const { return m_; }
+    size_t total() const;
+private:
+    std::map<size_t, size_t> m_;
+};
+
+struct MemSubscription {
+    IntervalSet read, write;
+    pmSubscriptionInfo cread{0, 0}, cwrite{0, 0};     // consolidated extents
+    bool has_read = false, has_write = false;
+    void add(pmSubscriptionType type, size_t offset, size_t length);
+    void add_scattered(pmSubscriptionType type, const pmScatteredSubscriptionInfo& s);
+    pmSubscriptionInfo unified() const;               // min offset / max span of read and write
+};
+
+// COMPACT view geometry: union of read and write records packed back to back in address order
+// (reference GetCompactedSubscription, pmSubscriptionManager.cpp:1978-2031).
+struct CompactLayout {
+    std::vector<size_t> src_off, dst_off, len;   // one entry per packed record
+    size_t total = 0;
+    size_t first_read = 0, first_write = 0;      // compact offsets of the first records
+    bool has_read = false, has_write = false;
+    size_t compact_offset(size_t addr) const;    // address must lie inside a record
+};
+CompactLayout make_compact(const MemSubscription& s);
+
+// ---------------------------------------------------------------- address spaces
+
+struct AddressSpace {
+    size_t length = 0;
+    size_t rows = 0, cols = 0;          // 2-D layout (cols in bytes), rows == 0 for linear
+    uint8_t* host = nullptr;
+    bool host_pinned = false;
+    uint8_t* dev = nullptr;             // HBM mirror
+    bool host_valid = true;
+    bool dev_valid = false;
+    int users = 0;                      // tasks currently holding it
+    uint64_t last_use = 0;
+    uint64_t uid = 0;                   // unique id (never reused)
+    std::atomic<uint64_t> version{0};   // bumped on every write; keys the PLAIN-mode cache
+    std::mutex mtx;
+    bool released = false;
+};
+
+class Memory {
+public:
+    static Memory& get();
+    AddressSpace* create(size_t length, size_t rows, size_t cols);
+    void release(AddressSpace* as);
+    // Make the host copy current (download from HBM if the mirror is newer).
+    void sync_host(AddressSpace* as);
+    // Make the HBM mirror current (allocate within the budget, upload if the host is newer).
+    void ensure_device(AddressSpace* as, void* stream);
+    void mark_device_written(AddressSpace* as);
+    void mark_host_written(AddressSpace* as);
+    // HBM budget accounting for mirrors and subtask buffers.
+    size_t budget() const { return budget_; }
+    size_t resident() const { return resident_.load(); }
+    void set_budget(size_t b) { budget_ = b; }
+    bool try_reserve(size_t bytes);
+    void unreserve(size_t bytes);
+    void* device_alloc(size_t bytes);   // budgeted hipMalloc; evicts idle mirrors on pressure
+    void device_free(void* p, size_t bytes);
+    void evict_idle(size_t need);
+    std::mutex list_mtx;
+    std::vector<AddressSpace*> spaces;
+private:
+    size_t budget_ = 0;
+    std::atomic<size_t> resident_{0};
+    uint64_t clock_ = 0;
+};
+
+// ---------------------------------------------------------------- tasks
+
+struct CallbackSet {
+    std::string key;
+    pmCallbacks cb;
+};
+
+struct Task;
+
+struct Subtask {
+    ulong id = 0;
+    pmSplitInfo split;
+    MemSubscription mem[MAX_MEM_SECTIONS_PER_TASK];
+    size_t reserved_gpu = 0;
+    bool has_launch = false;
+    pmGpuLaunchConf launch;
+    // scratch buffers per pmScratchBufferType (host side)
+    std::map<int, std::vector<uint8_t>> scratch;
+    // shadow buffers for reduced address spaces (host or device)
+    void* shadow[MAX_MEM_SECTIONS_PER_TASK] = {nullptr, nullptr, nullptr};
+    bool shadow_on_device[MAX_MEM_SECTIONS_PER_TASK] = {false, false, false};
+    size_t shadow_len[MAX_MEM_SECTIONS_PER_TASK] = {0, 0, 0};
+    pmDeviceInfo* device = nullptr;
+};
+
+enum class TaskState { Queued, Running, Done };
+
+struct Task {
+    pmTaskDetails details;
+    std::vector<uint8_t> conf;                   // owned copy of taskConf
+    std::vector<pmTaskMem> mem;
+    std::vector<AddressSpace*> spaces;
+    CallbackSet* callbacks = nullptr;
+    ulong subtask_count = 0;
+    std::atomic<ulong> next{0};                  // next unassigned subtask id
+    std::atomic<ulong> done{0};
+    std::atomic<int> status{pmSuccess};
+    bool cpu_allowed = false, gpu_allowed = false;
+    bool gpu_resident = false;                   // GPU subtasks use HBM mirrors (GPU-only task)
+    bool gpu_prepared = false;                   // mirrors made current by the HIP stub
+    double submit_time = 0, start_time = 0, end_time = 0;
+    double deadline = 0;
+    std::mutex mtx;
+    std::condition_variable cv;
+    TaskState state = TaskState::Queued;
+    uint64_t seq = 0;
+    // reduction: completed subtasks awaiting a partner
+    std::unique_ptr<Subtask> pending_reduce;
+    std::mutex reduce_mtx;
+    std::vector<std::unique_ptr<Subtask>> reduce_queue;
+    pmTaskInfo info();
+    bool is_reduced(uint i) const;
+    bool writable(uint i) const;
+    bool read_only(uint i) const;
+    bool write_only(uint i) const;
+    pmSubscriptionVisibilityType visibility(uint i, bool gpu) const;
+};
+
+// Thread-local context of the data-distribution callback being run, so that
+// pmSubscribeToMemory & co. can find the subtask they describe.
+struct DistributionContext {
+    Task* task = nullptr;
+    Subtask* subtask = nullptr;
+    pmDeviceHandle device = nullptr;
+};
+DistributionContext*& current_context();
+
+class Runtime {
+public:
+    static Runtime* get();               // null before pmInitialize
+    static void init();
+    static void finalize();
+    std::vector<pmDeviceInfo*> cpu_devices;
+    pmDeviceInfo* gpu_device = nullptr;
+    int gpu_count = 0;
+    // callbacks
+    std::mutex cb_mtx;
+    std::map<std::string, std::unique_ptr<CallbackSet>> callbacks;
+    // task queue
+    void submit(Task* t);
+    void wait(Task* t);
+    std::mutex q_mtx;
+    std::condition_variable q_cv;
+    std::vector<Task*> active;           // queued or running, ordered by (priority, seq)
+    uint64_t seq = 0;
+    bool stopping = false;
+    // stubs
+    std::vector<std::thread> cpu_threads;
+    std::thread gpu_thread;
+    void cpu_loop(int idx);
+    void gpu_loop();
+    // mode of the HIP stub
+    std::atomic<int> hip_mode{PM_HIP_MODE_BATCHED};
+    std::atomic<int> hip_view{SUBSCRIPTION_NATURAL};
+    std::atomic<int> use_graphs{1};
+    std::atomic<int> pipeline_depth{4};
+    std::atomic<int> batch_limit{4096};
+    std::atomic<int> cpu_workers{0};
+    pmHipStubStats stats{};
+    std::mutex stats_mtx;
+    void* gpu_streams[16] = {};
+    void* gpu_events[16] = {};
+    void* batch = nullptr;
+    void* batch_stream = nullptr;
+    void* copy_stream = nullptr;
+    // common helpers
+    Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count);
+    void finish_subtasks(Task* t, ulong n);
+    void complete(Task* t);
+private:
+    static Runtime* inst_;
+};
+
+// Runs the dataDistribution callback of one subtask and fills its subscriptions.
+void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev);
+// Builds the pmSubtaskInfo of a CPU (host, NATURAL/COMPACT) subtask; may allocate shadows.
+pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs);
+// Reduction helpers
+bool is_builtin_reduction(pmDataReductionCallback cb, pmReductionOpType* op, pmReductionDataType* dt);
+void reduce_host(void* dst, const void* src, size_t bytes, pmReductionOpType op, pmReductionDataType dt);
+
+}  // namespace rt
+}  // namespace pm

@@ -1,0 +1,210 @@
+// Subscription records and subtask views.
+//
+// Follows the reference's subscription manager semantics:
+//   - a READ_WRITE subscription registers both a read and a write record
+//     (pmSubscriptionManager.cpp:425-433);
+//   - records of one kind are kept as a union of byte ranges; overlapping and adjacent ranges
+//     merge (AddSubscriptionRecordToMap, pmSubscriptionManager.cpp:744-843);
+//   - the consolidated extent of a kind is the min offset / max span of its subscriptions
+//     (pmSubscriptionManager.cpp:475-489, 508-523); a scattered subscription spans
+//     step * count bytes from its offset;
+//   - the NATURAL view of a read-only space starts at the consolidated read extent, that of a
+//     writable space at the unified read/write extent (GetSubtaskInfo, 1882-1976);
+//   - the COMPACT view packs the union of read and write records in address order
+//     (GetCompactedSubscription, 1978-2031); readPtr / writePtr point at the packed position of
+//     the first read / write record.
+#include "pm_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+namespace pm {
+namespace rt {
+
+void IntervalSet::add(size_t offset, size_t length) {
+    if (!length) return;
+    size_t lo = offset, hi = offset + length;
+    // first interval that could overlap or touch [lo, hi): the last one starting <= lo
+    auto it = m_.upper_bound(lo);
+    if (it != m_.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second >= lo) it = prev;
+    }
+    while (it != m_.end() && it->first <= hi) {
+        lo = std::min(lo, it->first);
+        hi = std::max(hi, it->first + it->second);
+        it = m_.erase(it);
+    }
+    m_[lo] = hi - lo;
+}
+
+size_t IntervalSet::total() const {
+    size_t t = 0;
+    for (auto& kv : m_) t += kv.second;
+    return t;
+}
+
+static void extend(pmSubscriptionInfo& c, bool& has, size_t off, size_t span) {
+    if (!has) {
+        c = pmSubscriptionInfo(off, span);
+        has = true;
+        return;
+    }
+    size_t lo = std::min(c.offset, off);
+    size_t hi = std::max(c.offset + c.length, off + span);
+    c = pmSubscriptionInfo(lo, hi - lo);
+}
+
+void MemSubscription::add(pmSubscriptionType type, size_t offset, size_t length) {
+    if (!length) return;
+    if (type == READ_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION) {
+        read.add(offset, length);
+        extend(cread, has_read, offset, length);
+    }
+    if (type == WRITE_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION) {
+        write.add(offset, length);
+        extend(cwrite, has_write, offset, length);
+    }
+}
+
+void MemSubscription::add_scattered(pmSubscriptionType type, const pmScatteredSubscriptionInfo& s) {
+    if (!s.size || !s.count) return;
+    const bool r = (type == READ_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION);
+    const bool w = (type == WRITE_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION);
+    for (size_t i = 0; i < s.count; ++i) {
+        if (r) read.add(s.offset + i * s.step, s.size);
+        if (w) write.add(s.offset + i * s.step, s.size);
+    }
+    if (r) extend(cread, has_read, s.offset, s.step * s.count);
+    if (w) extend(cwrite, has_write, s.offset, s.step * s.count);
+}
+
+pmSubscriptionInfo MemSubscription::unified() const {
+    if (!has_read && !has_write) return pmSubscriptionInfo(0, 0);
+    if (has_read && !has_write) return cread;
+    if (!has_read && has_write) return cwrite;
+    size_t lo = std::min(cread.offset, cwrite.offset);
+    size_t hi = std::max(cread.offset + cread.length, cwrite.offset + cwrite.length);
+    return pmSubscriptionInfo(lo, hi - lo);
+}
+
+size_t CompactLayout::compact_offset(size_t addr) const {
+    // records are sorted by src_off
+    auto it = std::upper_bound(src_off.begin(), src_off.end(), addr);
+    if (it == src_off.begin()) raise(pmFatalError, "compact_offset: address before first record");
+    size_t i = (size_t)(it - src_off.begin()) - 1;
+    if (addr >= src_off[i] + len[i]) raise(pmFatalError, "compact_offset: address outside records");
+    return dst_off[i] + (addr - src_off[i]);
+}
+
+CompactLayout make_compact(const MemSubscription& s) {
+    IntervalSet u = s.read;
+    for (auto& kv : s.write.map()) u.add(kv.first, kv.second);
+    CompactLayout c;
+    size_t pos = 0;
+    for (auto& kv : u.map()) {
+        c.src_off.push_back(kv.first);
+        c.dst_off.push_back(pos);
+        c.len.push_back(kv.second);
+        pos += kv.second;
+    }
+    c.total = pos;
+    if (!s.read.empty()) {
+        c.has_read = true;
+        c.first_read = c.compact_offset(s.read.map().begin()->first);
+    }
+    if (!s.write.empty()) {
+        c.has_write = true;
+        c.first_write = c.compact_offset(s.write.map().begin()->first);
+    }
+    return c;
+}
+
+DistributionContext*& current_context() {
+    static thread_local DistributionContext* ctx = nullptr;
+    return ctx;
+}
+
+void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
+    st.device = dev;
+    pmDataDistributionCallback dd = t->callbacks->cb.dataDistribution;
+    if (!dd) return;
+    DistributionContext ctx;
+    ctx.task = t;
+    ctx.subtask = &st;
+    ctx.device = dev->deviceHandle;
+    DistributionContext* saved = current_context();
+    current_context() = &ctx;
+    pmSubtaskInfo si;
+    si.subtaskId = st.id;
+    si.memCount = (uint)t->spaces.size();
+    si.splitInfo = st.split;
+    pmStatus s = dd(t->info(), *dev, si);
+    current_context() = saved;
+    if (s != pmSuccess) raise(s, "dataDistribution callback failed");
+}
+
+// Host view of a subtask for the CPU stubs. NATURAL views point into the address space (or a
+// shadow buffer for reduced spaces); COMPACT views are packed into compact_bufs.
+pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs) {
+    pmSubtaskInfo si;
+    si.subtaskId = st.id;
+    si.memCount = (uint)t->spaces.size();
+    si.splitInfo = st.split;
+    compact_bufs.resize(t->spaces.size());
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        AddressSpace* as = t->spaces[i];
+        MemSubscription& ms = st.mem[i];
+        pmMemInfo mi;
+        mi.visibilityType = t->visibility(i, false);
+        if (mi.visibilityType == SUBSCRIPTION_NATURAL) {
+            if (t->read_only(i)) {
+                if (ms.has_read) {
+                    mi.ptr = mi.readPtr = as->host + ms.cread.offset;
+                    mi.length = ms.cread.length;
+                }
+            } else {
+                pmSubscriptionInfo u = ms.unified();
+                if (u.length) {
+                    uint8_t* base;
+                    if (t->is_reduced(i)) {
+                        if (!st.shadow[i]) {
+                            st.shadow[i] = malloc(u.length);
+                            if (!st.shadow[i]) raise(pmMemoryError, "shadow allocation");
+                            st.shadow_len[i] = u.length;
+                            st.shadow_on_device[i] = false;
+                            memcpy(st.shadow[i], as->host + u.offset, u.length);
+                        }
+                        base = (uint8_t*)st.shadow[i];
+                    } else {
+                        base = as->host + u.offset;
+                    }
+                    mi.ptr = base;
+                    mi.length = u.length;
+                    if (t->write_only(i)) {
+                        mi.writePtr = mi.ptr;
+                    } else {
+                        mi.readPtr = ms.has_read ? base + (ms.cread.offset - u.offset) : nullptr;
+                        mi.writePtr = ms.has_write ? base + (ms.cwrite.offset - u.offset) : nullptr;
+                    }
+                }
+            }
+        } else {
+            CompactLayout cl = make_compact(ms);
+            std::vector<uint8_t>& buf = compact_bufs[i];
+            buf.assign(cl.total, 0);
+            for (size_t r = 0; r < cl.src_off.size(); ++r)
+                memcpy(buf.data() + cl.dst_off[r], as->host + cl.src_off[r], cl.len[r]);
+            mi.ptr = cl.total ? buf.data() : nullptr;
+            mi.length = cl.total;
+            if (cl.has_read) mi.readPtr = buf.data() + cl.first_read;
+            if (cl.has_write) mi.writePtr = buf.data() + cl.first_write;
+        }
+        si.memInfo[i] = mi;
+    }
+    return si;
+}
+
+}  // namespace rt
+}  // namespace pm
