@@ -58,6 +58,14 @@ int ua_if_run(void* h, int device, double* seconds);
 int ua_if_get_output(void* h, uint8_t* out);
 int ua_if_destroy(void* h);
 
+/* ---- matrixMultiply (reference testSuite/matrixMultiplyBlas decomposition, plain CPU method) ----
+ * n x n row-major float32; subtasks = (n/block)^2 output blocks; precision as uh_gemm. */
+void* ua_mm_create(int n, int block, int precision);
+int ua_mm_set_inputs(void* h, const float* A, const float* B);
+int ua_mm_run(void* h, int device, double* seconds);
+int ua_mm_get_output(void* h, float* C);
+int ua_mm_destroy(void* h);
+
 #ifdef __cplusplus
 }
 #endif

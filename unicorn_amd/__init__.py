@@ -85,6 +85,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_if_create": (vp, [C.POINTER(ua_if_config)]), "ua_if_set_input": (i32, [vp, vp]),
         "ua_if_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_if_get_output": (i32, [vp, vp]),
         "ua_if_destroy": (i32, [vp]),
+        "ua_mm_create": (vp, [i32, i32, i32]), "ua_mm_set_inputs": (i32, [vp, vp, vp]),
+        "ua_mm_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_mm_get_output": (i32, [vp, vp]),
+        "ua_mm_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -176,6 +179,44 @@ class ImageFilter:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_if_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatMul:
+    """matrixMultiply task: C = A B, n x n float32, (n/block)^2 block subtasks."""
+
+    def __init__(self, n: int, block: int = 512, precision: int = 0):
+        init()
+        self.n = n
+        self._h = apps_lib().ua_mm_create(n, block, precision)
+        if not self._h:
+            _check(-1, "ua_mm_create")
+
+    def set_inputs(self, A: np.ndarray, B: np.ndarray) -> None:
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        assert A.shape == (self.n, self.n) and B.shape == (self.n, self.n)
+        _check(apps_lib().ua_mm_set_inputs(self._h, _ptr(A), _ptr(B)), "ua_mm_set_inputs")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_mm_run(self._h, device, C.byref(t)), "matrixMultiply task")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty((self.n, self.n), dtype=np.float32)
+        _check(apps_lib().ua_mm_get_output(self._h, _ptr(out)), "ua_mm_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_mm_destroy(self._h)
             self._h = None
 
     def __del__(self):
