@@ -66,6 +66,24 @@ int ua_mm_run(void* h, int device, double* seconds);
 int ua_mm_get_output(void* h, float* C);
 int ua_mm_destroy(void* h);
 
+/* ---- web graph generator (the role of the reference's webGenerator.pl) ----
+ * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
+ * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).
+ * Call once with col == NULL to get row_ptr and *edges, then again with col of that size. */
+int ua_webgen(uint32_t pages, uint64_t seed, double mean_out, uint32_t max_out, double dangling,
+              uint32_t* row_ptr, uint32_t* col, uint64_t col_capacity, uint64_t* edges);
+double ua_webgen_shift(double mean_out, uint32_t max_out, double dangling);
+
+/* ---- pageRank (reference testSuite/pageRank): graph as an out-link CSR (0-based targets);
+ * subtasks own `pages_per_subtask` destination pages; ua_pr_run runs the init task and
+ * `iterations` iteration tasks; ranks are those after the last iteration. */
+void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                   uint32_t pages_per_subtask, float damping);
+int ua_pr_run(void* h, int iterations, int device, double* seconds);
+int ua_pr_get_ranks(void* h, float* ranks);
+uint64_t ua_pr_edges(void* h);
+int ua_pr_destroy(void* h);
+
 #ifdef __cplusplus
 }
 #endif

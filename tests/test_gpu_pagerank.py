@@ -1,0 +1,95 @@
+"""GPU pageRank (pull SpMV over the HBM-resident in-link CSR) against the oracle.
+
+Tolerance: relative L1 <= 1e-5 against the oracle's restatement of the reference (push per
+subtask + REDUCE_ADD in subtask order); the error against the FP64 oracle is asserted too.
+Closed forms pin the recurrence: a ring gives 0.85 everywhere, a star gives the hub
+0.85 * leaves after the first iteration, and a graph of dangling pages gives zero.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_l1(a, b):
+    return float(np.abs(a.astype(np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+@pytest.mark.parametrize("hip_mode", [0, 1, 2])
+def test_pagerank_matches_oracle(gpu_runtime, hip_mode):
+    U = gpu_runtime
+    U.set_hip_mode(hip_mode)
+    rp, col = U.webgen(200_000, 11)
+    pr = U.PageRank(rp, col, 50_000)
+    try:
+        pr.run(20, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 20, 50_000)) <= 1e-5
+    assert _rel_l1(y, O.pagerank_f64(rp, col, 20)) <= 1e-5
+
+
+def test_pagerank_compact_views(gpu_runtime):
+    U = gpu_runtime
+    U.set_hip_mode(1, compact=True)
+    rp, col = U.webgen(30_000, 5)
+    pr = U.PageRank(rp, col, 7_000)      # ragged last subtask
+    try:
+        pr.run(4, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+        U.set_hip_mode(2)
+    assert _rel_l1(y, O.pagerank(rp, col, 4, 7_000)) <= 1e-5
+
+
+def _csr(n, edges):
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    for s, _ in edges:
+        rp[s + 1] += 1
+    rp = np.cumsum(rp).astype(np.uint32)
+    col = np.zeros(max(1, len(edges)), dtype=np.uint32)
+    pos = rp[:-1].copy()
+    for s, t in sorted(edges):
+        col[pos[s]] = t
+        pos[s] += 1
+    return rp, col[:len(edges)]
+
+
+def test_pagerank_closed_forms(gpu_runtime):
+    U = gpu_runtime
+    U.set_hip_mode(2)
+    n = 1000
+    rp, col = _csr(n, [(i, (i + 1) % n) for i in range(n)])       # ring
+    pr = U.PageRank(rp, col, 300)
+    pr.run(3, U.DEVICE_GPU)
+    np.testing.assert_allclose(pr.ranks(), 0.85, rtol=1e-6)
+    pr.close()
+    rp, col = _csr(n, [(i, 0) for i in range(1, n)])              # star into page 0
+    pr = U.PageRank(rp, col, 256)
+    pr.run(1, U.DEVICE_GPU)
+    y = pr.ranks()
+    pr.close()
+    assert abs(y[0] - 0.85 * (n - 1)) <= 1e-5 * 0.85 * (n - 1)
+    assert np.all(y[1:] == 0)
+    rp = np.zeros(n + 1, dtype=np.uint32)                           # all dangling
+    pr = U.PageRank(rp, np.zeros(0, dtype=np.uint32), 128)
+    pr.run(2, U.DEVICE_GPU)
+    assert np.all(pr.ranks() == 0)
+    pr.close()
+
+
+def test_pagerank_cpu_gpu_mix(gpu_runtime):
+    U = gpu_runtime
+    U.set_hip_mode(1)
+    rp, col = U.webgen(50_000, 2)
+    pr = U.PageRank(rp, col, 5_000)
+    try:
+        pr.run(5, U.DEVICE_BOTH)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 5, 5_000)) <= 1e-5

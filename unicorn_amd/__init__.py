@@ -88,6 +88,13 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_mm_create": (vp, [i32, i32, i32]), "ua_mm_set_inputs": (i32, [vp, vp, vp]),
         "ua_mm_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_mm_get_output": (i32, [vp, vp]),
         "ua_mm_destroy": (i32, [vp]),
+        "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
+                            C.POINTER(C.c_uint64)]),
+        "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
+        "ua_pr_create": (vp, [C.c_uint32, vp, vp, C.c_uint32, C.c_float]),
+        "ua_pr_run": (i32, [vp, i32, i32, C.POINTER(dbl)]),
+        "ua_pr_get_ranks": (i32, [vp, vp]), "ua_pr_edges": (C.c_uint64, [vp]),
+        "ua_pr_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -179,6 +186,53 @@ class ImageFilter:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_if_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def webgen(pages: int, seed: int, mean_out: float = 10.0, max_out: int = 100, dangling: float = 0.05):
+    """Synthetic power-law web graph as an out-link CSR (row_ptr, col) with 0-based targets."""
+    lib = apps_lib()
+    row_ptr = np.empty(pages + 1, dtype=np.uint32)
+    edges = C.c_uint64(0)
+    _check(lib.ua_webgen(pages, seed, mean_out, max_out, dangling, _ptr(row_ptr), None, 0, C.byref(edges)),
+           "ua_webgen")
+    col = np.empty(max(1, edges.value), dtype=np.uint32)
+    _check(lib.ua_webgen(pages, seed, mean_out, max_out, dangling, _ptr(row_ptr), _ptr(col), col.size,
+                         C.byref(edges)), "ua_webgen")
+    return row_ptr, col[:edges.value]
+
+
+class PageRank:
+    """pageRank over an out-link CSR; subtasks own `pages_per_subtask` destination pages."""
+
+    def __init__(self, row_ptr: np.ndarray, col: np.ndarray, pages_per_subtask: int, damping: float = 0.85):
+        init()
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        self.col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
+        self.pages = self.row_ptr.size - 1
+        self._h = apps_lib().ua_pr_create(self.pages, _ptr(self.row_ptr), _ptr(self.col), pages_per_subtask, damping)
+        if not self._h:
+            _check(-1, "ua_pr_create")
+
+    def run(self, iterations: int, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_pr_run(self._h, iterations, device, C.byref(t)), "pageRank tasks")
+        return t.value
+
+    def ranks(self) -> np.ndarray:
+        out = np.empty(self.pages, dtype=np.float32)
+        _check(apps_lib().ua_pr_get_ranks(self._h, _ptr(out)), "ua_pr_get_ranks")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_pr_destroy(self._h)
             self._h = None
 
     def __del__(self):

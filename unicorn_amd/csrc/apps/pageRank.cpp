@@ -1,0 +1,300 @@
+// pageRank app on the pm:: API (reference testSuite/pageRank/source/code/src/pageRank.cpp and
+// pageRankCUDA.cu).
+//
+// Recurrence (reference pageRank.cpp:237-276, 336-378): on iteration 0 every page has rank 1.0;
+// each iteration, y[t] = sum over links src -> t of (0.85 * x[src]) / outdeg(src); no teleport
+// term, dangling pages contribute nothing.
+//
+// Decomposition. The reference splits the *source* pages into subtasks, each pushing into a
+// private full-length output vector that a float REDUCE_ADD then folds on the host, and streams
+// the padded web dump from mmap'd files into the GPU every iteration. Here the graph is an
+// address space holding an in-link CSR (built once at load), subtasks own *destination* page
+// ranges, and each computes its rows by pulling from the sources' increments: no atomics, no
+// partial vectors, no reduction, and the graph stays resident in HBM across iterations.
+// Address spaces of one iteration task:
+//   0 graph  (READ_ONLY):  [in_row_ptr (N+1) | outdeg (N) | in_col (E)]   uint32
+//   1 state  (READ_ONLY):  [x (N) | w (N)] float, w[p] = (0.85 * x[p]) / outdeg(p)
+//   2 state' (WRITE_ONLY): [y (N) | w' (N)] float, the next state
+// Subtask s subscribes to rows [s0, s1) of the graph (row pointers, degrees and their in-link
+// range), to all of w, and writes y[s0:s1) and w'[s0:s1). An "init" task writes the iteration-0
+// state (x = 1, w = 0.85 / outdeg).
+#include "app_common.h"
+#include "unicorn_hip.h"
+
+#include <vector>
+
+namespace pageRank {
+
+using namespace pm;
+
+enum memIndex { GRAPH_MEM_INDEX = 0, STATE_IN_MEM_INDEX = 1, STATE_OUT_MEM_INDEX = 2 };
+
+struct pageRankTaskConf {
+    uint32_t totalWebPages;
+    uint32_t webPagesPerSubtask;
+    uint64_t totalLinks;
+    float dampingFactor;
+    float initialPageRank;
+    int kind;                        // 0 = init, 1 = iterate
+    const uint32_t* hostRowPtr;      // in-link row pointers on the host (subscription sizing)
+};
+
+static inline size_t row_ptr_off(uint32_t t) { return (size_t)t * 4; }
+static inline size_t deg_off(const pageRankTaskConf* c, uint32_t t) { return ((size_t)c->totalWebPages + 1 + t) * 4; }
+static inline size_t col_off(const pageRankTaskConf* c, uint64_t e) { return ((size_t)2 * c->totalWebPages + 1 + e) * 4; }
+
+static void range_of(const pageRankTaskConf* c, ulong id, uint32_t* s0, uint32_t* s1) {
+    *s0 = (uint32_t)(id * c->webPagesPerSubtask);
+    const uint64_t e = (uint64_t)*s0 + c->webPagesPerSubtask;
+    *s1 = (uint32_t)(e < c->totalWebPages ? e : c->totalWebPages);
+}
+
+static pmStatus prDataDistribution(pmTaskInfo ti, pmDeviceInfo di, pmSubtaskInfo si) {
+    const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    uint32_t s0, s1;
+    range_of(c, si.subtaskId, &s0, &s1);
+    const size_t N = c->totalWebPages;
+    // graph rows: row pointers s0..s1 (inclusive), degrees s0..s1-1 (the init task needs only these)
+    pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX, READ_SUBSCRIPTION,
+                        pmSubscriptionInfo(row_ptr_off(s0), (size_t)(s1 - s0 + 1) * 4));
+    pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX, READ_SUBSCRIPTION,
+                        pmSubscriptionInfo(deg_off(c, s0), (size_t)(s1 - s0) * 4));
+    if (c->kind == 1) {
+        const uint64_t e0 = c->hostRowPtr[s0], e1 = c->hostRowPtr[s1];
+        if (e1 > e0)
+            pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX,
+                                READ_SUBSCRIPTION, pmSubscriptionInfo(col_off(c, e0), (size_t)(e1 - e0) * 4));
+        pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_IN_MEM_INDEX,
+                            READ_SUBSCRIPTION, pmSubscriptionInfo(N * 4, N * 4));
+    }
+    pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_OUT_MEM_INDEX, WRITE_SUBSCRIPTION,
+                        pmScatteredSubscriptionInfo((size_t)s0 * 4, (size_t)(s1 - s0) * 4, N * 4, 2));
+    return pmSuccess;
+}
+
+// Device/host pointers of one subtask, rebased so that they are indexed by global page id
+// (and edge id for col), for NATURAL and COMPACT views alike.
+struct Views {
+    const uint32_t* row_ptr;
+    const uint32_t* deg;
+    const uint32_t* col;
+    const float* w;
+    float* y;
+    float* w_next;
+};
+
+static Views views_of(const pageRankTaskConf* c, const pmSubtaskInfo& si, uint32_t s0, uint32_t s1) {
+    Views v;
+    const pmMemInfo& g = si.memInfo[GRAPH_MEM_INDEX];
+    const pmMemInfo& in = si.memInfo[STATE_IN_MEM_INDEX];
+    const pmMemInfo& out = si.memInfo[STATE_OUT_MEM_INDEX];
+    const size_t N = c->totalWebPages;
+    const uint64_t e0 = c->hostRowPtr[s0], e1 = c->hostRowPtr[s1];
+    const uint8_t* gb = (const uint8_t*)g.ptr;
+    if (g.visibilityType == SUBSCRIPTION_NATURAL) {
+        // ptr is the start of the first subscribed record (row_ptr[s0]); distances are preserved
+        v.row_ptr = (const uint32_t*)gb - s0;
+        v.deg = (const uint32_t*)(gb + (deg_off(c, s0) - row_ptr_off(s0))) - s0;
+        v.col = (const uint32_t*)(gb + (col_off(c, e0) - row_ptr_off(s0))) - e0;
+    } else {
+        // packed records in address order: row pointers, degrees, in-links
+        const size_t rp = (size_t)(s1 - s0 + 1) * 4, dg = (size_t)(s1 - s0) * 4;
+        v.row_ptr = (const uint32_t*)gb - s0;
+        v.deg = (const uint32_t*)(gb + rp) - s0;
+        v.col = (const uint32_t*)(gb + rp + dg) - e0;
+        (void)e1;
+    }
+    v.w = nullptr;
+    if (c->kind == 1) {
+        // state_in: one record, the whole w vector
+        v.w = (const float*)in.ptr;
+    }
+    uint8_t* ob = (uint8_t*)out.ptr;
+    if (out.visibilityType == SUBSCRIPTION_NATURAL) {
+        v.y = (float*)ob - s0;
+        v.w_next = (float*)(ob + N * 4) - s0;
+    } else {
+        v.y = (float*)ob - s0;
+        v.w_next = (float*)(ob + (size_t)(s1 - s0) * 4) - s0;
+    }
+    return v;
+}
+
+static pmStatus pr_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
+    const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    uint32_t s0, s1;
+    range_of(c, si.subtaskId, &s0, &s1);
+    Views v = views_of(c, si, s0, s1);
+    for (uint32_t t = s0; t < s1; ++t) {
+        float y;
+        if (c->kind == 0) {
+            y = c->initialPageRank;
+        } else {
+            y = 0.f;
+            for (uint32_t e = v.row_ptr[t]; e < v.row_ptr[t + 1]; ++e) y += v.w[v.col[e]];
+        }
+        v.y[t] = y;
+        const uint32_t d = v.deg[t];
+        v.w_next[t] = d ? (c->dampingFactor * y) / (float)d : 0.f;
+    }
+    return pmSuccess;
+}
+
+static pmStatus pr_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* stream) {
+    const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    uint32_t s0, s1;
+    range_of(c, si.subtaskId, &s0, &s1);
+    Views v = views_of(c, si, s0, s1);
+    int rc;
+    if (c->kind == 0) {
+        // the init task is tiny and runs once per chain: y = x0 on the device, w = damping x0 / deg
+        rc = uh_fill_u32(v.y + s0, *(const uint32_t*)&c->initialPageRank, s1 - s0, stream);
+        if (!rc) rc = uh_pagerank_init_w(v.w_next + s0, v.deg + s0, s1 - s0, c->initialPageRank, c->dampingFactor, stream);
+    } else {
+        uh_pagerank_args a;
+        a.row_ptr = v.row_ptr;
+        a.col = v.col;
+        a.deg = v.deg;
+        a.w = v.w;
+        a.y = v.y;
+        a.w_next = v.w_next;
+        a.row0 = s0;
+        a.rows = s1 - s0;
+        a.damping = c->dampingFactor;
+        a.pad = 0;
+        rc = uh_pagerank_pull(&a, stream);
+    }
+    if (rc) {
+        apps::set_error(std::string("pageRank GPU callback: ") + uh_last_error());
+        return pmGraphicsCardError;
+    }
+    return pmSuccess;
+}
+
+struct Instance {
+    uint32_t pages = 0, per_subtask = 0;
+    uint64_t edges = 0;
+    float damping = 0.85f;
+    std::vector<uint32_t> in_row_ptr;   // host copy for subscription sizing
+    pmMemHandle graph = nullptr, state[2] = {nullptr, nullptr};
+    int current = 0;                    // state holding the latest ranks
+};
+
+static apps::CallbackVariants g_cb;
+static std::mutex g_mtx;
+
+static int run_kind(Instance* in, int kind, int device, double* secs) {
+    pageRankTaskConf conf;
+    conf.totalWebPages = in->pages;
+    conf.webPagesPerSubtask = in->per_subtask;
+    conf.totalLinks = in->edges;
+    conf.dampingFactor = in->damping;
+    conf.initialPageRank = 1.0f;
+    conf.kind = kind;
+    conf.hostRowPtr = in->in_row_ptr.data();
+    const int src = in->current, dst = kind == 0 ? in->current : 1 - in->current;
+    pmTaskMem mem[3];
+    mem[GRAPH_MEM_INDEX] = pmTaskMem(in->graph, READ_ONLY, SUBSCRIPTION_OPTIMAL);
+    mem[STATE_IN_MEM_INDEX] = pmTaskMem(in->state[kind == 0 ? 1 - dst : src], READ_ONLY, SUBSCRIPTION_OPTIMAL);
+    mem[STATE_OUT_MEM_INDEX] = pmTaskMem(in->state[dst], WRITE_ONLY, SUBSCRIPTION_OPTIMAL);
+    const ulong subtasks = (in->pages + in->per_subtask - 1) / in->per_subtask;
+    pmTaskDetails d(&conf, sizeof(conf), mem, 3, g_cb.h[device], subtasks);
+    int rc = apps::run_task(d, secs);
+    if (rc == 0) in->current = dst;
+    return rc;
+}
+
+}  // namespace pageRank
+
+using namespace pageRank;
+
+// out_row_ptr / out_col: out-link CSR (0-based targets), e.g. from ua_webgen.
+extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                              uint32_t pages_per_subtask, float damping) {
+    if (!pages || !out_row_ptr || !pages_per_subtask) {
+        apps::set_error("ua_pr_create: invalid arguments");
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_mtx);
+        pmCallbacks cb;
+        cb.dataDistribution = prDataDistribution;
+        cb.subtask_cpu = pr_cpu;
+        cb.subtask_gpu_custom = pr_gpu;
+        if (apps::register_variants("PAGERANK", cb, g_cb) != 0) {
+            apps::set_error(std::string("register callbacks: ") + pmGetLastError());
+            return nullptr;
+        }
+    }
+    Instance* in = new Instance();
+    in->pages = pages;
+    in->per_subtask = pages_per_subtask;
+    in->damping = damping;
+    in->edges = out_row_ptr[pages];
+    const size_t N = pages;
+    const size_t graph_bytes = (2 * N + 1 + in->edges) * 4;
+    if (pmCreateMemory(graph_bytes, &in->graph) != pmSuccess ||
+        pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
+        pmCreateMemory(2 * N * 4, &in->state[1]) != pmSuccess) {
+        apps::set_error(std::string("create memory: ") + pmGetLastError());
+        delete in;
+        return nullptr;
+    }
+    pmRawMemPtr p;
+    pmGetRawMemPtr(in->graph, &p);
+    uint32_t* g = (uint32_t*)p;
+    uint32_t* rp = g;
+    uint32_t* deg = g + N + 1;
+    uint32_t* col = g + 2 * N + 1;
+    // in-link CSR by counting sort over targets; sources ascending within each row
+    std::vector<uint32_t> cnt(N + 1, 0);
+    for (uint64_t e = 0; e < in->edges; ++e) cnt[out_col[e] + 1]++;
+    rp[0] = 0;
+    for (size_t t = 0; t < N; ++t) rp[t + 1] = rp[t] + cnt[t + 1];
+    std::vector<uint32_t> pos(rp, rp + N);
+    for (uint32_t s = 0; s < pages; ++s) {
+        deg[s] = out_row_ptr[s + 1] - out_row_ptr[s];
+        for (uint32_t e = out_row_ptr[s]; e < out_row_ptr[s + 1]; ++e) col[pos[out_col[e]]++] = s;
+    }
+    in->in_row_ptr.assign(rp, rp + N + 1);
+    return in;
+}
+
+// Runs the init task and `iterations` iteration tasks on the device set.
+extern "C" int ua_pr_run(void* h, int iterations, int device, double* seconds) {
+    Instance* in = (Instance*)h;
+    if (device < 0 || device > 2 || iterations < 0) return -1;
+    double t = 0, s = 0;
+    int rc = run_kind(in, 0, device, &s);
+    t += s;
+    for (int i = 0; i < iterations && rc == 0; ++i) {
+        rc = run_kind(in, 1, device, &s);
+        t += s;
+    }
+    if (seconds) *seconds = t;
+    return rc;
+}
+
+extern "C" int ua_pr_get_ranks(void* h, float* y) {
+    Instance* in = (Instance*)h;
+    pmMemHandle m = in->state[in->current];
+    pmStatus st = pmFetchMemory(m);
+    if (st != pmSuccess) return (int)st;
+    pmRawMemPtr p;
+    st = pmGetRawMemPtr(m, &p);
+    if (st != pmSuccess) return (int)st;
+    memcpy(y, p, (size_t)in->pages * 4);
+    return 0;
+}
+
+extern "C" uint64_t ua_pr_edges(void* h) { return ((Instance*)h)->edges; }
+
+extern "C" int ua_pr_destroy(void* h) {
+    Instance* in = (Instance*)h;
+    if (!in) return 0;
+    pmReleaseMemory(in->graph);
+    pmReleaseMemory(in->state[0]);
+    pmReleaseMemory(in->state[1]);
+    delete in;
+    return 0;
+}

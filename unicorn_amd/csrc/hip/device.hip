@@ -47,6 +47,15 @@ Batch* as_batch(void* stream) {
     return nullptr;
 }
 
+hipStream_t plain_stream_of(void* stream, int* rc) {
+    *rc = UH_OK;
+    if (Batch* b = as_batch(stream)) {
+        *rc = uh_batch_flush(stream, 0);
+        return b->stream;
+    }
+    return (hipStream_t)stream;
+}
+
 void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles) {
     const uint8_t* p = (const uint8_t*)desc;
     b->items[f].insert(b->items[f].end(), p, p + bytes);
@@ -158,9 +167,11 @@ int uh_host_free(void* ptr) { if (ptr) UH_HIP(hipHostFree(ptr)); return UH_OK; }
 
 static int copy_kind(const char* what, void* dst, const void* src, size_t bytes, void* stream,
                      hipMemcpyKind kind) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "%s on a batch handle", what);
     if (!bytes) return UH_OK;
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, (hipStream_t)stream);
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
     if (e != hipSuccess) return check_hip(e, what);
     return UH_OK;
 }
@@ -177,21 +188,25 @@ int uh_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
 
 int uh_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
                  size_t rows, int kind, void* stream) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_memcpy_2d on a batch handle");
     if (!width || !rows) return UH_OK;
     if (width > dpitch || width > spitch) return fail(UH_ERR_ARG, "uh_memcpy_2d: width > pitch");
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
     hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
                     : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    UH_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, k, (hipStream_t)stream));
+    UH_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, k, s));
     return UH_OK;
 }
 
 int uh_fill_u32(void* dst, uint32_t value, size_t n, void* stream) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_fill_u32 on a batch handle");
     if (!n) return UH_OK;
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s,
                        (uint32_t*)dst, value, n);
     UH_LAUNCH_CHECK("k_fill_u32");
     g_plain_launches++;
@@ -272,38 +287,42 @@ int uh_graph_destroy(void* graph_exec) {
 
 int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, int bz,
                      size_t shared_mem, void* stream, void** args) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "kernel-kind callbacks cannot be batched");
     if (gx <= 0 || gy <= 0 || gz <= 0 || bx <= 0 || by <= 0 || bz <= 0 || bx * by * bz > 1024)
         return fail(UH_ERR_ARG, "uh_launch_kernel: bad geometry");
-    UH_HIP(hipLaunchKernel(func, dim3(gx, gy, gz), dim3(bx, by, bz), args, shared_mem,
-                           (hipStream_t)stream));
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
+    UH_HIP(hipLaunchKernel(func, dim3(gx, gy, gz), dim3(bx, by, bz), args, shared_mem, s));
     g_plain_launches++;
     return UH_OK;
 }
 
 int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const uint64_t* dst_off,
                     const uint64_t* len, uint32_t n_records, uint64_t max_len, void* stream) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_copy_records on a batch handle");
     if (!n_records || !max_len) return UH_OK;
     uint64_t chunks = (max_len + 65535) / 65536;
     if (chunks > 65535 || n_records > 0x7fffffffu) return fail(UH_ERR_ARG, "uh_copy_records: too large");
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_copy_records, dim3(n_records, (unsigned)chunks), dim3(256), 0,
-                       (hipStream_t)stream, (uint8_t*)dst, (const uint8_t*)src, src_off, dst_off, len);
+                       s, (uint8_t*)dst, (const uint8_t*)src, src_off, dst_off, len);
     UH_LAUNCH_CHECK("k_copy_records");
     g_plain_launches++;
     return UH_OK;
 }
 
 int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream) {
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_reduce_add_f32 on a batch handle");
     if (!n) return UH_OK;
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
     if (((uintptr_t)dst | (uintptr_t)src) & 15u) return fail(UH_ERR_ARG, "uh_reduce_add_f32: unaligned");
     size_t blocks = (n / 4 + 255) / 256;
     size_t cap = (size_t)cu_count() * 8;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                       dst, src, n);
+    hipLaunchKernelGGL(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
     UH_LAUNCH_CHECK("k_reduce_add_f32");
     g_plain_launches++;
     return UH_OK;

@@ -117,9 +117,10 @@ extern "C" int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, flo
                                   void* stream) {
     if (!n) return UH_OK;
     if (!w || !deg) return fail(UH_ERR_ARG, "uh_pagerank_init_w: null pointer");
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_pagerank_init_w on a batch handle");
-    hipLaunchKernelGGL(k_pagerank_init_w, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                       w, deg, n, x0, damping);
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pagerank_init_w, dim3((n + 255) / 256), dim3(256), 0, s, w, deg, n, x0, damping);
     UH_LAUNCH_CHECK("k_pagerank_init_w");
     g_plain_launches++;
     return UH_OK;
@@ -130,8 +131,10 @@ extern "C" int uh_pagerank_push_ell(const uint32_t* records, uint32_t pages, uin
                                     float* y, void* stream) {
     if (!pages) return UH_OK;
     if (!records || !y || (!first_iteration && !x)) return fail(UH_ERR_ARG, "uh_pagerank_push_ell: null pointer");
-    if (as_batch(stream)) return fail(UH_ERR_BATCH, "uh_pagerank_push_ell on a batch handle");
-    hipLaunchKernelGGL(k_pagerank_push_ell, dim3((pages + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+    int rc;
+    hipStream_t s = plain_stream_of(stream, &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pagerank_push_ell, dim3((pages + 255) / 256), dim3(256), 0, s,
                        records, pages, max_out, x, x0, first_iteration, damping, y);
     UH_LAUNCH_CHECK("k_pagerank_push_ell");
     g_plain_launches++;

@@ -72,6 +72,9 @@ struct Batch {
 };
 
 Batch* as_batch(void* stream);   // nullptr if the stream is not a batch handle
+// Stream for an operation that cannot be recorded as a work item. A batch handle acts as a
+// barrier: the items recorded so far are flushed, then the operation runs on the batch's stream.
+hipStream_t plain_stream_of(void* stream, int* rc);
 void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles);
 
 // Per-family persistent launchers used by uh_batch_flush (queue = device pointer to the

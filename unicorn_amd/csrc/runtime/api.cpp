@@ -328,7 +328,8 @@ pmStatus pmSubmitTask(pmTaskDetails d, pmTaskHandle* th) {
     t->cpu_allowed = cs->cb.subtask_cpu != nullptr;
     t->gpu_allowed = r->gpu_count > 0 && (cs->cb.subtask_gpu_hip || cs->cb.subtask_gpu_custom);
     if (!t->cpu_allowed && !t->gpu_allowed) raise(pmNoCompatibleDevice, "no device can run this task's callbacks");
-    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && r->hip_mode.load() != PM_HIP_MODE_PLAIN;
+    t->hip_mode = r->hip_mode.load();
+    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
         AddressSpace* as = t->spaces[i];

@@ -403,6 +403,7 @@ struct GpuSlot {
     std::vector<DevScatter> dev_scatters;
     // PLAIN: read-only cache entries this subtask reads (pinned until it completes)
     std::vector<std::pair<uint64_t, std::vector<size_t>>> cache_refs;
+    bool plain = false;     // staged the PLAIN way (host gather / scatter)
     bool busy = false;
 };
 
@@ -839,7 +840,7 @@ void Runtime::gpu_loop() {
         for (int i = 0; i < 16; ++i) {
             GpuSlot& s = gs.slots[i];
             if (s.busy && uh_event_query(s.event) == 0) {
-                bool plain = !s.task->gpu_resident || hip_mode.load() == PM_HIP_MODE_PLAIN;
+                const bool plain = s.plain;
                 if (!plain && !s.dev_scatters.empty()) {
                     try { scatter_resident(s.task, s, s.stream); hipchk(uh_stream_sync(s.stream), "scatter"); }
                     catch (Error& e) { fail_task(s.task, e); }
@@ -849,32 +850,33 @@ void Runtime::gpu_loop() {
             }
         }
         retire_batches(false);
-        const int mode = hip_mode.load();
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
         Task* t = nullptr;
         ulong first = 0, count = 0;
+        int mode = PM_HIP_MODE_PLAIN;
         {
             std::unique_lock<std::mutex> g(q_mtx);
-            auto has_work = [&] {
+            auto front = [&]() -> Task* {
                 for (Task* x : active)
-                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return true;
-                return false;
+                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return x;
+                return nullptr;
             };
             if (inflight == 0 && batches.empty()) {
-                q_cv.wait(g, [&] { return stopping || has_work(); });
-                if (stopping && !has_work()) break;
-            } else if (!has_work() || (mode != PM_HIP_MODE_BATCHED && inflight >= depth) ||
-                       (mode == PM_HIP_MODE_BATCHED && (int)batches.size() >= 2)) {
+                q_cv.wait(g, [&] { return stopping || front() != nullptr; });
+                if (stopping && !front()) break;
+            } else if (!front() || inflight >= depth || (int)batches.size() >= 2) {
                 g.unlock();
                 // wait for the oldest work to finish
-                if (mode == PM_HIP_MODE_BATCHED && !batches.empty()) retire_batches(true);
+                if (!batches.empty()) retire_batches(true);
                 else {
                     for (int i = 0; i < 16; ++i)
                         if (gs.slots[i].busy) { uh_event_sync(gs.slots[i].event); break; }
                 }
                 continue;
             }
-            ulong max_take = mode == PM_HIP_MODE_BATCHED ? (ulong)batch_limit.load() : 1;
+            // the mode is the one the task was submitted under
+            mode = front()->hip_mode;
+            ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? (ulong)batch_limit.load() : 1;
             t = pick(true, first, count, max_take);
             if (!t) continue;
         }
@@ -892,7 +894,8 @@ void Runtime::gpu_loop() {
                     Task* t2 = nullptr;
                     // only tasks that can be batched (GPU-resident)
                     for (Task* x : active) {
-                        if (x->gpu_allowed && x->gpu_resident && x->next.load() < x->subtask_count) { t2 = x; break; }
+                        if (x->gpu_allowed && x->gpu_resident && x->hip_mode == PM_HIP_MODE_BATCHED &&
+                            x->next.load() < x->subtask_count) { t2 = x; break; }
                     }
                     if (!t2) break;
                     Task* got = pick(true, f2, c2, (ulong)batch_limit.load() - taken);
@@ -959,7 +962,7 @@ void Runtime::gpu_loop() {
                     GpuSlot& s = gs.slots[k];
                     if (s.busy) {
                         uh_event_sync(s.event);
-                        bool plain = !s.task->gpu_resident || hip_mode.load() == PM_HIP_MODE_PLAIN;
+                        const bool plain = s.plain;
                         if (!plain && !s.dev_scatters.empty()) {
                             try { scatter_resident(s.task, s, s.stream); hipchk(uh_stream_sync(s.stream), "scatter"); }
                             catch (Error& e) { fail_task(s.task, e); }
@@ -980,6 +983,7 @@ void Runtime::gpu_loop() {
             s.event = gpu_events[si_idx];
             s.busy = true;
             const bool plain = !t->gpu_resident || mode == PM_HIP_MODE_PLAIN;
+            s.plain = plain;
             if (t->status.load() == pmSuccess) {
                 try {
                     if (t->callbacks->cb.deviceSelection && !t->callbacks->cb.deviceSelection(t->info(), *gpu_device))

@@ -156,6 +156,7 @@ struct Task {
     bool cpu_allowed = false, gpu_allowed = false;
     bool gpu_resident = false;                   // GPU subtasks use HBM mirrors (GPU-only task)
     bool gpu_prepared = false;                   // mirrors made current by the HIP stub
+    int hip_mode = PM_HIP_MODE_BATCHED;          // HIP stub mode at submission
     double submit_time = 0, start_time = 0, end_time = 0;
     double deadline = 0;
     std::mutex mtx;
