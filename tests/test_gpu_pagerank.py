@@ -66,7 +66,7 @@ def test_pagerank_closed_forms(gpu_runtime):
     rp, col = _csr(n, [(i, (i + 1) % n) for i in range(n)])       # ring
     pr = U.PageRank(rp, col, 300)
     pr.run(3, U.DEVICE_GPU)
-    np.testing.assert_allclose(pr.ranks(), 0.85, rtol=1e-6)
+    np.testing.assert_allclose(pr.ranks(), np.float32(0.85) ** 3, rtol=1e-6)   # x_k = 0.85^k
     pr.close()
     rp, col = _csr(n, [(i, 0) for i in range(1, n)])              # star into page 0
     pr = U.PageRank(rp, col, 256)

@@ -206,7 +206,7 @@ int uh_fill_u32(void* dst, uint32_t value, size_t n, void* stream) {
     if (rc) return rc;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s,
+    UH_LAUNCH(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s,
                        (uint32_t*)dst, value, n);
     UH_LAUNCH_CHECK("k_fill_u32");
     g_plain_launches++;
@@ -305,7 +305,7 @@ int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const u
     int rc;
     hipStream_t s = plain_stream_of(stream, &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_copy_records, dim3(n_records, (unsigned)chunks), dim3(256), 0,
+    UH_LAUNCH(k_copy_records, dim3(n_records, (unsigned)chunks), dim3(256), 0,
                        s, (uint8_t*)dst, (const uint8_t*)src, src_off, dst_off, len);
     UH_LAUNCH_CHECK("k_copy_records");
     g_plain_launches++;
@@ -322,7 +322,7 @@ int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream) {
     size_t cap = (size_t)cu_count() * 8;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    UH_LAUNCH(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
     UH_LAUNCH_CHECK("k_reduce_add_f32");
     g_plain_launches++;
     return UH_OK;

@@ -307,7 +307,7 @@ static int pick_tile(const uh_gemm_args& g) {
 uint32_t gemm_tiles(const uh_gemm_args& g) { return gemm_tiles_for(g, g.precision >> 8); }
 
 int gemm_persistent(const uint8_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_gemm_persistent, dim3(cu_count() * 2), dim3(G_THREADS), 0, s, queue);
+    UH_LAUNCH(k_gemm_persistent, dim3(cu_count() * 2), dim3(G_THREADS), 0, s, queue);
     UH_LAUNCH_CHECK("k_gemm_persistent");
     g_persistent_launches++;
     return UH_OK;
@@ -334,7 +334,7 @@ extern "C" int uh_gemm(const uh_gemm_args* args, void* stream) {
         record_item(b, FAM_GEMM, &item, sizeof(item), tiles);
         return UH_OK;
     }
-    hipLaunchKernelGGL(k_gemm, dim3(tiles), dim3(G_THREADS), 0, (hipStream_t)stream, g, cfg);
+    UH_LAUNCH(k_gemm, dim3(tiles), dim3(G_THREADS), 0, (hipStream_t)stream, g, cfg);
     UH_LAUNCH_CHECK("k_gemm");
     g_plain_launches++;
     return UH_OK;

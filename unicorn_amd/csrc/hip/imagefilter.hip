@@ -305,7 +305,7 @@ uint32_t filter_tiles(const uh_imagefilter_args& a) {
 }
 
 int filter_persistent(const uint8_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_imagefilter_persistent, dim3(cu_count() * 4), dim3(FT_THREADS), 0, s, queue);
+    UH_LAUNCH(k_imagefilter_persistent, dim3(cu_count() * 4), dim3(FT_THREADS), 0, s, queue);
     UH_LAUNCH_CHECK("k_imagefilter_persistent");
     g_persistent_launches++;
     return UH_OK;
@@ -350,7 +350,7 @@ extern "C" int uh_imagefilter(const uh_imagefilter_args* args, void* stream) {
         record_item(b, FAM_FILTER, &a, sizeof(a), tiles);
         return UH_OK;
     }
-    hipLaunchKernelGGL(k_imagefilter, dim3(tiles), dim3(FT_THREADS), 0, (hipStream_t)stream, a);
+    UH_LAUNCH(k_imagefilter, dim3(tiles), dim3(FT_THREADS), 0, (hipStream_t)stream, a);
     UH_LAUNCH_CHECK("k_imagefilter");
     g_plain_launches++;
     return UH_OK;

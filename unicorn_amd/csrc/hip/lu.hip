@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) k_trsm_persistent(const uint8_t* queue) {
 uint32_t trsm_tiles(const TrsmItem& a) { return (uint32_t)((a.m + 31) / 32); }
 
 int trsm_persistent(const uint8_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_trsm_persistent, dim3(cu_count() * 2), dim3(256), 0, s, queue);
+    UH_LAUNCH(k_trsm_persistent, dim3(cu_count() * 2), dim3(256), 0, s, queue);
     UH_LAUNCH_CHECK("k_trsm_persistent");
     g_persistent_launches++;
     return UH_OK;
@@ -322,7 +322,7 @@ extern "C" int uh_lu_panel(float* a, uint64_t lda, int n, void* stream) {
     } else {
         s = (hipStream_t)stream;
     }
-    hipLaunchKernelGGL(k_lu_panel, dim3(1), dim3(1024), 0, s, a, lda, n);
+    UH_LAUNCH(k_lu_panel, dim3(1), dim3(1024), 0, s, a, lda, n);
     UH_LAUNCH_CHECK("k_lu_panel");
     g_plain_launches++;
     return UH_OK;
@@ -341,7 +341,7 @@ static int trsm_common(const float* t, uint64_t ldt, float* x, uint64_t ldx, int
         return UH_OK;
     }
     const size_t shm = (size_t)n * 33 * sizeof(float);
-    hipLaunchKernelGGL(k_trsm, dim3(trsm_tiles(it)), dim3(256), shm, (hipStream_t)stream, it);
+    UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), shm, (hipStream_t)stream, it);
     UH_LAUNCH_CHECK("k_trsm");
     g_plain_launches++;
     return UH_OK;
@@ -383,10 +383,10 @@ extern "C" int uh_residual_f64(const float* a, uint64_t lda, int n, const double
         UH_HIP(hipMalloc(&g_part, need));
         g_part_bytes = need;
     }
-    hipLaunchKernelGGL(k_colblock_gemv<double>, dim3((n + 255) / 256, chunks), dim3(256), 0, s, a, lda, n, 0, n,
+    UH_LAUNCH(k_colblock_gemv<double>, dim3((n + 255) / 256, chunks), dim3(256), 0, s, a, lda, n, 0, n,
                        x, (uint64_t)n, nrhs, g_part);
     UH_LAUNCH_CHECK("k_colblock_gemv");
-    hipLaunchKernelGGL(k_residual_sum, dim3((n + 255) / 256), dim3(256), 0, s, g_part, chunks, n, b, nrhs, r, rf);
+    UH_LAUNCH(k_residual_sum, dim3((n + 255) / 256), dim3(256), 0, s, g_part, chunks, n, b, nrhs, r, rf);
     UH_LAUNCH_CHECK("k_residual_sum");
     g_plain_launches += 2;
     return UH_OK;
@@ -405,7 +405,7 @@ extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, 
         rc = uh_trsm_lln(lu + (uint64_t)b0 * ld + b0, ld, rhs + b0, (uint64_t)n, bn, nrhs, s);
         if (rc) return rc;
         if (b0 + bn < n) {
-            hipLaunchKernelGGL(k_update_f32, dim3((n - b0 - bn + 255) / 256), dim3(256), 0, s, lu, ld, b0 + bn, n,
+            UH_LAUNCH(k_update_f32, dim3((n - b0 - bn + 255) / 256), dim3(256), 0, s, lu, ld, b0 + bn, n,
                                b0, b0 + bn, rhs, (uint64_t)n, nrhs);
             UH_LAUNCH_CHECK("k_update_f32");
             g_plain_launches++;
@@ -417,11 +417,11 @@ extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, 
         const int b0 = max(0, b1 - B);
         const int bn = b1 - b0;
         TrsmItem it{lu + (uint64_t)b0 * ld + b0, rhs + b0, ld, (uint64_t)n, bn, nrhs, 2, 0};
-        hipLaunchKernelGGL(k_trsm, dim3(trsm_tiles(it)), dim3(256), (size_t)bn * 33 * sizeof(float), s, it);
+        UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), (size_t)bn * 33 * sizeof(float), s, it);
         UH_LAUNCH_CHECK("k_trsm");
         g_plain_launches++;
         if (b0 > 0) {
-            hipLaunchKernelGGL(k_update_f32, dim3((b0 + 255) / 256), dim3(256), 0, s, lu, ld, 0, b0, b0, b1, rhs,
+            UH_LAUNCH(k_update_f32, dim3((b0 + 255) / 256), dim3(256), 0, s, lu, ld, 0, b0, b0, b1, rhs,
                                (uint64_t)n, nrhs);
             UH_LAUNCH_CHECK("k_update_f32");
             g_plain_launches++;
@@ -435,7 +435,7 @@ extern "C" int uh_refine_update(double* x, const float* z, int n, int nrhs, void
     hipStream_t s = plain_stream(stream, &rc);
     if (rc) return rc;
     const uint64_t tot = (uint64_t)n * nrhs;
-    hipLaunchKernelGGL(k_refine_update, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, z, n, nrhs);
+    UH_LAUNCH(k_refine_update, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, z, n, nrhs);
     UH_LAUNCH_CHECK("k_refine_update");
     g_plain_launches++;
     return UH_OK;
@@ -445,7 +445,7 @@ extern "C" int uh_colmax_abs_f64(const double* v, int n, int ncols, double* out,
     int rc;
     hipStream_t s = plain_stream(stream, &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_colmax_abs, dim3(ncols), dim3(256), 0, s, v, n, ncols, out);
+    UH_LAUNCH(k_colmax_abs, dim3(ncols), dim3(256), 0, s, v, n, ncols, out);
     UH_LAUNCH_CHECK("k_colmax_abs");
     g_plain_launches++;
     return UH_OK;

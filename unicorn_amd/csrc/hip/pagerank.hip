@@ -87,7 +87,7 @@ uint32_t pagerank_tiles(const uh_pagerank_args& a) {
 }
 
 int pagerank_persistent(const uint8_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_pagerank_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
+    UH_LAUNCH(k_pagerank_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
     UH_LAUNCH_CHECK("k_pagerank_persistent");
     g_persistent_launches++;
     return UH_OK;
@@ -107,7 +107,7 @@ extern "C" int uh_pagerank_pull(const uh_pagerank_args* a, void* stream) {
         record_item(b, FAM_PAGERANK, a, sizeof(*a), tiles);
         return UH_OK;
     }
-    hipLaunchKernelGGL(k_pagerank_pull, dim3(tiles), dim3(PR_THREADS), 0, (hipStream_t)stream, *a);
+    UH_LAUNCH(k_pagerank_pull, dim3(tiles), dim3(PR_THREADS), 0, (hipStream_t)stream, *a);
     UH_LAUNCH_CHECK("k_pagerank_pull");
     g_plain_launches++;
     return UH_OK;
@@ -120,7 +120,7 @@ extern "C" int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, flo
     int rc;
     hipStream_t s = plain_stream_of(stream, &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pagerank_init_w, dim3((n + 255) / 256), dim3(256), 0, s, w, deg, n, x0, damping);
+    UH_LAUNCH(k_pagerank_init_w, dim3((n + 255) / 256), dim3(256), 0, s, w, deg, n, x0, damping);
     UH_LAUNCH_CHECK("k_pagerank_init_w");
     g_plain_launches++;
     return UH_OK;
@@ -134,7 +134,7 @@ extern "C" int uh_pagerank_push_ell(const uint32_t* records, uint32_t pages, uin
     int rc;
     hipStream_t s = plain_stream_of(stream, &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pagerank_push_ell, dim3((pages + 255) / 256), dim3(256), 0, s,
+    UH_LAUNCH(k_pagerank_push_ell, dim3((pages + 255) / 256), dim3(256), 0, s,
                        records, pages, max_out, x, x0, first_iteration, damping, y);
     UH_LAUNCH_CHECK("k_pagerank_push_ell");
     g_plain_launches++;

@@ -22,6 +22,14 @@ int check_hip(hipError_t err, const char* what);
         if (uh_e_ != hipSuccess) return uh::check_hip(uh_e_, #call); \
     } while (0)
 
+// Launch with a clean error slot, so that UH_LAUNCH_CHECK reports this launch only and not an
+// earlier failure that was already returned to its caller.
+#define UH_LAUNCH(...)                                          \
+    do {                                                        \
+        (void)hipGetLastError();                                \
+        hipLaunchKernelGGL(__VA_ARGS__);                        \
+    } while (0)
+
 #define UH_LAUNCH_CHECK(name)                                   \
     do {                                                        \
         hipError_t uh_e_ = hipGetLastError();                   \

@@ -414,6 +414,7 @@ struct CacheEntry {
     size_t bytes = 0;
     uint64_t last_use = 0;
     int users = 0;
+    void* ready = nullptr;   // event recorded after the upload; later users wait on it
 };
 
 struct GpuState {
@@ -432,6 +433,7 @@ void cache_purge(AddressSpace* as, bool all) {
     for (auto it = c.begin(); it != c.end();) {
         if ((all || (as && it->first.first == as->uid)) && it->second.users == 0) {
             Memory::get().device_free(it->second.buf, it->second.bytes);
+            if (it->second.ready) uh_event_destroy(it->second.ready);
             it = c.erase(it);
         } else {
             ++it;
@@ -448,6 +450,7 @@ void cache_shrink(size_t need) {
                 victim = it;
         if (victim == c.end()) return;
         Memory::get().device_free(victim->second.buf, victim->second.bytes);
+        if (victim->second.ready) uh_event_destroy(victim->second.ready);
         c.erase(victim);
     }
 }
@@ -498,7 +501,8 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
     si.memCount = (uint)t->spaces.size();
     si.splitInfo = st.split;
     // gather everything to upload into one pinned buffer
-    struct Up { void* dst; size_t pin_off, len; };
+    struct Up { void* dst; size_t pin_off, len; uint mem; };
+    std::vector<std::pair<uint64_t, std::vector<size_t>>> fresh;   // cache entries uploaded here
     std::vector<Up> ups;
     std::vector<std::pair<size_t, std::pair<const uint8_t*, size_t>>> gathers;  // pin_off <- src,len
     size_t pin_total = 0;
@@ -541,6 +545,8 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
                 dbuf = (uint8_t*)it->second.buf;
                 stat_add(&pmHipStubStats::cache_hits, 1);
                 rsrc.clear();
+                // the upload may still be in flight on another subtask's stream
+                hipchk(uh_stream_wait_event(s.stream, it->second.ready), "cache wait");
             } else {
                 cache_shrink(total);
                 CacheEntry e;
@@ -548,8 +554,10 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
                 e.bytes = total;
                 e.users = 1;
                 e.last_use = ++gstate().clock;
+                hipchk(uh_event_create(&e.ready), "cache event");
                 cache[ck] = e;
                 dbuf = (uint8_t*)e.buf;
+                fresh.push_back(ck);
             }
             s.cache_refs.push_back(ck);
         } else {
@@ -557,7 +565,7 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
         }
         for (size_t r = 0; r < rsrc.size(); ++r) {
             gathers.push_back({pin_total, {as->host + rsrc[r], rlen[r]}});
-            ups.push_back({dbuf + rdst[r], pin_total, rlen[r]});
+            ups.push_back({dbuf + rdst[r], pin_total, rlen[r], i});
             pin_total += rlen[r];
         }
         if (t->writable(i)) {
@@ -592,7 +600,7 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
     // (a COMPACT read-only view becomes a single H2D, as pmCudaInterface.cu:482-487)
     for (size_t u = 0; u < ups.size();) {
         size_t v = u + 1, len = ups[u].len;
-        while (v < ups.size() && ups[v].pin_off == ups[u].pin_off + len &&
+        while (v < ups.size() && ups[v].mem == ups[u].mem && ups[v].pin_off == ups[u].pin_off + len &&
                (uint8_t*)ups[v].dst == (uint8_t*)ups[u].dst + len) {
             len += ups[v].len;
             ++v;
@@ -600,6 +608,7 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
         hipchk(uh_memcpy_h2d(ups[u].dst, pin + ups[u].pin_off, len, s.stream), "H2D staging");
         u = v;
     }
+    for (auto& ck : fresh) hipchk(uh_event_record(gstate().cache[ck].ready, s.stream), "cache event");
     stat_add(&pmHipStubStats::h2d_bytes, pin_total);
     // the D2H commands are issued after the callback (see run_plain)
     s.scatters.clear();
@@ -1001,6 +1010,7 @@ void Runtime::gpu_loop() {
                         for (size_t k = 0; k < s.scatters.size();) {
                             size_t v = k + 1, len = s.scatters[k].len;
                             while (v < s.scatters.size() &&
+                                   s.scatters[v].mem == s.scatters[k].mem &&
                                    s.scatters[v].pinned_off == s.scatters[k].pinned_off + len &&
                                    (uint8_t*)s.dev_scatters[v].buf == (uint8_t*)s.dev_scatters[k].buf + len) {
                                 len += s.scatters[v].len;
