@@ -75,6 +75,10 @@ pmStatus pmPrefetchToDevice(pmMemHandle pMemHandle);
  * read it on the GPU; writes through it must be followed by pmMarkDeviceWritten. */
 pmStatus pmGetDeviceMemPtr(pmMemHandle pMemHandle, void** pDevicePtr);
 pmStatus pmMarkDeviceWritten(pmMemHandle pMemHandle);
+/* Device time of the BATCHED mode's flushes (work-queue upload + persistent kernels), measured
+ * with events on the stream the persistent kernels run on. */
+pmStatus pmSetHipKernelTiming(bool pEnable);
+pmStatus pmGetHipKernelTime(double* pMilliseconds, uint64_t* pFlushes, bool pReset);
 /* Whether a HIP device is available to the runtime. */
 bool pmHasHipDevice();
 

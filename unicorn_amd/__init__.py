@@ -82,6 +82,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_set_batch_limit": (i32, [i32]), "ua_set_hbm_budget": (i32, [C.c_uint64]),
         "ua_get_stats": (i32, [C.POINTER(ua_stats)]), "ua_reset_stats": (i32, []),
         "ua_last_error": (C.c_char_p, []),
+        "ua_set_kernel_timing": (i32, [i32]),
+        "ua_kernel_time": (i32, [C.POINTER(dbl), C.POINTER(C.c_uint64), i32]),
+        "ua_device_sync": (i32, []),
         "ua_if_create": (vp, [C.POINTER(ua_if_config)]), "ua_if_set_input": (i32, [vp, vp]),
         "ua_if_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_if_get_output": (i32, [vp, vp]),
         "ua_if_destroy": (i32, [vp]),
@@ -142,6 +145,21 @@ def stats() -> dict:
 
 def reset_stats() -> None:
     _check(apps_lib().ua_reset_stats(), "reset stats")
+
+
+def set_kernel_timing(on: bool) -> None:
+    _check(apps_lib().ua_set_kernel_timing(1 if on else 0), "pmSetHipKernelTiming")
+
+
+def kernel_time(reset: bool = False):
+    """(milliseconds, flushes) of BATCHED flushes since the last reset, timed on their stream."""
+    ms, n = C.c_double(0), C.c_uint64(0)
+    _check(apps_lib().ua_kernel_time(C.byref(ms), C.byref(n), 1 if reset else 0), "pmGetHipKernelTime")
+    return ms.value, int(n.value)
+
+
+def device_sync() -> None:
+    _check(apps_lib().ua_device_sync(), "device sync")
 
 
 def _ptr(a: np.ndarray) -> int:

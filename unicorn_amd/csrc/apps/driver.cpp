@@ -60,6 +60,14 @@ int ua_get_stats(ua_stats* s) {
 
 int ua_reset_stats(void) { return (int)pmResetHipStubStats(); }
 
+int ua_set_kernel_timing(int on) { return (int)pmSetHipKernelTiming(on != 0); }
+
+int ua_kernel_time(double* ms, uint64_t* flushes, int reset) {
+    return (int)pmGetHipKernelTime(ms, flushes, reset != 0);
+}
+
+int ua_device_sync(void) { return pmHasHipDevice() ? uh_device_sync() : 0; }
+
 const char* ua_last_error(void) {
     static thread_local std::string copy;
     {

@@ -440,6 +440,17 @@ int uh_batch_flush(void* handle, int use_graph) {
         return rc;
     };
     int rc;
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    std::unique_lock<std::mutex> tlock(b->tmtx);
+    if (b->timing) {
+        if (b->spare.empty()) {
+            UH_HIP(hipEventCreate(&ev.first));
+            UH_HIP(hipEventCreate(&ev.second));
+        } else {
+            ev = b->spare.back();
+            b->spare.pop_back();
+        }
+    }
     if (use_graph) {
         hipGraphExec_t& ge = b->graphs[pos][mask];
         if (!ge) {
@@ -453,14 +464,47 @@ int uh_batch_flush(void* handle, int use_graph) {
             hipGraphDestroy(g);
             if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
         }
+        if (ev.first) UH_HIP(hipEventRecord(ev.first, b->stream));
         UH_HIP(hipGraphLaunch(ge, b->stream));
         rc = UH_OK;
     } else {
+        if (ev.first) UH_HIP(hipEventRecord(ev.first, b->stream));
         rc = enqueue(b->stream);
+    }
+    if (ev.first) {
+        UH_HIP(hipEventRecord(ev.second, b->stream));
+        b->timed.push_back(ev);
     }
     UH_HIP(hipEventRecord(b->slot_free[pos], b->stream));
     for (int f = 0; f < FAM_COUNT; ++f) { b->items[f].clear(); b->tiles[f].clear(); }
     return rc;
+}
+
+int uh_batch_set_timing(void* handle, int on) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_set_timing: not a batch");
+    std::lock_guard<std::mutex> tlock(b->tmtx);
+    b->timing = on != 0;
+    return UH_OK;
+}
+
+int uh_batch_timing(void* handle, double* total_ms, uint64_t* flushes, int reset) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_timing: not a batch");
+    std::lock_guard<std::mutex> tlock(b->tmtx);
+    for (auto& ev : b->timed) {
+        UH_HIP(hipEventSynchronize(ev.second));
+        float ms = 0.f;
+        UH_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+        b->timed_ms += ms;
+        b->timed_flushes++;
+        b->spare.push_back(ev);
+    }
+    b->timed.clear();
+    if (total_ms) *total_ms = b->timed_ms;
+    if (flushes) *flushes = b->timed_flushes;
+    if (reset) { b->timed_ms = 0; b->timed_flushes = 0; }
+    return UH_OK;
 }
 
 int uh_launch_counters(uint64_t* plain, uint64_t* persistent, uint64_t* items) {

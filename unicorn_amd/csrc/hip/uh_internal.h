@@ -77,6 +77,12 @@ struct Batch {
     int ring_pos = 0;
     // graphs captured per (ring slot, family mask)
     hipGraphExec_t graphs[RING][1 << FAM_COUNT] = {};
+    // device time of the flushes (events recorded on the batch stream around each flush)
+    bool timing = false;
+    std::mutex tmtx;                 // guards timed / spare / the totals
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
+    double timed_ms = 0;
+    uint64_t timed_flushes = 0;
 };
 
 Batch* as_batch(void* stream);   // nullptr if the stream is not a batch handle

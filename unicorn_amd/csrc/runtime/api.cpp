@@ -549,4 +549,16 @@ pmStatus pmMarkDeviceWritten(pmMemHandle h) {
 }
 bool pmHasHipDevice() { Runtime* r = Runtime::get(); return r && r->gpu_count > 0; }
 
+pmStatus pmSetHipKernelTiming(bool on) {
+    Runtime* r = Runtime::get();
+    if (!r || !r->gpu_count) return pmNoCompatibleDevice;
+    return uh_batch_set_timing(r->batch, on ? 1 : 0) ? pmGraphicsCardError : pmSuccess;
+}
+
+pmStatus pmGetHipKernelTime(double* ms, uint64_t* flushes, bool reset) {
+    Runtime* r = Runtime::get();
+    if (!r || !r->gpu_count) return pmNoCompatibleDevice;
+    return uh_batch_timing(r->batch, ms, flushes, reset ? 1 : 0) ? pmGraphicsCardError : pmSuccess;
+}
+
 }  // namespace pm
