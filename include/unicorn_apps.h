@@ -87,6 +87,18 @@ int ua_pr_get_ranks(void* h, float* ranks);
 uint64_t ua_pr_edges(void* h);
 int ua_pr_destroy(void* h);
 
+/* ---- luDecomposition (reference testSuite/luDecomposition) + refinement solve ----
+ * n x n float32 column-major, blocks of `block` (<= 512); precision 0 = split-BF16 trailing
+ * update, 2 = fp32. ua_lu_solve: b, x n x nrhs float64 column-major (nrhs <= 8); info[0] = steps,
+ * info[1] = final normwise backward error, info[2] = after the first step, info[3] = ||A||inf. */
+void* ua_lu_create(int n, int block, int precision);
+int ua_lu_set_matrix(void* h, const float* A);
+int ua_lu_factor(void* h, int device, double* seconds);
+int ua_lu_get_factors(void* h, float* LU);
+int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, int device, double* x, double* info,
+                double* seconds);
+int ua_lu_destroy(void* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,10 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_pr_run": (i32, [vp, i32, i32, C.POINTER(dbl)]),
         "ua_pr_get_ranks": (i32, [vp, vp]), "ua_pr_edges": (C.c_uint64, [vp]),
         "ua_pr_destroy": (i32, [vp]),
+        "ua_lu_create": (vp, [i32, i32, i32]), "ua_lu_set_matrix": (i32, [vp, vp]),
+        "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
+        "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
+        "ua_lu_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -251,6 +255,54 @@ class PageRank:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_pr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LU:
+    """luDecomposition (blocked, 3 tasks per block step) and the refinement solve."""
+
+    def __init__(self, n: int, block: int = 512, precision: int = 0):
+        init()
+        self.n = n
+        self._h = apps_lib().ua_lu_create(n, block, precision)
+        if not self._h:
+            _check(-1, "ua_lu_create")
+
+    def set_matrix(self, A_colmajor: np.ndarray) -> None:
+        a = np.asfortranarray(A_colmajor, dtype=np.float32)
+        assert a.shape == (self.n, self.n)
+        _check(apps_lib().ua_lu_set_matrix(self._h, a.ctypes.data), "ua_lu_set_matrix")
+
+    def factor(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_lu_factor(self._h, device, C.byref(t)), "luDecomposition tasks")
+        return t.value
+
+    def factors(self) -> np.ndarray:
+        out = np.empty((self.n, self.n), dtype=np.float32, order="F")
+        _check(apps_lib().ua_lu_get_factors(self._h, out.ctypes.data), "ua_lu_get_factors")
+        return out
+
+    def solve(self, b: np.ndarray, max_steps: int = 5, device: int = DEVICE_GPU):
+        b = np.asfortranarray(b, dtype=np.float64)
+        nrhs = b.shape[1]
+        x = np.empty_like(b, order="F")
+        info = (C.c_double * 4)()
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_lu_solve(self._h, b.ctypes.data, nrhs, max_steps, device, x.ctypes.data, info,
+                                      C.byref(t)), "refinement solve")
+        return x, {"steps": int(info[0]), "backward_error": info[1], "backward_error_first": info[2],
+                   "a_inf": info[3], "seconds": t.value}
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_lu_destroy(self._h)
             self._h = None
 
     def __del__(self):
