@@ -366,7 +366,7 @@ int uh_batch_pending(void* handle, int* n) {
 }
 
 static const size_t kItemBytes[FAM_COUNT] = {
-    sizeof(uh_imagefilter_args), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem)};
+    sizeof(FilterItem), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem)};
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

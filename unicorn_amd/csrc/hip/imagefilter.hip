@@ -22,6 +22,8 @@
 // Radii 1 and 2 take this path; other radii use a direct per-pixel kernel.
 #include "uh_internal.h"
 
+#include <cstring>
+
 namespace uh {
 
 static constexpr int FT_W = 256;      // output tile width (pixels)
@@ -108,43 +110,45 @@ __device__ __forceinline__ void filter_stage(const uh_imagefilter_args& a, const
 // Weight vectors: pixel x+q of lane l (x = x0 + 4l, LDS dword l+1 holds pixels x..x+3) needs
 // LDS columns 4l+4+q-R .. 4l+4+q+R. Column c (relative to 4l) lives in dword c/4, byte c%4;
 // for R <= 2 the span touches dwords d0 = (4+q-R)/4 and d0+1. wv[dy][q][0] / [1] pack the
-// taps of filter row dy that fall into those two dwords (zero bytes elsewhere).
-template <int R>
-__device__ __forceinline__ void filter_weights(const uh_imagefilter_args& a,
-                                               uint32_t (&wv)[2 * R + 1][4][2], int& wsum) {
-    constexpr int D = 2 * R + 1;
+// taps of filter row dy that fall into those two dwords (zero bytes elsewhere). They are packed
+// once per work item on the host (pack_filter_weights) and read as uniform values per tile.
+static void pack_filter_weights(const int8_t* w, int R, uint32_t* wv, int* wsum) {
+    const int D = 2 * R + 1;
     int s = 0;
-#pragma unroll
     for (int dy = 0; dy < D; ++dy) {
-#pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c0 = 4 + q - R;          // first LDS column (relative to 4l)
             const int d0 = c0 / 4;             // first dword (relative to l)
             uint32_t w0 = 0, w1 = 0;
-#pragma unroll
             for (int dx = 0; dx < D; ++dx) {
                 const int c = c0 + dx;
-                const uint32_t wb = (uint32_t)(uint8_t)a.weights[dy * D + dx];
+                const uint32_t wb = (uint32_t)(uint8_t)w[dy * D + dx];
                 if (c / 4 == d0) w0 |= wb << ((c % 4) * 8);
                 else w1 |= wb << ((c % 4) * 8);
             }
-            wv[dy][q][0] = w0;
-            wv[dy][q][1] = w1;
+            wv[(dy * 4 + q) * 2 + 0] = w0;
+            wv[(dy * 4 + q) * 2 + 1] = w1;
         }
-#pragma unroll
-        for (int dx = 0; dx < D; ++dx) s += a.weights[dy * D + dx];
+        for (int dx = 0; dx < D; ++dx) s += w[dy * D + dx];
     }
-    wsum = s;
+    *wsum = s;
 }
 
 template <int R>
-__device__ __forceinline__ void filter_compute(const uh_imagefilter_args& a, int x0, int y0,
+__device__ __forceinline__ void filter_compute(const FilterItem& it, int x0, int y0,
                                                const uint32_t* lds) {
     constexpr int D = 2 * R + 1;
     constexpr int LROWS = FT_H + 2 * R;
+    const uh_imagefilter_args& a = it.a;
     uint32_t wv[D][4][2];
-    int wsum;
-    filter_weights<R>(a, wv, wsum);
+#pragma unroll
+    for (int dy = 0; dy < D; ++dy)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            wv[dy][q][0] = it.wv[(dy * 4 + q) * 2 + 0];
+            wv[dy][q][1] = it.wv[(dy * 4 + q) * 2 + 1];
+        }
+    const int wsum = it.wsum;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -220,7 +224,8 @@ __device__ __forceinline__ void filter_compute(const uh_imagefilter_args& a, int
 }
 
 template <int R>
-__device__ __forceinline__ void filter_tile(const uh_imagefilter_args& a, uint32_t tile, uint32_t* lds) {
+__device__ __forceinline__ void filter_tile(const FilterItem& it, uint32_t tile, uint32_t* lds) {
+    const uh_imagefilter_args& a = it.a;
     int x0, y0;
     filter_tile_origin(a, tile, x0, y0);
     FilterView v;
@@ -231,7 +236,7 @@ __device__ __forceinline__ void filter_tile(const uh_imagefilter_args& a, uint32
     v.first_row = a.row0 - a.radius < 0 ? 0 : a.row0 - a.radius;
     filter_stage<R>(a, v, x0, y0, lds);
     __syncthreads();
-    filter_compute<R>(a, x0, y0, lds);
+    filter_compute<R>(it, x0, y0, lds);
 }
 
 // Direct kernel for any radius <= 15: one output pixel per thread.
@@ -268,24 +273,24 @@ __device__ __forceinline__ void filter_tile_direct(const uh_imagefilter_args& a,
     }
 }
 
-__device__ __forceinline__ void filter_dispatch(const uh_imagefilter_args& a, uint32_t tile, uint32_t* lds) {
-    if (a.radius == 2) filter_tile<2>(a, tile, lds);
-    else if (a.radius == 1) filter_tile<1>(a, tile, lds);
-    else filter_tile_direct(a, tile);
+__device__ __forceinline__ void filter_dispatch(const FilterItem& it, uint32_t tile, uint32_t* lds) {
+    if (it.a.radius == 2) filter_tile<2>(it, tile, lds);
+    else if (it.a.radius == 1) filter_tile<1>(it, tile, lds);
+    else filter_tile_direct(it.a, tile);
 }
 
 static constexpr int FT_LDS_DWORDS = 3 * (FT_H + 2 * 2) * FT_LWD;
 
-__global__ void __launch_bounds__(FT_THREADS) k_imagefilter(uh_imagefilter_args a) {
+__global__ void __launch_bounds__(FT_THREADS) k_imagefilter(FilterItem it) {
     __shared__ uint32_t lds[FT_LDS_DWORDS];
-    filter_dispatch(a, blockIdx.x, lds);
+    filter_dispatch(it, blockIdx.x, lds);
 }
 
 __global__ void __launch_bounds__(FT_THREADS) k_imagefilter_persistent(const uint8_t* queue) {
     __shared__ uint32_t lds[FT_LDS_DWORDS];
     __shared__ uint32_t s_tile;
     QueueHeader* h = (QueueHeader*)queue;
-    const uh_imagefilter_args* items = (const uh_imagefilter_args*)(queue + h->items_offset);
+    const FilterItem* items = (const FilterItem*)(queue + h->items_offset);
     const uint32_t* prefix = (const uint32_t*)(queue + h->prefix_offset);
     const uint32_t total = h->total_tiles, n_items = h->n_items;
     for (;;) {
@@ -300,8 +305,8 @@ __global__ void __launch_bounds__(FT_THREADS) k_imagefilter_persistent(const uin
     }
 }
 
-uint32_t filter_tiles(const uh_imagefilter_args& a) {
-    return (uint32_t)(((a.cols + FT_W - 1) / FT_W) * ((a.rows + FT_H - 1) / FT_H));
+uint32_t filter_tiles(const FilterItem& it) {
+    return (uint32_t)(((it.a.cols + FT_W - 1) / FT_W) * ((it.a.rows + FT_H - 1) / FT_H));
 }
 
 int filter_persistent(const uint8_t* queue, hipStream_t s) {
@@ -343,14 +348,17 @@ extern "C" int uh_imagefilter(const uh_imagefilter_args* args, void* stream) {
     int vcols = 0;
     int rc = filter_validate(args, &vcols);
     if (rc) return rc;
-    uh_imagefilter_args a = *args;
-    a.pad = vcols;   // the kernels read the view width from pad
-    const uint32_t tiles = filter_tiles(a);
+    FilterItem it;
+    memset(&it, 0, sizeof(it));
+    it.a = *args;
+    it.a.pad = vcols;   // the kernels read the view width from pad
+    if (it.a.radius <= 2) pack_filter_weights(it.a.weights, it.a.radius, it.wv, &it.wsum);
+    const uint32_t tiles = filter_tiles(it);
     if (Batch* b = as_batch(stream)) {
-        record_item(b, FAM_FILTER, &a, sizeof(a), tiles);
+        record_item(b, FAM_FILTER, &it, sizeof(it), tiles);
         return UH_OK;
     }
-    UH_LAUNCH(k_imagefilter, dim3(tiles), dim3(FT_THREADS), 0, (hipStream_t)stream, a);
+    UH_LAUNCH(k_imagefilter, dim3(tiles), dim3(FT_THREADS), 0, (hipStream_t)stream, it);
     UH_LAUNCH_CHECK("k_imagefilter");
     g_plain_launches++;
     return UH_OK;

@@ -45,6 +45,14 @@ int cu_count();
 // Kernel families that a batch can hold. Each family records fixed-size descriptors.
 enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_COUNT = 4 };
 
+// Work item of the filter family: the launcher's arguments (pad = view width in pixels) plus the
+// dot4 weight vectors of radius <= 2, packed once on the host (wv[dy][q][d], see imagefilter.hip).
+struct FilterItem {
+    uh_imagefilter_args a;
+    uint32_t wv[5 * 4 * 2];
+    int32_t wsum, pad[3];
+};
+
 // Work item of the TRSM family (both triangular kinds of the LU stage 2).
 struct TrsmItem {
     const float* tri; float* x;
@@ -99,7 +107,7 @@ int pagerank_persistent(const uint8_t* queue, hipStream_t s);
 int trsm_persistent(const uint8_t* queue, hipStream_t s);
 
 // Tile counts of one item of each family (host side, for the queue prefix).
-uint32_t filter_tiles(const uh_imagefilter_args& a);
+uint32_t filter_tiles(const FilterItem& it);
 uint32_t gemm_tiles(const uh_gemm_args& a);
 uint32_t pagerank_tiles(const uh_pagerank_args& a);
 uint32_t trsm_tiles(const TrsmItem& a);

@@ -362,7 +362,7 @@ void Runtime::cpu_loop(int idx) {
                     for (uint i = 0; i < t->spaces.size(); ++i) {
                         if (!t->writable(i) || si.memInfo[i].visibilityType != SUBSCRIPTION_COMPACT) continue;
                         CompactLayout cl = make_compact(st->mem[i]);
-                        for (auto& kv : st->mem[i].write.map()) {
+                        for (auto& kv : st->mem[i].write().map()) {
                             size_t co = cl.compact_offset(kv.first);
                             memcpy(t->spaces[i]->host + kv.first, compact[i].data() + co, kv.second);
                         }
@@ -523,11 +523,11 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
             pmSubscriptionInfo u = t->read_only(i) ? ms.cread : ms.unified();
             base_off = u.offset;
             total = u.length;
-            for (auto& kv : ms.read.map()) { rsrc.push_back(kv.first); rdst.push_back(kv.first - base_off); rlen.push_back(kv.second); }
+            for (auto& kv : ms.read().map()) { rsrc.push_back(kv.first); rdst.push_back(kv.first - base_off); rlen.push_back(kv.second); }
         } else {
             cl = make_compact(ms);
             total = cl.total;
-            for (auto& kv : ms.read.map()) { rsrc.push_back(kv.first); rdst.push_back(cl.compact_offset(kv.first)); rlen.push_back(kv.second); }
+            for (auto& kv : ms.read().map()) { rsrc.push_back(kv.first); rdst.push_back(cl.compact_offset(kv.first)); rlen.push_back(kv.second); }
         }
         if (!total) { si.memInfo[i] = mi; continue; }
         uint8_t* dbuf = nullptr;
@@ -535,7 +535,7 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
             std::vector<size_t> key;
             key.push_back(natural ? 1 : 0);
             key.push_back((size_t)as->version.load());
-            for (auto& kv : ms.read.map()) { key.push_back(kv.first); key.push_back(kv.second); }
+            for (auto& kv : ms.read().map()) { key.push_back(kv.first); key.push_back(kv.second); }
             auto& cache = gstate().cache;
             auto ck = std::make_pair(as->uid, key);
             auto it = cache.find(ck);
@@ -569,7 +569,7 @@ pmSubtaskInfo stage_plain(Task* t, GpuSlot& s) {
             pin_total += rlen[r];
         }
         if (t->writable(i)) {
-            for (auto& kv : ms.write.map()) {
+            for (auto& kv : ms.write().map()) {
                 size_t doff = natural ? kv.first - base_off : cl.compact_offset(kv.first);
                 pend.push_back({i, kv.first, kv.second, dbuf + doff});
                 scatter_total += kv.second;
@@ -707,7 +707,7 @@ void scatter_resident(Task* t, GpuSlot& s, void* stream) {
         MemSubscription* ms = nullptr;
         for (uint i = 0; i < t->spaces.size(); ++i) if (t->spaces[i] == d.as) ms = &s.st->mem[i];
         if (!ms) continue;
-        for (auto& kv : ms->write.map()) {
+        for (auto& kv : ms->write().map()) {
             size_t co = d.cl.compact_offset(kv.first);
             hipchk(uh_memcpy_d2d(d.as->dev + kv.first, (uint8_t*)d.buf + co, kv.second, stream), "device scatter");
             stat_add(&pmHipStubStats::d2d_bytes, kv.second);

@@ -50,13 +50,23 @@ private:
     std::map<size_t, size_t> m_;
 };
 
+// Subscriptions of one subtask to one address space. The subscriptions are kept as given
+// (contiguous and scattered) together with their consolidated extents; the per-record union
+// (what the reference materialises row by row in a map) is built only when a COMPACT view or
+// record-level staging needs it — NATURAL views of HBM mirrors only use the extents.
 struct MemSubscription {
-    IntervalSet read, write;
+    std::vector<pmSubscriptionInfo> rlin, wlin;
+    std::vector<pmScatteredSubscriptionInfo> rsc, wsc;
     pmSubscriptionInfo cread{0, 0}, cwrite{0, 0};     // consolidated extents
     bool has_read = false, has_write = false;
     void add(pmSubscriptionType type, size_t offset, size_t length);
     void add_scattered(pmSubscriptionType type, const pmScatteredSubscriptionInfo& s);
     pmSubscriptionInfo unified() const;               // min offset / max span of read and write
+    const IntervalSet& read() const;                  // union of read records (built lazily)
+    const IntervalSet& write() const;                 // union of write records (built lazily)
+private:
+    mutable bool built_r_ = false, built_w_ = false;
+    mutable IntervalSet read_, write_;
 };
 
 // COMPACT view geometry: union of read and write records packed back to back in address order

@@ -25,6 +25,13 @@ namespace rt {
 void IntervalSet::add(size_t offset, size_t length) {
     if (!length) return;
     size_t lo = offset, hi = offset + length;
+    // records usually arrive in ascending address order: append or extend the last interval
+    if (!m_.empty()) {
+        auto last = std::prev(m_.end());
+        const size_t lend = last->first + last->second;
+        if (lo > lend) { m_.emplace_hint(m_.end(), lo, hi - lo); return; }
+        if (lo >= last->first) { if (hi > lend) last->second = hi - last->first; return; }
+    }
     // first interval that could overlap or touch [lo, hi): the last one starting <= lo
     auto it = m_.upper_bound(lo);
     if (it != m_.begin()) {
@@ -59,12 +66,14 @@ static void extend(pmSubscriptionInfo& c, bool& has, size_t off, size_t span) {
 void MemSubscription::add(pmSubscriptionType type, size_t offset, size_t length) {
     if (!length) return;
     if (type == READ_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION) {
-        read.add(offset, length);
+        rlin.emplace_back(offset, length);
         extend(cread, has_read, offset, length);
+        built_r_ = false;
     }
     if (type == WRITE_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION) {
-        write.add(offset, length);
+        wlin.emplace_back(offset, length);
         extend(cwrite, has_write, offset, length);
+        built_w_ = false;
     }
 }
 
@@ -72,12 +81,30 @@ void MemSubscription::add_scattered(pmSubscriptionType type, const pmScatteredSu
     if (!s.size || !s.count) return;
     const bool r = (type == READ_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION);
     const bool w = (type == WRITE_SUBSCRIPTION || type == READ_WRITE_SUBSCRIPTION);
-    for (size_t i = 0; i < s.count; ++i) {
-        if (r) read.add(s.offset + i * s.step, s.size);
-        if (w) write.add(s.offset + i * s.step, s.size);
+    // consolidated extent of a scattered subscription: step * count bytes from its offset
+    // (reference pmSubscriptionManager.cpp:508-523)
+    if (r) { rsc.push_back(s); extend(cread, has_read, s.offset, s.step * s.count); built_r_ = false; }
+    if (w) { wsc.push_back(s); extend(cwrite, has_write, s.offset, s.step * s.count); built_w_ = false; }
+}
+
+static void build_union(IntervalSet& u, const std::vector<pmSubscriptionInfo>& lin,
+                        const std::vector<pmScatteredSubscriptionInfo>& sc) {
+    u = IntervalSet();
+    for (const auto& l : lin) u.add(l.offset, l.length);
+    for (const auto& s : sc) {
+        if (s.size == s.step) { u.add(s.offset, s.size * s.count); continue; }   // adjacent records
+        for (size_t i = 0; i < s.count; ++i) u.add(s.offset + i * s.step, s.size);
     }
-    if (r) extend(cread, has_read, s.offset, s.step * s.count);
-    if (w) extend(cwrite, has_write, s.offset, s.step * s.count);
+}
+
+const IntervalSet& MemSubscription::read() const {
+    if (!built_r_) { build_union(read_, rlin, rsc); built_r_ = true; }
+    return read_;
+}
+
+const IntervalSet& MemSubscription::write() const {
+    if (!built_w_) { build_union(write_, wlin, wsc); built_w_ = true; }
+    return write_;
 }
 
 pmSubscriptionInfo MemSubscription::unified() const {
@@ -99,8 +126,8 @@ size_t CompactLayout::compact_offset(size_t addr) const {
 }
 
 CompactLayout make_compact(const MemSubscription& s) {
-    IntervalSet u = s.read;
-    for (auto& kv : s.write.map()) u.add(kv.first, kv.second);
+    IntervalSet u = s.read();
+    for (auto& kv : s.write().map()) u.add(kv.first, kv.second);
     CompactLayout c;
     size_t pos = 0;
     for (auto& kv : u.map()) {
@@ -110,13 +137,13 @@ CompactLayout make_compact(const MemSubscription& s) {
         pos += kv.second;
     }
     c.total = pos;
-    if (!s.read.empty()) {
+    if (!s.read().empty()) {
         c.has_read = true;
-        c.first_read = c.compact_offset(s.read.map().begin()->first);
+        c.first_read = c.compact_offset(s.read().map().begin()->first);
     }
-    if (!s.write.empty()) {
+    if (!s.write().empty()) {
         c.has_write = true;
-        c.first_write = c.compact_offset(s.write.map().begin()->first);
+        c.first_write = c.compact_offset(s.write().map().begin()->first);
     }
     return c;
 }
