@@ -61,7 +61,9 @@ pmStatus pmSetHipGraphs(bool pEnable);
 /* Maximum subtasks in flight on the HIP stub (PLAIN/RESIDENT) and per batch (BATCHED). */
 pmStatus pmSetHipPipelineDepth(int pDepth);
 pmStatus pmSetHipBatchLimit(int pSubtasks);
-/* Bytes of HBM the runtime may hold for mirrors and subtask buffers. */
+/* Bytes of HBM the runtime may hold for mirrors and subtask buffers (0 = the default chosen at
+ * initialisation: PMLIB_HBM_BUDGET_MB, else 85 % of the free HBM). GPU-resident tasks are admitted
+ * in order while their address spaces fit; mirrors no admitted task uses are evicted LRU. */
 pmStatus pmSetHbmBudget(size_t pBytes);
 size_t pmGetHbmBudget();
 size_t pmGetHbmResident();

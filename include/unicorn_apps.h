@@ -59,6 +59,8 @@ void* ua_if_create(const ua_if_config* cfg);
 int ua_if_set_input(void* h, const uint8_t* image);
 int ua_if_run(void* h, int device, double* seconds);
 int ua_if_get_output(void* h, uint8_t* out);
+void* ua_if_clone(void* h);                  /* same input image, own output */
+int ua_if_fetch(void* h, uint64_t* checksum); /* output to the host; checksum may be NULL */
 int ua_if_destroy(void* h);
 
 /* ---- matrixMultiply (reference testSuite/matrixMultiplyBlas decomposition, plain CPU method) ----
@@ -67,6 +69,8 @@ void* ua_mm_create(int n, int block, int precision);
 int ua_mm_set_inputs(void* h, const float* A, const float* B);
 int ua_mm_run(void* h, int device, double* seconds);
 int ua_mm_get_output(void* h, float* C);
+void* ua_mm_clone(void* h);                  /* same A and B, own C */
+int ua_mm_fetch(void* h, uint64_t* checksum);
 int ua_mm_destroy(void* h);
 
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
@@ -85,6 +89,8 @@ void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* 
 int ua_pr_run(void* h, int iterations, int device, double* seconds);
 int ua_pr_get_ranks(void* h, float* ranks);
 uint64_t ua_pr_edges(void* h);
+void* ua_pr_clone(void* h);                  /* same graph, own rank state */
+int ua_pr_fetch(void* h, uint64_t* checksum);
 int ua_pr_destroy(void* h);
 
 /* ---- luDecomposition (reference testSuite/luDecomposition) + refinement solve ----
@@ -98,6 +104,21 @@ int ua_lu_get_factors(void* h, float* LU);
 int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, int device, double* x, double* info,
                 double* seconds);
 int ua_lu_destroy(void* h);
+
+/* ---- serving (no reference counterpart: the request loop of the served mode) ----
+ * A catalogue of datasets (ua_serve_add_* return a dataset id, or -1) and a stream of requests
+ * (dataset id, arrival time in seconds from the start, sorted). `slots` worker threads serve the
+ * stream in arrival order; a request runs one task of its dataset's app on a free clone (pageRank:
+ * init + iterations), then, if `fetch`, brings its output to the host (and hashes it into
+ * checksums[i] when checksums is not NULL). latency_ms[i] = completion - arrival. Returns 0 or
+ * the first failing request's status. Outputs are FNV-1a-64 hashed 8 bytes at a time. */
+void* ua_serve_create(void);
+int ua_serve_add_mm(void* s, int n, int block, int precision, uint64_t seed);
+int ua_serve_add_if(void* s, int side, int radius, int mode, int shift, int tile, uint64_t seed);
+int ua_serve_add_pr(void* s, uint32_t pages, uint32_t pages_per_subtask, int iterations, uint64_t seed);
+int ua_serve_run(void* s, int n, const int* dataset, const double* arrival_s, int slots, int device,
+                 int fetch, double* latency_ms, uint64_t* checksums, double* wall_s);
+int ua_serve_destroy(void* s);
 
 #ifdef __cplusplus
 }

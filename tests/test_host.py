@@ -95,3 +95,22 @@ def test_gpu_only_task_without_device_fails_cleanly(runtime):
             f.run(U.DEVICE_GPU)
     finally:
         f.close()
+
+
+def test_serving_alone_equals_mix_cpu(runtime):
+    """Each request's output is the same served alone (one slot, back to back) and in a Poisson
+    mix of concurrent requests over shared resident inputs (CPU stubs)."""
+    U = runtime
+    S = U.Server()
+    try:
+        cat = {"mm": [(64, S.add_mm(64, 32, seed=1)), (128, S.add_mm(128, 64, seed=2))],
+               "if": [(64, S.add_if(64, tile=32, seed=3)), (130, S.add_if(130, tile=64, seed=4))],
+               "pr": [(2000, S.add_pr(2000, 700, 4, seed=5)), (5000, S.add_pr(5000, 2000, 4, seed=6))]}
+        ds, arr = U.serving_stream(7, 48, 3000.0, cat)
+        _, alone, _ = S.run(ds, np.zeros(len(ds)), 1, U.DEVICE_CPU, checksums=True)
+        lat, mix, wall = S.run(ds, arr, 6, U.DEVICE_CPU, checksums=True)
+    finally:
+        S.close()
+    assert np.array_equal(alone, mix)
+    assert np.all(lat >= 0) and wall > 0
+    assert len(set(ds.tolist())) >= 4          # the stream mixes classes and sizes

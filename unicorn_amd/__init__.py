@@ -102,6 +102,14 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
         "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
         "ua_lu_destroy": (i32, [vp]),
+        "ua_mm_clone": (vp, [vp]), "ua_if_clone": (vp, [vp]), "ua_pr_clone": (vp, [vp]),
+        "ua_mm_fetch": (i32, [vp, vp]), "ua_if_fetch": (i32, [vp, vp]), "ua_pr_fetch": (i32, [vp, vp]),
+        "ua_serve_create": (vp, []),
+        "ua_serve_add_mm": (i32, [vp, i32, i32, i32, C.c_uint64]),
+        "ua_serve_add_if": (i32, [vp, i32, i32, i32, i32, i32, C.c_uint64]),
+        "ua_serve_add_pr": (i32, [vp, C.c_uint32, C.c_uint32, i32, C.c_uint64]),
+        "ua_serve_run": (i32, [vp, i32, vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
+        "ua_serve_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -139,6 +147,11 @@ def set_hip_mode(mode: int, compact: bool = False, graphs: bool = True) -> None:
     _check(lib.ua_set_hip_mode(mode), "pmSetHipStubMode")
     _check(lib.ua_set_hip_compact(1 if compact else 0), "pmSetHipOptimalView")
     _check(lib.ua_set_hip_graphs(1 if graphs else 0), "pmSetHipGraphs")
+
+
+def set_hbm_budget(nbytes: int) -> None:
+    """HBM budget of the runtime in bytes (0 = the default)."""
+    _check(apps_lib().ua_set_hbm_budget(nbytes), "ua_set_hbm_budget")
 
 
 def stats() -> dict:
@@ -348,3 +361,87 @@ class MatMul:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Serving driver over a catalogue of resident datasets (ua_serve_* of libunicorn_apps.so)."""
+
+    def __init__(self):
+        self._h = apps_lib().ua_serve_create()
+        self.kinds: list[tuple] = []
+
+    def _add(self, ds: int, what: str, desc: tuple) -> int:
+        if ds < 0:
+            _check(-1, what)
+        self.kinds.append(desc)
+        return ds
+
+    def add_mm(self, n: int, block: int = 512, precision: int = 0, seed: int = 1) -> int:
+        return self._add(apps_lib().ua_serve_add_mm(self._h, n, min(block, n), precision, seed),
+                         "ua_serve_add_mm", ("mm", n))
+
+    def add_if(self, side: int, radius: int = 2, mode: int = 1, shift: int = 4, tile: int = 2048,
+               seed: int = 1) -> int:
+        return self._add(apps_lib().ua_serve_add_if(self._h, side, radius, mode, shift, tile, seed),
+                         "ua_serve_add_if", ("if", side))
+
+    def add_pr(self, pages: int, pages_per_subtask: int = 1_000_000, iterations: int = 20,
+               seed: int = 1) -> int:
+        return self._add(apps_lib().ua_serve_add_pr(self._h, pages, pages_per_subtask, iterations, seed),
+                         "ua_serve_add_pr", ("pr", pages))
+
+    def run(self, dataset: np.ndarray, arrival_s: np.ndarray, slots: int, device: int = DEVICE_GPU,
+            fetch: bool = True, checksums: bool = False):
+        """Serve the request stream; returns (latency_ms, checksums or None, wall seconds)."""
+        ds = np.ascontiguousarray(dataset, dtype=np.int32)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        n = len(ds)
+        lat = np.zeros(n, dtype=np.float64)
+        cs = np.zeros(n, dtype=np.uint64) if checksums else None
+        wall = C.c_double(0.0)
+        _check(apps_lib().ua_serve_run(self._h, n, _ptr(ds), _ptr(arr), slots, device, 1 if fetch else 0,
+                                       _ptr(lat), _ptr(cs) if cs is not None else None, C.byref(wall)),
+               "serving run")
+        return lat, cs, wall.value
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serving_stream(seed: int, n: int, rate_per_s: float, catalogue: dict, mix=(1, 1, 1)):
+    """Seeded request stream of the serving mix: Poisson arrivals at `rate_per_s` (0 = all at
+    t = 0) and a class per request drawn with weights `mix` over (matrixMultiply, imageFiltering,
+    pageRank). Sizes follow the mix's distributions: matrixMultiply N log-uniform, imageFiltering
+    side uniform, pageRank pages uniform; each is snapped to the nearest size of `catalogue`
+    ({"mm": [(N, ds)], "if": [(side, ds)], "pr": [(pages, ds)]}, nearest in log scale for N).
+    Returns (dataset ids, arrival seconds)."""
+    rng = np.random.default_rng(seed)
+    classes = [k for k in ("mm", "if", "pr")]
+    w = np.asarray(mix, dtype=np.float64)
+    w = w / w.sum()
+    ds = np.empty(n, dtype=np.int32)
+    for i in range(n):
+        k = classes[rng.choice(3, p=w)]
+        entries = catalogue[k]
+        sizes = np.array([e[0] for e in entries], dtype=np.float64)
+        lo, hi = sizes.min(), sizes.max()
+        if k == "mm":
+            v = np.exp(rng.uniform(np.log(lo), np.log(hi)))
+            j = int(np.argmin(np.abs(np.log(sizes) - np.log(v))))
+        else:
+            v = rng.uniform(lo, hi)
+            j = int(np.argmin(np.abs(sizes - v)))
+        ds[i] = entries[j][1]
+    if rate_per_s > 0:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, n))
+    else:
+        arrivals = np.zeros(n)
+    return ds, arrivals

@@ -6,8 +6,10 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "pmPublicDefinitions.h"
 #include "pmPublicUtilities.h"
@@ -55,6 +57,43 @@ inline int run_task(pmTaskDetails& d, double* secs) {
     pmReleaseTask(th);
     if (secs) *secs = t1 - t0;
     return (int)s;
+}
+
+// Input address spaces shared by an app instance and its clones (a clone has its own outputs);
+// they are released together with the last holder.
+struct SharedMem {
+    std::vector<pmMemHandle> handles;
+    ~SharedMem() {
+        for (pmMemHandle h : handles)
+            if (h) pmReleaseMemory(h);
+    }
+};
+
+// 64-bit FNV-1a over 8-byte words (then the tail bytes): a cheap content hash of task outputs.
+inline uint64_t hash_bytes(const void* p, size_t n) {
+    const uint8_t* b = (const uint8_t*)p;
+    uint64_t h = 0xcbf29ce484222325ull;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, b + i, 8);
+        h = (h ^ w) * 0x100000001b3ull;
+    }
+    for (; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+// Brings an output address space to the host (pmFetchMemory) and hashes it when asked.
+inline int fetch_output(pmMemHandle m, size_t bytes, uint64_t* checksum) {
+    pmStatus s = pmFetchMemory(m);
+    if (s != pmSuccess) return (int)s;
+    if (checksum) {
+        pmRawMemPtr p;
+        s = pmGetRawMemPtr(m, &p);
+        if (s != pmSuccess) return (int)s;
+        *checksum = hash_bytes(p, bytes);
+    }
+    return 0;
 }
 
 void set_error(const std::string& s);

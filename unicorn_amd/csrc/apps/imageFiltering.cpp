@@ -173,6 +173,7 @@ static pmStatus imageFilter_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, v
 
 struct Instance {
     imageFilterTaskConf conf;
+    std::shared_ptr<apps::SharedMem> inputs;   // the input image, shared with clones
     pmMemHandle in = nullptr, out = nullptr;
     size_t bytes = 0;
     unsigned long subtasks = 0;
@@ -217,8 +218,28 @@ extern "C" void* ua_if_create(const ua_if_config* cfg) {
     in->bytes = (size_t)cfg->width * cfg->height * PIXEL_COUNT;
     const unsigned long tx = (cfg->width + cfg->tile - 1) / cfg->tile, ty = (cfg->height + cfg->tile - 1) / cfg->tile;
     in->subtasks = tx * ty;
-    if (pmCreateMemory2D(cfg->height, (size_t)cfg->width * PIXEL_COUNT, &in->in) != pmSuccess ||
-        pmCreateMemory2D(cfg->height, (size_t)cfg->width * PIXEL_COUNT, &in->out) != pmSuccess) {
+    in->inputs = std::make_shared<apps::SharedMem>();
+    if (pmCreateMemory2D(cfg->height, (size_t)cfg->width * PIXEL_COUNT, &in->in) != pmSuccess) in->in = nullptr;
+    in->inputs->handles.push_back(in->in);
+    if (!in->in || pmCreateMemory2D(cfg->height, (size_t)cfg->width * PIXEL_COUNT, &in->out) != pmSuccess) {
+        apps::set_error(std::string("create memory: ") + pmGetLastError());
+        delete in;
+        return nullptr;
+    }
+    return in;
+}
+
+// A second instance over the same input image with its own output.
+extern "C" void* ua_if_clone(void* h) {
+    const Instance* src = (const Instance*)h;
+    if (!src) return nullptr;
+    Instance* in = new Instance();
+    in->conf = src->conf;
+    in->bytes = src->bytes;
+    in->subtasks = src->subtasks;
+    in->inputs = src->inputs;
+    in->in = src->in;
+    if (pmCreateMemory2D(src->conf.imageHeight, src->conf.imageWidth * PIXEL_COUNT, &in->out) != pmSuccess) {
         apps::set_error(std::string("create memory: ") + pmGetLastError());
         delete in;
         return nullptr;
@@ -257,11 +278,15 @@ extern "C" int ua_if_get_output(void* h, uint8_t* out) {
     return 0;
 }
 
+extern "C" int ua_if_fetch(void* h, uint64_t* checksum) {
+    Instance* in = (Instance*)h;
+    return apps::fetch_output(in->out, in->bytes, checksum);
+}
+
 extern "C" int ua_if_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;
-    pmReleaseMemory(in->in);
     pmReleaseMemory(in->out);
-    delete in;
+    delete in;    // the input goes with the last holder of `inputs`
     return 0;
 }

@@ -100,6 +100,7 @@ static pmStatus mm_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* stre
 
 struct Instance {
     matrixMultiplyTaskConf conf;
+    std::shared_ptr<apps::SharedMem> inputs;   // A and B, shared with clones
     pmMemHandle a = nullptr, b = nullptr, c = nullptr;
     unsigned long subtasks = 0;
 };
@@ -133,8 +134,31 @@ extern "C" void* ua_mm_create(int n, int block, int precision) {
     in->conf.precision = precision;
     in->subtasks = (unsigned long)(n / block) * (n / block);
     const size_t row = (size_t)n * sizeof(float);
-    if (pmCreateMemory2D(n, row, &in->a) != pmSuccess || pmCreateMemory2D(n, row, &in->b) != pmSuccess ||
-        pmCreateMemory2D(n, row, &in->c) != pmSuccess) {
+    in->inputs = std::make_shared<apps::SharedMem>();
+    if (pmCreateMemory2D(n, row, &in->a) != pmSuccess) in->a = nullptr;
+    in->inputs->handles.push_back(in->a);
+    if (in->a && pmCreateMemory2D(n, row, &in->b) != pmSuccess) in->b = nullptr;
+    in->inputs->handles.push_back(in->b);
+    if (!in->a || !in->b || pmCreateMemory2D(n, row, &in->c) != pmSuccess) {
+        apps::set_error(std::string("create memory: ") + pmGetLastError());
+        delete in;
+        return nullptr;
+    }
+    return in;
+}
+
+// A second instance over the same A and B with its own C (for concurrent tasks on one input).
+extern "C" void* ua_mm_clone(void* h) {
+    const Instance* src = (const Instance*)h;
+    if (!src) return nullptr;
+    Instance* in = new Instance();
+    in->conf = src->conf;
+    in->subtasks = src->subtasks;
+    in->inputs = src->inputs;
+    in->a = src->a;
+    in->b = src->b;
+    const size_t n = src->conf.matrixDim;
+    if (pmCreateMemory2D(n, n * sizeof(float), &in->c) != pmSuccess) {
         apps::set_error(std::string("create memory: ") + pmGetLastError());
         delete in;
         return nullptr;
@@ -177,12 +201,15 @@ extern "C" int ua_mm_get_output(void* h, float* C) {
     return 0;
 }
 
+extern "C" int ua_mm_fetch(void* h, uint64_t* checksum) {
+    Instance* in = (Instance*)h;
+    return apps::fetch_output(in->c, in->conf.matrixDim * in->conf.matrixDim * sizeof(float), checksum);
+}
+
 extern "C" int ua_mm_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;
-    pmReleaseMemory(in->a);
-    pmReleaseMemory(in->b);
     pmReleaseMemory(in->c);
-    delete in;
+    delete in;    // A and B go with the last holder of `inputs`
     return 0;
 }

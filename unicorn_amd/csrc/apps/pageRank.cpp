@@ -175,7 +175,8 @@ struct Instance {
     uint32_t pages = 0, per_subtask = 0;
     uint64_t edges = 0;
     float damping = 0.85f;
-    std::vector<uint32_t> in_row_ptr;   // host copy for subscription sizing
+    std::shared_ptr<std::vector<uint32_t>> in_row_ptr;   // host copy for subscription sizing
+    std::shared_ptr<apps::SharedMem> inputs;             // the graph, shared with clones
     pmMemHandle graph = nullptr, state[2] = {nullptr, nullptr};
     int current = 0;                    // state holding the latest ranks
 };
@@ -191,7 +192,7 @@ static int run_kind(Instance* in, int kind, int device, double* secs) {
     conf.dampingFactor = in->damping;
     conf.initialPageRank = 1.0f;
     conf.kind = kind;
-    conf.hostRowPtr = in->in_row_ptr.data();
+    conf.hostRowPtr = in->in_row_ptr->data();
     const int src = in->current, dst = kind == 0 ? in->current : 1 - in->current;
     pmTaskMem mem[3];
     mem[GRAPH_MEM_INDEX] = pmTaskMem(in->graph, READ_ONLY, SUBSCRIPTION_OPTIMAL);
@@ -233,10 +234,13 @@ extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const
     in->edges = out_row_ptr[pages];
     const size_t N = pages;
     const size_t graph_bytes = (2 * N + 1 + in->edges) * 4;
-    if (pmCreateMemory(graph_bytes, &in->graph) != pmSuccess ||
-        pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
+    in->inputs = std::make_shared<apps::SharedMem>();
+    if (pmCreateMemory(graph_bytes, &in->graph) != pmSuccess) in->graph = nullptr;
+    in->inputs->handles.push_back(in->graph);
+    if (!in->graph || pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
         pmCreateMemory(2 * N * 4, &in->state[1]) != pmSuccess) {
         apps::set_error(std::string("create memory: ") + pmGetLastError());
+        if (in->state[0]) pmReleaseMemory(in->state[0]);
         delete in;
         return nullptr;
     }
@@ -256,7 +260,30 @@ extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const
         deg[s] = out_row_ptr[s + 1] - out_row_ptr[s];
         for (uint32_t e = out_row_ptr[s]; e < out_row_ptr[s + 1]; ++e) col[pos[out_col[e]]++] = s;
     }
-    in->in_row_ptr.assign(rp, rp + N + 1);
+    in->in_row_ptr = std::make_shared<std::vector<uint32_t>>(rp, rp + N + 1);
+    return in;
+}
+
+// A second instance over the same graph with its own rank state.
+extern "C" void* ua_pr_clone(void* h) {
+    const Instance* src = (const Instance*)h;
+    if (!src) return nullptr;
+    Instance* in = new Instance();
+    in->pages = src->pages;
+    in->per_subtask = src->per_subtask;
+    in->edges = src->edges;
+    in->damping = src->damping;
+    in->in_row_ptr = src->in_row_ptr;
+    in->inputs = src->inputs;
+    in->graph = src->graph;
+    const size_t N = src->pages;
+    if (pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
+        pmCreateMemory(2 * N * 4, &in->state[1]) != pmSuccess) {
+        apps::set_error(std::string("create memory: ") + pmGetLastError());
+        if (in->state[0]) pmReleaseMemory(in->state[0]);
+        delete in;
+        return nullptr;
+    }
     return in;
 }
 
@@ -287,12 +314,17 @@ extern "C" int ua_pr_get_ranks(void* h, float* y) {
     return 0;
 }
 
+// Fetches the latest state and hashes its ranks (the first N floats).
+extern "C" int ua_pr_fetch(void* h, uint64_t* checksum) {
+    Instance* in = (Instance*)h;
+    return apps::fetch_output(in->state[in->current], (size_t)in->pages * 4, checksum);
+}
+
 extern "C" uint64_t ua_pr_edges(void* h) { return ((Instance*)h)->edges; }
 
 extern "C" int ua_pr_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;
-    pmReleaseMemory(in->graph);
     pmReleaseMemory(in->state[0]);
     pmReleaseMemory(in->state[1]);
     delete in;

@@ -499,7 +499,10 @@ pmStatus pmSetHipBatchLimit(int n) {
     r->batch_limit = n;
     return pmSuccess;
 }
-pmStatus pmSetHbmBudget(size_t b) { Memory::get().set_budget(b); return pmSuccess; }
+pmStatus pmSetHbmBudget(size_t b) {
+    Memory::get().set_budget(b ? b : Memory::get().default_budget);
+    return pmSuccess;
+}
 size_t pmGetHbmBudget() { return Memory::get().budget(); }
 size_t pmGetHbmResident() { return Memory::get().resident(); }
 

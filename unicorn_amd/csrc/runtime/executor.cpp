@@ -101,6 +101,7 @@ void Runtime::init() {
         int mb = env_int("PMLIB_HBM_BUDGET_MB", 0);
         if (mb > 0) budget = (size_t)mb << 20;
         Memory::get().set_budget(budget);
+        Memory::get().default_budget = budget;
         for (int i = 0; i < 16; ++i) {
             uh_stream_create(&r->gpu_streams[i]);
             uh_event_create(&r->gpu_events[i]);
@@ -184,6 +185,7 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
     for (auto it = active.begin(); it != active.end(); ++it) {
         Task* t = *it;
         if (gpu ? !t->gpu_allowed : !t->cpu_allowed) continue;
+        if (gpu && t->gpu_resident && !t->gpu_prepared) continue;   // not admitted yet
         if (t->deadline > 0) {
             if (now == 0) now = now_secs();
             if (now > t->deadline) {
@@ -213,6 +215,16 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
         return t;
     }
     return nullptr;
+}
+
+void Runtime::cancel_locked(Task* t, pmStatus status) {
+    ulong f = t->next.exchange(t->subtask_count);
+    if (f >= t->subtask_count) return;
+    int expected = pmSuccess;
+    t->status.compare_exchange_strong(expected, (int)status);
+    auto it = std::find(active.begin(), active.end(), t);
+    if (it != active.end()) active.erase(it);
+    finish_subtasks(t, t->subtask_count - f);
 }
 
 void Runtime::finish_subtasks(Task* t, ulong n) {
@@ -309,10 +321,13 @@ void Runtime::complete(Task* t) {
         t->status.compare_exchange_strong(expected, (int)e.status);
         set_last_error(e.what);
     }
+    const bool pinned = t->gpu_resident && t->gpu_prepared;
     for (AddressSpace* as : t->spaces) {
         std::lock_guard<std::mutex> g(as->mtx);
         as->users--;
+        if (pinned) as->gpu_pins--;
     }
+    if (pinned) gpu_admitted--;
     if (t->callbacks->cb.taskCompletionCallback) t->callbacks->cb.taskCompletionCallback(t->info());
     std::lock_guard<std::mutex> g(t->mtx);
     t->end_time = now_secs();
@@ -820,8 +835,48 @@ void Runtime::gpu_loop() {
         if (t->gpu_resident) {
             for (AddressSpace* as : t->spaces) Memory::get().ensure_device(as, copy_stream);
             hipchk(uh_stream_sync(copy_stream), "mirror upload");
+            for (AddressSpace* as : t->spaces) {
+                std::lock_guard<std::mutex> g(as->mtx);
+                as->gpu_pins++;
+            }
+            gpu_admitted++;
         }
         t->gpu_prepared = true;
+    };
+
+    // Admission of GPU-resident tasks, in queue order: a task is admitted (its mirrors made
+    // current and pinned) once its address spaces fit the HBM budget, evicting mirrors that no
+    // admitted task uses. A task that does not fit holds back the tasks behind it until admitted
+    // work completes; with nothing admitted it can never fit and fails with pmMemoryError.
+    auto admit_queued = [&]() {
+        std::vector<Task*> cand;
+        {
+            std::lock_guard<std::mutex> g(q_mtx);
+            for (Task* x : active)
+                if (x->gpu_allowed && x->gpu_resident && !x->gpu_prepared && x->next.load() < x->subtask_count)
+                    cand.push_back(x);
+        }
+        for (Task* x : cand) {
+            bool ok = false;
+            try {
+                ok = Memory::get().admit(x->spaces);
+                if (ok) ensure_task_ready(x);
+            } catch (Error& e) {
+                fail_task(x, e);
+                std::lock_guard<std::mutex> g(q_mtx);
+                cancel_locked(x, e.status);
+                continue;
+            }
+            if (!ok) {
+                if (gpu_admitted.load() == 0) {
+                    set_last_error("task address spaces exceed the HBM budget");
+                    std::lock_guard<std::mutex> g(q_mtx);
+                    cancel_locked(x, pmMemoryError);
+                    continue;
+                }
+                break;
+            }
+        }
     };
 
     auto retire_batches = [&](bool block) {
@@ -859,20 +914,29 @@ void Runtime::gpu_loop() {
             }
         }
         retire_batches(false);
+        admit_queued();
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
         Task* t = nullptr;
         ulong first = 0, count = 0;
         int mode = PM_HIP_MODE_PLAIN;
         {
             std::unique_lock<std::mutex> g(q_mtx);
+            auto pending = [&]() -> bool {
+                for (Task* x : active)
+                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return true;
+                return false;
+            };
+            // first task whose subtasks can be staged now (admitted, or staged per subtask)
             auto front = [&]() -> Task* {
                 for (Task* x : active)
-                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return x;
+                    if (x->gpu_allowed && x->next.load() < x->subtask_count && (!x->gpu_resident || x->gpu_prepared))
+                        return x;
                 return nullptr;
             };
             if (inflight == 0 && batches.empty()) {
-                q_cv.wait(g, [&] { return stopping || front() != nullptr; });
-                if (stopping && !front()) break;
+                q_cv.wait(g, [&] { return stopping || pending(); });
+                if (stopping && !pending()) break;
+                if (!front()) continue;   // admission runs first
             } else if (!front() || inflight >= depth || (int)batches.size() >= 2) {
                 g.unlock();
                 // wait for the oldest work to finish
@@ -903,7 +967,7 @@ void Runtime::gpu_loop() {
                     Task* t2 = nullptr;
                     // only tasks that can be batched (GPU-resident)
                     for (Task* x : active) {
-                        if (x->gpu_allowed && x->gpu_resident && x->hip_mode == PM_HIP_MODE_BATCHED &&
+                        if (x->gpu_allowed && x->gpu_resident && x->gpu_prepared && x->hip_mode == PM_HIP_MODE_BATCHED &&
                             x->next.load() < x->subtask_count) { t2 = x; break; }
                     }
                     if (!t2) break;

@@ -10,6 +10,7 @@
 #include "pm_internal.h"
 #include "unicorn_hip.h"
 
+#include <algorithm>
 #include <cstring>
 #include <cstdlib>
 
@@ -77,20 +78,24 @@ bool Memory::try_reserve(size_t bytes) {
 
 void Memory::unreserve(size_t bytes) { resident_ -= bytes; }
 
-void Memory::evict_idle(size_t need) {
-    // Evict least-recently-used mirrors that no running task holds.
+void Memory::evict_idle(size_t need, const std::vector<AddressSpace*>* keep) {
+    // Evict least-recently-used mirrors that no GPU-admitted task uses. A queued task that is
+    // not admitted yet re-uploads its spaces on admission; the host copy is made current first.
+    auto kept = [&](AddressSpace* as) {
+        return keep && std::find(keep->begin(), keep->end(), as) != keep->end();
+    };
     while (resident_.load() + need > budget_) {
         AddressSpace* victim = nullptr;
         {
             std::lock_guard<std::mutex> g(list_mtx);
             for (AddressSpace* as : spaces) {
-                if (!as->dev || as->users > 0) continue;
+                if (!as->dev || as->gpu_pins > 0 || kept(as)) continue;
                 if (!victim || as->last_use < victim->last_use) victim = as;
             }
         }
         if (!victim) return;
         std::lock_guard<std::mutex> g(victim->mtx);
-        if (!victim->dev || victim->users > 0) continue;
+        if (!victim->dev || victim->gpu_pins > 0) continue;
         if (victim->dev_valid && !victim->host_valid) {
             uh_memcpy_d2h(victim->host, victim->dev, victim->length, nullptr);
             uh_stream_sync(nullptr);
@@ -102,6 +107,18 @@ void Memory::evict_idle(size_t need) {
         Runtime* r = Runtime::get();
         if (r) { std::lock_guard<std::mutex> s(r->stats_mtx); r->stats.evictions++; }
     }
+}
+
+bool Memory::admit(const std::vector<AddressSpace*>& req) {
+    size_t need = 0;
+    for (size_t i = 0; i < req.size(); ++i) {
+        if (std::find(req.begin(), req.begin() + i, req[i]) != req.begin() + i) continue;   // listed twice
+        std::lock_guard<std::mutex> g(req[i]->mtx);
+        if (!req[i]->dev) need += req[i]->length ? req[i]->length : 1;
+    }
+    if (resident_.load() + need <= budget_) return true;
+    evict_idle(need, &req);
+    return resident_.load() + need <= budget_;
 }
 
 void* Memory::device_alloc(size_t bytes) {

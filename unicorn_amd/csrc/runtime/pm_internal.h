@@ -91,6 +91,7 @@ struct AddressSpace {
     bool host_valid = true;
     bool dev_valid = false;
     int users = 0;                      // tasks currently holding it
+    int gpu_pins = 0;                   // GPU-admitted tasks using the mirror (not evictable)
     uint64_t last_use = 0;
     uint64_t uid = 0;                   // unique id (never reused)
     std::atomic<uint64_t> version{0};   // bumped on every write; keys the PLAIN-mode cache
@@ -113,11 +114,16 @@ public:
     size_t budget() const { return budget_; }
     size_t resident() const { return resident_.load(); }
     void set_budget(size_t b) { budget_ = b; }
+    size_t default_budget = 0;          // the budget chosen at initialisation
     bool try_reserve(size_t bytes);
     void unreserve(size_t bytes);
     void* device_alloc(size_t bytes);   // budgeted hipMalloc; evicts idle mirrors on pressure
     void device_free(void* p, size_t bytes);
-    void evict_idle(size_t need);
+    // Evicts mirrors that no GPU-admitted task uses (LRU) until `need` more bytes fit; the
+    // spaces in `keep` are never evicted.
+    void evict_idle(size_t need, const std::vector<AddressSpace*>* keep = nullptr);
+    // Whether the missing mirrors of `spaces` fit the budget (after evicting idle mirrors).
+    bool admit(const std::vector<AddressSpace*>& spaces);
     std::mutex list_mtx;
     std::vector<AddressSpace*> spaces;
 private:
@@ -232,10 +238,13 @@ public:
     void* batch = nullptr;
     void* batch_stream = nullptr;
     void* copy_stream = nullptr;
+    std::atomic<int> gpu_admitted{0};    // GPU-resident tasks admitted and not yet complete
     // common helpers
     Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count);
     void finish_subtasks(Task* t, ulong n);
     void complete(Task* t);
+    // Cancels the unpicked subtasks of a queued task with `status` (caller holds q_mtx).
+    void cancel_locked(Task* t, pmStatus status);
 private:
     static Runtime* inst_;
 };
