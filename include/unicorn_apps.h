@@ -99,6 +99,7 @@ int ua_pr_destroy(void* h);
  * info[1] = final normwise backward error, info[2] = after the first step, info[3] = ||A||inf. */
 void* ua_lu_create(int n, int block, int precision);
 int ua_lu_set_matrix(void* h, const float* A);
+int ua_lu_prefetch(void* h);                 /* matrix + pristine copy to HBM (no-op without GPU) */
 int ua_lu_factor(void* h, int device, double* seconds);
 int ua_lu_get_factors(void* h, float* LU);
 int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, int device, double* x, double* info,

@@ -146,9 +146,15 @@ typedef struct uh_gemm_args {
     float alpha, beta;
 } uh_gemm_args;
 int uh_gemm(const uh_gemm_args* args, void* stream);
-/* Tile configuration chosen by the autotuner for a shape class (see uh_gemm_tune). */
+/* Tile configuration for a shape class (log2 buckets of M, N, K) and precision. */
 int uh_gemm_set_tile(int M, int N, int K, int precision, int tile_id);
 int uh_gemm_tile_count(int* n);
+/* Autotuner (no reference counterpart; the reference calls cuBLAS): times every tile
+ * configuration on an M x N x K product in scratch device memory (`reps` launches each, after one
+ * warm-up) on a private stream, records the fastest for the shape class with uh_gemm_set_tile,
+ * and returns it (and its time) when the pointers are non-NULL. Synchronous. A shape class that
+ * was already tuned is not timed again unless `force`. */
+int uh_gemm_autotune(int M, int N, int K, int precision, int reps, int force, int* tile_id, float* ms);
 
 /* ---- LU (column-major, no pivoting, reference convention: L lower with the pivots on its
  *      diagonal, U unit upper; luDecomposition.cpp:116-125) ---- */

@@ -51,7 +51,7 @@ def test_serving_hbm_budget_admission(gpu_runtime):
         ds, arr = U.serving_stream(77, 40, 4000.0, cat)
         U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
         _, want, _ = S.run(ds, np.zeros(len(ds)), 1, U.DEVICE_GPU, checksums=True)
-        U.set_hbm_budget(96 << 20)
+        U.set_hbm_budget(48 << 20)     # the largest request needs 25 MB; all datasets ~70 MB
         _, got, _ = S.run(ds, arr, 8, U.DEVICE_GPU, checksums=True)
         st = U.stats()
     finally:

@@ -101,7 +101,7 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_lu_create": (vp, [i32, i32, i32]), "ua_lu_set_matrix": (i32, [vp, vp]),
         "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
         "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
-        "ua_lu_destroy": (i32, [vp]),
+        "ua_lu_destroy": (i32, [vp]), "ua_lu_prefetch": (i32, [vp]),
         "ua_mm_clone": (vp, [vp]), "ua_if_clone": (vp, [vp]), "ua_pr_clone": (vp, [vp]),
         "ua_mm_fetch": (i32, [vp, vp]), "ua_if_fetch": (i32, [vp, vp]), "ua_pr_fetch": (i32, [vp, vp]),
         "ua_serve_create": (vp, []),
@@ -291,6 +291,9 @@ class LU:
         a = np.asfortranarray(A_colmajor, dtype=np.float32)
         assert a.shape == (self.n, self.n)
         _check(apps_lib().ua_lu_set_matrix(self._h, a.ctypes.data), "ua_lu_set_matrix")
+
+    def prefetch(self) -> None:
+        _check(apps_lib().ua_lu_prefetch(self._h), "ua_lu_prefetch")
 
     def factor(self, device: int = DEVICE_GPU) -> float:
         t = C.c_double(0.0)

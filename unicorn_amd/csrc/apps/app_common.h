@@ -15,6 +15,9 @@
 #include "pmPublicUtilities.h"
 #include "pmExtensions.h"
 #include "unicorn_apps.h"
+#include "unicorn_hip.h"
+
+#include <cstdlib>
 
 namespace apps {
 
@@ -94,6 +97,15 @@ inline int fetch_output(pmMemHandle m, size_t bytes, uint64_t* checksum) {
         *checksum = hash_bytes(p, bytes);
     }
     return 0;
+}
+
+// Picks the GEMM tile configuration for a split-BF16 product shape by timing (once per shape
+// class); UA_GEMM_AUTOTUNE=0 keeps the size-based default.
+inline void autotune_gemm(int M, int N, int K, int precision) {
+    if (precision != 0 || !pmHasHipDevice()) return;
+    const char* e = getenv("UA_GEMM_AUTOTUNE");
+    if (e && atoi(e) == 0) return;
+    uh_gemm_autotune(M, N, K, 0, 5, 0, nullptr, nullptr);
 }
 
 void set_error(const std::string& s);

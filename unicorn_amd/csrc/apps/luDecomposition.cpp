@@ -395,6 +395,7 @@ extern "C" void* ua_lu_create(int n, int block, int precision) {
         return nullptr;
     }
     (void)bytes;
+    apps::autotune_gemm(block, block, block, precision);   // the stage-3 trailing-update shape
     return in;
 }
 
@@ -417,6 +418,15 @@ extern "C" int ua_lu_set_matrix(void* h, const float* A) {
     }
     in->a_inf = m;
     return 0;
+}
+
+// Makes the matrix and its pristine copy resident in HBM ahead of the factorisation.
+extern "C" int ua_lu_prefetch(void* h) {
+    Instance* in = (Instance*)h;
+    if (!pmHasHipDevice()) return 0;
+    pmStatus s = pmPrefetchToDevice(in->mat);
+    if (s == pmSuccess) s = pmPrefetchToDevice(in->orig);
+    return (int)s;
 }
 
 // Blocked factorisation: 3 tasks per block step (reference luDecomposition.cpp:466-480).

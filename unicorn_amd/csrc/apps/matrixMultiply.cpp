@@ -144,6 +144,7 @@ extern "C" void* ua_mm_create(int n, int block, int precision) {
         delete in;
         return nullptr;
     }
+    apps::autotune_gemm(block, block, n, precision);         // one subtask's product shape
     return in;
 }
 

@@ -6,6 +6,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <mutex>
 
 namespace uh {
@@ -56,8 +57,10 @@ hipStream_t plain_stream_of(void* stream, int* rc) {
     return (hipStream_t)stream;
 }
 
-void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles) {
+void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles,
+                 uint32_t kind_bits) {
     const uint8_t* p = (const uint8_t*)desc;
+    b->kinds[f] |= kind_bits;
     b->items[f].insert(b->items[f].end(), p, p + bytes);
     b->tiles[f].push_back(tiles);
 }
@@ -332,6 +335,9 @@ int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream) {
 
 int uh_batch_create(void** handle, void* stream) {
     if (as_batch(stream)) return fail(UH_ERR_ARG, "uh_batch_create: stream is a batch");
+    // device queries used by the persistent launchers, answered here and not during a capture
+    (void)cu_count();
+    (void)filter_wg_per_cu();
     Batch* b = new Batch();
     b->stream = (hipStream_t)stream;
     b->slot_bytes = 0;
@@ -344,8 +350,8 @@ int uh_batch_destroy(void* handle) {
     if (!b) return fail(UH_ERR_ARG, "uh_batch_destroy: not a batch");
     hipStreamSynchronize(b->stream);
     for (int i = 0; i < Batch::RING; ++i) {
-        for (int m = 0; m < (1 << FAM_COUNT); ++m)
-            if (b->graphs[i][m]) hipGraphExecDestroy(b->graphs[i][m]);
+        for (auto& kv : b->graphs[i]) hipGraphExecDestroy(kv.second);
+        b->graphs[i].clear();
         if (b->host_slot[i]) hipHostFree(b->host_slot[i]);
         if (b->dev_slot[i]) hipFree(b->dev_slot[i]);
         if (b->slot_free[i]) hipEventDestroy(b->slot_free[i]);
@@ -393,8 +399,8 @@ int uh_batch_flush(void* handle, int use_graph) {
     if (slot > b->slot_bytes) {
         hipStreamSynchronize(b->stream);
         for (int i = 0; i < Batch::RING; ++i) {
-            for (int m = 0; m < (1 << FAM_COUNT); ++m)
-                if (b->graphs[i][m]) { hipGraphExecDestroy(b->graphs[i][m]); b->graphs[i][m] = nullptr; }
+            for (auto& kv : b->graphs[i]) hipGraphExecDestroy(kv.second);
+            b->graphs[i].clear();
             if (b->host_slot[i]) hipHostFree(b->host_slot[i]);
             if (b->dev_slot[i]) hipFree(b->dev_slot[i]);
             b->host_slot[i] = nullptr; b->dev_slot[i] = nullptr;
@@ -427,13 +433,20 @@ int uh_batch_flush(void* handle, int use_graph) {
         h->total_tiles = acc;
         g_items += n;
     }
+    // Bytes to upload: up to the end of the last family present. A captured graph copies a
+    // power-of-two bucket of that (at least 4 KB, at most the slot), and the bucket is part of
+    // the graph's key, so one capture serves every batch of that family mix and size class.
+    int last = 0;
+    for (int f = 0; f < FAM_COUNT; ++f)
+        if ((mask >> f) & 1) last = f;
+    size_t used = (size_t)last * family_stride + need_family[last];
+    int bucket = 12;
+    while (((size_t)1 << bucket) < used) ++bucket;
+    if (use_graph) used = std::min(b->slot_bytes, (size_t)1 << bucket);
     auto enqueue = [&](hipStream_t s) -> int {
-        // Only the bytes in use are copied unless a graph is being captured (graphs copy the
-        // whole slot so one capture serves every batch shape).
-        size_t bytes = use_graph ? b->slot_bytes : (size_t)(FAM_COUNT - 1) * family_stride + need_family[FAM_COUNT - 1];
-        UH_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+        UH_HIP(hipMemcpyAsync(dev, host, used, hipMemcpyHostToDevice, s));
         int rc = UH_OK;
-        if ((mask >> FAM_FILTER) & 1) rc |= filter_persistent(dev + FAM_FILTER * family_stride, s);
+        if ((mask >> FAM_FILTER) & 1) rc |= filter_persistent(dev + FAM_FILTER * family_stride, s, b->kinds[FAM_FILTER]);
         if ((mask >> FAM_GEMM) & 1) rc |= gemm_persistent(dev + FAM_GEMM * family_stride, s);
         if ((mask >> FAM_PAGERANK) & 1) rc |= pagerank_persistent(dev + FAM_PAGERANK * family_stride, s);
         if ((mask >> FAM_TRSM) & 1) rc |= trsm_persistent(dev + FAM_TRSM * family_stride, s);
@@ -452,7 +465,9 @@ int uh_batch_flush(void* handle, int use_graph) {
         }
     }
     if (use_graph) {
-        hipGraphExec_t& ge = b->graphs[pos][mask];
+        const uint32_t key = (uint32_t)mask | ((uint32_t)bucket << 12) |
+            (((mask >> FAM_FILTER) & 1) ? (uint32_t)filter_variant_for(b->kinds[FAM_FILTER]) << 8 : 0u);
+        hipGraphExec_t& ge = b->graphs[pos][key];
         if (!ge) {
             UH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             rc = enqueue(b->stream);
@@ -476,7 +491,7 @@ int uh_batch_flush(void* handle, int use_graph) {
         b->timed.push_back(ev);
     }
     UH_HIP(hipEventRecord(b->slot_free[pos], b->stream));
-    for (int f = 0; f < FAM_COUNT; ++f) { b->items[f].clear(); b->tiles[f].clear(); }
+    for (int f = 0; f < FAM_COUNT; ++f) { b->items[f].clear(); b->tiles[f].clear(); b->kinds[f] = 0; }
     return rc;
 }
 

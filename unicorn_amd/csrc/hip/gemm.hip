@@ -25,17 +25,17 @@ static constexpr int G_THREADS = 256;
 static constexpr int G_BK = 32;
 static constexpr int G_LDK = G_BK + 8;        // LDS row stride in bf16 (80 bytes, 16B aligned)
 
-__device__ __forceinline__ uint32_t bf16_rn_bits(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);           // round to nearest even (finite inputs)
-    return u >> 16;
-}
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
 
-// hi/lo split of one fp32 value into two bf16 bit patterns
-__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
-    hi = bf16_rn_bits(x);
-    const float r = x - __uint_as_float(hi << 16);
-    lo = bf16_rn_bits(r);
+// hi/lo split of two fp32 values into packed bf16 pairs (x in the low half): hi = RN_bf16(v),
+// lo = RN_bf16(v - hi), with v_cvt_pk_bf16_f32 (round to nearest even) doing two roundings per
+// instruction.
+__device__ __forceinline__ void split2_bf16(float x, float y, uint32_t& hi, uint32_t& lo) {
+    const f32x2_t v = {x, y};
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+    const f32x2_t r = {x - __uint_as_float(hi << 16), y - __uint_as_float(hi & 0xffff0000u)};
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
 }
 
 template <int TM, int TN>
@@ -121,12 +121,9 @@ __device__ __forceinline__ void gemm_tile_split3(const uh_gemm_args& g, uint32_t
         for (int l = 0; l < T::A_LOADS; ++l) {
             const int idx = tid + l * G_THREADS;
             const int row = idx / (G_BK / 4), kq = (idx % (G_BK / 4)) * 4;
-            const float v[4] = {ra[l].x, ra[l].y, ra[l].z, ra[l].w};
-            uint32_t h[4], o[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) split_bf16(v[e], h[e], o[e]);
-            uint2 hv = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
-            uint2 lv = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+            uint2 hv, lv;
+            split2_bf16(ra[l].x, ra[l].y, hv.x, lv.x);
+            split2_bf16(ra[l].z, ra[l].w, hv.y, lv.y);
             *(uint2*)(a_hi + row * G_LDK + kq) = hv;
             *(uint2*)(a_lo + row * G_LDK + kq) = lv;
         }
@@ -138,11 +135,10 @@ __device__ __forceinline__ void gemm_tile_split3(const uh_gemm_args& g, uint32_t
             const float v1[4] = {rb[l][1].x, rb[l][1].y, rb[l][1].z, rb[l][1].w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                uint32_t h0, o0, h1, o1;
-                split_bf16(v0[e], h0, o0);
-                split_bf16(v1[e], h1, o1);
-                *(uint32_t*)(b_hi + (nq + e) * G_LDK + kp) = h0 | (h1 << 16);
-                *(uint32_t*)(b_lo + (nq + e) * G_LDK + kp) = o0 | (o1 << 16);
+                uint32_t hw, lw;
+                split2_bf16(v0[e], v1[e], hw, lw);     // k in the low half, k + 1 in the high half
+                *(uint32_t*)(b_hi + (nq + e) * G_LDK + kp) = hw;
+                *(uint32_t*)(b_lo + (nq + e) * G_LDK + kp) = lw;
             }
         }
     };
@@ -246,7 +242,8 @@ __host__ __device__ inline uint32_t gemm_tiles_for(const uh_gemm_args& g, int cf
 }
 
 // The tile configuration travels in the item (stored in the upper bits of precision).
-__global__ void __launch_bounds__(G_THREADS) k_gemm(uh_gemm_args g, int cfg) {
+__global__ void __launch_bounds__(G_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm(uh_gemm_args g, int cfg) {
     __shared__ short lds[kMaxLdsShorts];
     // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
     // range of tiles so that tiles sharing A rows share that XCD's L2.
@@ -257,22 +254,25 @@ __global__ void __launch_bounds__(G_THREADS) k_gemm(uh_gemm_args g, int cfg) {
     gemm_dispatch(g, tile, cfg, lds);
 }
 
-__global__ void __launch_bounds__(G_THREADS) k_gemm_persistent(const uint8_t* queue) {
+__global__ void __launch_bounds__(G_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_persistent(const uint8_t* queue) {
+    // 80 KB: two workgroups per CU. The claimed tile index is passed through the first dword of
+    // the tile buffers, which the tile itself overwrites only after the second barrier.
     __shared__ short lds[kMaxLdsShorts];
-    __shared__ uint32_t s_tile;
+    uint32_t* s_tile = (uint32_t*)lds;
     QueueHeader* h = (QueueHeader*)queue;
     const uh_gemm_args* items = (const uh_gemm_args*)(queue + h->items_offset);
     const uint32_t* prefix = (const uint32_t*)(queue + h->prefix_offset);
     const uint32_t total = h->total_tiles, n_items = h->n_items;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&h->next_tile, 1u);
+        if (threadIdx.x == 0) *s_tile = atomicAdd(&h->next_tile, 1u);
         __syncthreads();
-        const uint32_t t = s_tile;
+        const uint32_t t = __builtin_amdgcn_readfirstlane(*s_tile);
         __syncthreads();
         if (t >= total) break;
         uint32_t item, local;
         queue_locate(prefix, n_items, t, item, local);
-        uh_gemm_args g = items[item];
+        uh_gemm_args g = uniform_load(items + item);
         const int cfg = g.precision >> 8;
         g.precision &= 0xff;
         gemm_dispatch(g, local, cfg, lds);
@@ -351,3 +351,59 @@ extern "C" int uh_gemm_set_tile(int M, int N, int K, int precision, int tile_id)
 }
 
 extern "C" int uh_gemm_tile_count(int* n) { *n = kNumTiles; return UH_OK; }
+
+extern "C" int uh_gemm_autotune(int M, int N, int K, int precision, int reps, int force, int* tile_id, float* ms) {
+    if (M <= 0 || N <= 0 || K <= 0 || precision != 0 || reps < 1) return fail(UH_ERR_ARG, "uh_gemm_autotune: arguments");
+    const TuneKey key{size_class(M), size_class(N), size_class(K), precision};
+    if (!force) {
+        std::lock_guard<std::mutex> l(g_tune_mtx);
+        for (auto& e : g_tuned)
+            if (e.first.m == key.m && e.first.n == key.n && e.first.k == key.k && e.first.p == key.p) {
+                if (tile_id) *tile_id = e.second;
+                if (ms) *ms = -1.f;
+                return UH_OK;
+            }
+    }
+    float* buf = nullptr;
+    const size_t elems = (size_t)M * K + (size_t)K * N + (size_t)M * N;
+    UH_HIP(hipMalloc((void**)&buf, elems * sizeof(float)));
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = -1, rc = UH_OK;
+    float best_ms = 0.f;
+    auto run = [&]() -> int {
+        UH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        UH_HIP(hipEventCreate(&e0));
+        UH_HIP(hipEventCreate(&e1));
+        UH_HIP(hipMemsetAsync(buf, 0, elems * sizeof(float), s));
+        uh_gemm_args g{};
+        g.A = buf; g.B = buf + (size_t)M * K; g.C = buf + (size_t)M * K + (size_t)K * N;
+        g.lda = (uint64_t)K; g.ldb = (uint64_t)N; g.ldc = (uint64_t)N;
+        g.M = M; g.N = N; g.K = K; g.precision = precision; g.alpha = 1.f; g.beta = 0.f;
+        for (int cfg = 0; cfg < kNumTiles; ++cfg) {
+            const uint32_t tiles = gemm_tiles_for(g, cfg);
+            UH_LAUNCH(k_gemm, dim3(tiles), dim3(G_THREADS), 0, s, g, cfg);
+            UH_LAUNCH_CHECK("k_gemm (autotune)");
+            UH_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) UH_LAUNCH(k_gemm, dim3(tiles), dim3(G_THREADS), 0, s, g, cfg);
+            UH_LAUNCH_CHECK("k_gemm (autotune)");
+            UH_HIP(hipEventRecord(e1, s));
+            UH_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            UH_HIP(hipEventElapsedTime(&t, e0, e1));
+            t /= (float)reps;
+            if (best < 0 || t < best_ms) { best = cfg; best_ms = t; }
+        }
+        return UH_OK;
+    };
+    rc = run();
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    hipFree(buf);
+    if (rc) return rc;
+    uh_gemm_set_tile(M, N, K, precision, best);
+    if (tile_id) *tile_id = best;
+    if (ms) *ms = best_ms;
+    return UH_OK;
+}

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include <atomic>
+#include <map>
 #include <mutex>
 
 #include "unicorn_hip.h"
@@ -47,10 +48,13 @@ enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_
 
 // Work item of the filter family: the launcher's arguments (pad = view width in pixels) plus the
 // dot4 weight vectors of radius <= 2, packed once on the host (wv[dy][q][d], see imagefilter.hip).
-struct FilterItem {
-    uh_imagefilter_args a;
+struct FilterWeights {
     uint32_t wv[5 * 4 * 2];
     int32_t wsum, pad[3];
+};
+struct FilterItem {
+    uh_imagefilter_args a;
+    FilterWeights w;
 };
 
 // Work item of the TRSM family (both triangular kinds of the LU stage 2).
@@ -76,6 +80,7 @@ struct Batch {
     hipStream_t stream = nullptr;
     std::vector<uint8_t> items[FAM_COUNT];     // raw descriptors per family
     std::vector<uint32_t> tiles[FAM_COUNT];    // tile count of each recorded item
+    uint32_t kinds[FAM_COUNT] = {};            // OR of the kernel-variant bits of the items
     // upload ring: pinned host slot + device slot per ring entry
     static const int RING = 4;
     uint8_t* host_slot[RING] = {};
@@ -83,8 +88,8 @@ struct Batch {
     hipEvent_t slot_free[RING] = {};
     size_t slot_bytes = 0;
     int ring_pos = 0;
-    // graphs captured per (ring slot, family mask)
-    hipGraphExec_t graphs[RING][1 << FAM_COUNT] = {};
+    // graphs captured per ring slot, keyed by family mask and the filter kernel variant
+    std::map<uint32_t, hipGraphExec_t> graphs[RING];
     // device time of the flushes (events recorded on the batch stream around each flush)
     bool timing = false;
     std::mutex tmtx;                 // guards timed / spare / the totals
@@ -97,20 +102,52 @@ Batch* as_batch(void* stream);   // nullptr if the stream is not a batch handle
 // Stream for an operation that cannot be recorded as a work item. A batch handle acts as a
 // barrier: the items recorded so far are flushed, then the operation runs on the batch's stream.
 hipStream_t plain_stream_of(void* stream, int* rc);
-void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles);
+// kind_bits: bits of the kernel variants the item can run on (a family with one variant passes 1).
+void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t tiles,
+                 uint32_t kind_bits = 1);
 
 // Per-family persistent launchers used by uh_batch_flush (queue = device pointer to the
 // family's QueueHeader).
-int filter_persistent(const uint8_t* queue, hipStream_t s);
+int filter_persistent(const uint8_t* queue, hipStream_t s, uint32_t kinds);
+// Filter kernel variant that a flush with these kind bits launches.
+int filter_variant_for(uint32_t kinds);
 int gemm_persistent(const uint8_t* queue, hipStream_t s);
 int pagerank_persistent(const uint8_t* queue, hipStream_t s);
 int trsm_persistent(const uint8_t* queue, hipStream_t s);
+// Workgroups of the persistent filter kernel resident per CU (an occupancy query, cached).
+int filter_wg_per_cu();
 
 // Tile counts of one item of each family (host side, for the queue prefix).
 uint32_t filter_tiles(const FilterItem& it);
 uint32_t gemm_tiles(const uh_gemm_args& a);
 uint32_t pagerank_tiles(const uh_pagerank_args& a);
 uint32_t trsm_tiles(const TrsmItem& a);
+
+// Address spaces. A pointer that the compiler cannot trace to a kernel argument (for example
+// one read from a work item) is "generic" and its accesses become FLAT instructions, which count
+// against the LDS wait counter too: every LDS wait would then also wait for outstanding global
+// loads and stores. Kernels access HBM through global-space pointers instead.
+#define UH_GLOBAL __attribute__((address_space(1)))
+#define UH_CONSTANT __attribute__((address_space(4)))
+
+template <typename T>
+__device__ __forceinline__ const UH_GLOBAL T* as_global(const T* p) { return (const UH_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ UH_GLOBAL T* as_global(T* p) { return (UH_GLOBAL T*)p; }
+
+// Device helper: load a work item whose address is the same in every lane into scalar registers
+// with scalar loads. Valid for memory that no lane writes while the kernel runs (work items are
+// uploaded before the launch; the scalar cache is invalidated at each dispatch).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "work items are dword-sized");
+    T v;
+    const UH_CONSTANT uint32_t* s = (const UH_CONSTANT uint32_t*)p;
+    uint32_t* d = (uint32_t*)&v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = s[i];
+    return v;
+}
 
 // Device helper: map a global tile index of a work queue to (item, local tile) by a binary
 // search over the prefix array held in global memory.

@@ -456,6 +456,19 @@ void cache_purge(AddressSpace* as, bool all) {
     }
 }
 
+void cache_purge_uid(uint64_t uid) {
+    auto& c = gstate().cache;
+    for (auto it = c.begin(); it != c.end();) {
+        if (it->first.first == uid && it->second.users == 0) {
+            Memory::get().device_free(it->second.buf, it->second.bytes);
+            if (it->second.ready) uh_event_destroy(it->second.ready);
+            it = c.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 void cache_shrink(size_t need) {
     auto& c = gstate().cache;
     while (Memory::get().resident() + need > Memory::get().budget()) {
@@ -860,6 +873,11 @@ void Runtime::gpu_loop() {
             bool ok = false;
             try {
                 ok = Memory::get().admit(x->spaces);
+                if (!ok) {
+                    // idle read-only buffers of the PLAIN mode count against the budget too
+                    cache_shrink(Memory::get().missing_bytes(x->spaces));
+                    ok = Memory::get().admit(x->spaces);
+                }
                 if (ok) ensure_task_ready(x);
             } catch (Error& e) {
                 fail_task(x, e);
@@ -914,6 +932,7 @@ void Runtime::gpu_loop() {
             }
         }
         retire_batches(false);
+        for (uint64_t uid : Memory::get().take_released()) cache_purge_uid(uid);
         admit_queued();
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
         Task* t = nullptr;
@@ -1104,7 +1123,6 @@ void Runtime::gpu_loop() {
     cache_purge(nullptr, true);
 }
 
-void purge_gpu_cache_for(AddressSpace* as) { cache_purge(as, false); }
 
 }  // namespace rt
 }  // namespace pm

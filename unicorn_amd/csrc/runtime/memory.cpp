@@ -55,6 +55,8 @@ void Memory::release(AddressSpace* as) {
         std::lock_guard<std::mutex> g(list_mtx);
         for (auto it = spaces.begin(); it != spaces.end(); ++it)
             if (*it == as) { spaces.erase(it); break; }
+        Runtime* r = Runtime::get();
+        if (r && r->gpu_count > 0) released_uids.push_back(as->uid);
     }
     std::lock_guard<std::mutex> g(as->mtx);
     if (as->dev) {
@@ -109,13 +111,25 @@ void Memory::evict_idle(size_t need, const std::vector<AddressSpace*>* keep) {
     }
 }
 
-bool Memory::admit(const std::vector<AddressSpace*>& req) {
+std::vector<uint64_t> Memory::take_released() {
+    std::lock_guard<std::mutex> g(list_mtx);
+    std::vector<uint64_t> out;
+    out.swap(released_uids);
+    return out;
+}
+
+size_t Memory::missing_bytes(const std::vector<AddressSpace*>& req) {
     size_t need = 0;
     for (size_t i = 0; i < req.size(); ++i) {
         if (std::find(req.begin(), req.begin() + i, req[i]) != req.begin() + i) continue;   // listed twice
         std::lock_guard<std::mutex> g(req[i]->mtx);
         if (!req[i]->dev) need += req[i]->length ? req[i]->length : 1;
     }
+    return need;
+}
+
+bool Memory::admit(const std::vector<AddressSpace*>& req) {
+    const size_t need = missing_bytes(req);
     if (resident_.load() + need <= budget_) return true;
     evict_idle(need, &req);
     return resident_.load() + need <= budget_;

@@ -124,8 +124,14 @@ public:
     void evict_idle(size_t need, const std::vector<AddressSpace*>* keep = nullptr);
     // Whether the missing mirrors of `spaces` fit the budget (after evicting idle mirrors).
     bool admit(const std::vector<AddressSpace*>& spaces);
+    // Missing-mirror bytes of `spaces` (each space counted once).
+    size_t missing_bytes(const std::vector<AddressSpace*>& spaces);
+    // Ids of address spaces released since the last call (the HIP stub drops their cached
+    // device buffers).
+    std::vector<uint64_t> take_released();
     std::mutex list_mtx;
     std::vector<AddressSpace*> spaces;
+    std::vector<uint64_t> released_uids;
 private:
     size_t budget_ = 0;
     std::atomic<size_t> resident_{0};

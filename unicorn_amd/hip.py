@@ -39,6 +39,7 @@ def lib() -> C.CDLL:
             "uh_free": (i32, [vp]),
             "uh_memcpy_h2d": (i32, [vp, vp, C.c_size_t, vp]),
             "uh_memcpy_d2h": (i32, [vp, vp, C.c_size_t, vp]),
+            "uh_memcpy_d2d": (i32, [vp, vp, C.c_size_t, vp]),
             "uh_fill_u32": (i32, [vp, u32, C.c_size_t, vp]),
             "uh_stream_create": (i32, [C.POINTER(vp)]),
             "uh_stream_destroy": (i32, [vp]),
