@@ -38,6 +38,14 @@ __device__ __forceinline__ void split2_bf16(float x, float y, uint32_t& hi, uint
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
 }
 
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// 16-byte global load (p 16-byte aligned)
+__device__ __forceinline__ float4 ld_f4(const UH_GLOBAL float* p) {
+    const f32x4_t t = *(const UH_GLOBAL f32x4_t*)p;
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+
 template <int TM, int TN>
 struct GemmTile {
     static constexpr int BM = 64 * TM;
@@ -77,8 +85,8 @@ __device__ __forceinline__ void gemm_tile_split3(const uh_gemm_args& g, uint32_t
             const int gr = m0 + row, gk = k0 + kq;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gr < g.M) {
-                const float* p = g.A + (uint64_t)gr * g.lda + gk;
-                if (gk + 3 < g.K && ((((uintptr_t)p) & 15u) == 0)) v = *(const float4*)p;
+                const UH_GLOBAL float* p = as_global(g.A) + (uint64_t)gr * g.lda + gk;
+                if (gk + 3 < g.K && ((((uintptr_t)p) & 15u) == 0)) v = ld_f4(p);
                 else {
                     if (gk + 0 < g.K) v.x = p[0];
                     if (gk + 1 < g.K) v.y = p[1];
@@ -98,8 +106,8 @@ __device__ __forceinline__ void gemm_tile_split3(const uh_gemm_args& g, uint32_t
                 const int gk = k0 + kp + h;
                 float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (gk < g.K) {
-                    const float* p = g.B + (uint64_t)gk * g.ldb + gn;
-                    if (gn + 3 < g.N && ((((uintptr_t)p) & 15u) == 0)) v = *(const float4*)p;
+                    const UH_GLOBAL float* p = as_global(g.B) + (uint64_t)gk * g.ldb + gn;
+                    if (gn + 3 < g.N && ((((uintptr_t)p) & 15u) == 0)) v = ld_f4(p);
                     else {
                         if (gn + 0 < g.N) v.x = p[0];
                         if (gn + 1 < g.N) v.y = p[1];
@@ -195,7 +203,7 @@ __device__ __forceinline__ void gemm_tile_split3(const uh_gemm_args& g, uint32_t
                 const int row = m0 + wm + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 const int col = n0 + wn + j * 32 + (lane & 31);
                 if (row < g.M && col < g.N) {
-                    float* c = g.C + (uint64_t)row * g.ldc + col;
+                    UH_GLOBAL float* c = as_global(g.C) + (uint64_t)row * g.ldc + col;
                     const float v = g.alpha * acc[i][j][r];
                     *c = g.beta == 0.f ? v : v + g.beta * *c;
                 }
@@ -211,9 +219,10 @@ __device__ __forceinline__ void gemm_tile_valu(const uh_gemm_args& g, uint32_t t
         const int row = m0 + e / 64, col = n0 + e % 64;
         if (row >= g.M || col >= g.N) continue;
         float s = 0.f;
-        const float* a = g.A + (uint64_t)row * g.lda;
-        for (int k = 0; k < g.K; ++k) s = fmaf(a[k], g.B[(uint64_t)k * g.ldb + col], s);
-        float* c = g.C + (uint64_t)row * g.ldc + col;
+        const UH_GLOBAL float* a = as_global(g.A) + (uint64_t)row * g.lda;
+        const UH_GLOBAL float* b = as_global(g.B);
+        for (int k = 0; k < g.K; ++k) s = fmaf(a[k], b[(uint64_t)k * g.ldb + col], s);
+        UH_GLOBAL float* c = as_global(g.C) + (uint64_t)row * g.ldc + col;
         const float v = g.alpha * s;
         *c = g.beta == 0.f ? v : v + g.beta * *c;
     }

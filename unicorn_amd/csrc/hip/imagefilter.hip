@@ -411,33 +411,42 @@ k_imagefilter(FilterItem it, uint32_t tiles) {
 #endif
 }
 
-// Persistent launch over a device work queue: tiles are claimed from an atomic counter one claim
+// Persistent launch over a device work queue. Tiles are claimed from an atomic counter one claim
 // ahead (the claim for the tile after next is issued before a tile is computed and published to
-// the workgroup after it). Item arguments and weights are scalar loads. Every workgroup leaves
-// once a claim returns an index past the total, so the grid always drains.
+// the workgroup after it), and the next tile is looked up (item search over the LDS copy of the
+// tile prefix, scalar loads of its arguments) while the current one is computed. Every workgroup
+// leaves once a claim returns an index past the total, so the grid always drains.
+static constexpr uint32_t FT_PREFIX_LDS = 1024;   // items whose tile prefix is searched in LDS
+
 template <int V>
 __global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
 k_imagefilter_persistent(const uint8_t* queue) {
     __shared__ uint32_t lds[FT_LDS_DWORDS];
     __shared__ uint32_t s_tile[2];
+    __shared__ uint32_t s_prefix[FT_PREFIX_LDS + 1];
     QueueHeader* h = (QueueHeader*)queue;
     const QueueHeader hd = uniform_load(h);       // n_items, total_tiles, offsets (not next_tile)
     const FilterItem* items = (const FilterItem*)(queue + hd.items_offset);
     const uint32_t* prefix = (const uint32_t*)(queue + hd.prefix_offset);
     const uint32_t total = hd.total_tiles, n_items = hd.n_items;
+    const bool lds_prefix = stage_prefix(prefix, n_items, s_prefix, FT_PREFIX_LDS + 1);
     if (threadIdx.x == 0) {
         s_tile[0] = atomicAdd(&h->next_tile, 1u);
         s_tile[1] = atomicAdd(&h->next_tile, 1u);
     }
     __syncthreads();
+    auto locate = [&](uint32_t tile, uint32_t& item, uint32_t& local) {
+        if (lds_prefix) queue_locate_lds(s_prefix, n_items, tile, item, local);
+        else queue_locate(prefix, n_items, tile, item, local);
+    };
     uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[0]);
     uint32_t tn = __builtin_amdgcn_readfirstlane(s_tile[1]);
-#if FT_PREFETCH
-    // the input of tile n+1 is fetched into registers while tile n is computed
     if (t >= total) return;
     uint32_t item, local;
-    queue_locate(prefix, n_items, t, item, local);
+    locate(t, item, local);
     FilterArgs a = uniform_load((const FilterArgs*)&items[item].a);
+#if FT_PREFETCH
+    // the input of tile n+1 is fetched into registers while tile n is computed
     FilterFetch f;
     filter_fetch_v<V>(a, local, f);
     for (;;) {
@@ -446,7 +455,7 @@ k_imagefilter_persistent(const uint8_t* queue) {
         uint32_t item_n = 0, local_n = 0;
         FilterArgs an = a;
         if (tn < total) {
-            queue_locate(prefix, n_items, tn, item_n, local_n);
+            locate(tn, item_n, local_n);
             an = uniform_load((const FilterArgs*)&items[item_n].a);
             filter_fetch_v<V>(an, local_n, f);
         }
@@ -466,10 +475,7 @@ k_imagefilter_persistent(const uint8_t* queue) {
         a = an;
     }
 #else
-    while (t < total) {
-        uint32_t item, local;
-        queue_locate(prefix, n_items, t, item, local);
-        const FilterArgs a = uniform_load((const FilterArgs*)&items[item].a);
+    for (;;) {
         {
             FilterFetch f;
             filter_fetch_v<V>(a, local, f);
@@ -478,14 +484,25 @@ k_imagefilter_persistent(const uint8_t* queue) {
         uint32_t claim = total;
         if (threadIdx.x == 0 && tn < total) claim = atomicAdd(&h->next_tile, 1u);
         __syncthreads();
+        // look up the next tile now; its scalar loads complete while this tile is computed
+        uint32_t item_n = 0, local_n = 0;
+        FilterArgs an = a;
+        if (tn < total) {
+            locate(tn, item_n, local_n);
+            an = uniform_load((const FilterArgs*)&items[item_n].a);
+        }
         {
             const FilterWeights w = uniform_load(&items[item].w);
             filter_compute_v<V>(a, w, items[item].a.weights, local, lds);
         }
+        if (tn >= total) break;
         if (threadIdx.x == 0) s_tile[0] = claim;
         __syncthreads();          // LDS is restaged and s_tile is read below
         t = tn;
         tn = __builtin_amdgcn_readfirstlane(s_tile[0]);
+        item = item_n;
+        local = local_n;
+        a = an;
     }
 #endif
 }

@@ -120,15 +120,17 @@ __device__ __forceinline__ void trsm_slab(const TrsmItem& it, uint32_t slab, flo
     const int n = it.n, m = it.m, tid = threadIdx.x;
     const int s0 = (int)slab * 32;
     const int nv = min(32, m - s0);
-    auto xel = [&](int v, int i) -> float* {
-        if (it.kind == 0) return it.x + (uint64_t)(s0 + v) * it.ldx + i;          // column s0+v, row i
-        if (it.kind == 1) return it.x + (uint64_t)i * it.ldx + (s0 + v);          // row s0+v, column i
-        return it.x + (uint64_t)(s0 + v) * it.ldx + (n - 1 - i);                  // column, reversed
+    UH_GLOBAL float* x = as_global(it.x);
+    const UH_GLOBAL float* tri = as_global(it.tri);
+    auto xel = [&](int v, int i) -> UH_GLOBAL float* {
+        if (it.kind == 0) return x + (uint64_t)(s0 + v) * it.ldx + i;          // column s0+v, row i
+        if (it.kind == 1) return x + (uint64_t)i * it.ldx + (s0 + v);          // row s0+v, column i
+        return x + (uint64_t)(s0 + v) * it.ldx + (n - 1 - i);                  // column, reversed
     };
     auto tval = [&](int i, int j) -> float {
-        if (it.kind == 0) return it.tri[(uint64_t)j * it.ldt + i];
-        if (it.kind == 1) return it.tri[(uint64_t)i * it.ldt + j];
-        return it.tri[(uint64_t)(n - 1 - j) * it.ldt + (n - 1 - i)];
+        if (it.kind == 0) return tri[(uint64_t)j * it.ldt + i];
+        if (it.kind == 1) return tri[(uint64_t)i * it.ldt + j];
+        return tri[(uint64_t)(n - 1 - j) * it.ldt + (n - 1 - i)];
     };
     // load the slab: xs[i * 33 + v]
     for (int e = tid; e < n * 32; e += 256) {
@@ -206,12 +208,12 @@ __global__ void __launch_bounds__(256) k_trsm_persistent(const uint8_t* queue) {
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(&h->next_tile, 1u);
         __syncthreads();
-        const uint32_t t = s_tile;
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= total) break;
         uint32_t item, local;
         queue_locate(prefix, n_items, t, item, local);
-        trsm_slab(items[item], local, xs, tt, rd);
+        trsm_slab(uniform_load(items + item), local, xs, tt, rd);
         __syncthreads();
     }
 }

@@ -15,26 +15,32 @@ namespace uh {
 
 static constexpr int PR_THREADS = 256;
 static constexpr int PR_G = 4;                           // lanes per destination row
-static constexpr int PR_ROWS_PER_TILE = PR_THREADS / PR_G;
+static constexpr int PR_GROUPS = PR_THREADS / PR_G;      // rows in flight per workgroup
+static constexpr int PR_ROWS_PER_TILE = 16 * PR_GROUPS;  // rows per work-queue tile
 
 __device__ __forceinline__ void pagerank_rows(const uh_pagerank_args& a, uint32_t tile) {
     const uint32_t lrow = threadIdx.x / PR_G;
     const uint32_t sub = threadIdx.x % PR_G;
-    const uint32_t r = tile * PR_ROWS_PER_TILE + lrow;
-    const bool active = r < a.rows;
-    const uint32_t t = a.row0 + r;
-    float s = 0.f;
-    if (active) {
-        const uint32_t b = a.row_ptr[t], e = a.row_ptr[t + 1];
-        for (uint32_t k = b + sub; k < e; k += PR_G) s += a.w[a.col[k]];
-    }
-    // butterfly over the PR_G lanes of the row (fixed order -> deterministic)
+    const UH_GLOBAL uint32_t* row_ptr = as_global(a.row_ptr);
+    const UH_GLOBAL uint32_t* col = as_global(a.col);
+    const UH_GLOBAL uint32_t* deg = as_global(a.deg);
+    const UH_GLOBAL float* w = as_global(a.w);
+    for (uint32_t rr = lrow; rr < (uint32_t)PR_ROWS_PER_TILE; rr += PR_GROUPS) {
+        const uint32_t r = tile * PR_ROWS_PER_TILE + rr;
+        if (r >= a.rows) break;                          // the group's remaining rows are past the end
+        const uint32_t t = a.row0 + r;
+        float s = 0.f;
+        const uint32_t b = row_ptr[t], e = row_ptr[t + 1];
+        for (uint32_t k = b + sub; k < e; k += PR_G) s += w[col[k]];
+        // butterfly over the PR_G lanes of the row (fixed order -> deterministic); the PR_G lanes
+        // of a group always reach it together
 #pragma unroll
-    for (int off = PR_G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PR_G);
-    if (active && sub == 0) {
-        a.y[t] = s;
-        const uint32_t d = a.deg[t];
-        a.w_next[t] = d ? (a.damping * s) / (float)d : 0.f;
+        for (int off = PR_G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PR_G);
+        if (sub == 0) {
+            as_global(a.y)[t] = s;
+            const uint32_t d = deg[t];
+            as_global(a.w_next)[t] = d ? (a.damping * s) / (float)d : 0.f;
+        }
     }
 }
 
@@ -51,12 +57,12 @@ __global__ void __launch_bounds__(PR_THREADS) k_pagerank_persistent(const uint8_
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(&h->next_tile, 1u);
         __syncthreads();
-        const uint32_t t = s_tile;
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= total) break;
         uint32_t item, local;
         queue_locate(prefix, n_items, t, item, local);
-        pagerank_rows(items[item], local);
+        pagerank_rows(uniform_load(items + item), local);
     }
 }
 

@@ -150,16 +150,38 @@ __device__ __forceinline__ T uniform_load(const T* p) {
 }
 
 // Device helper: map a global tile index of a work queue to (item, local tile) by a binary
-// search over the prefix array held in global memory.
+// search over the prefix array (prefix[i] = first tile of item i). With a uniform tile index the
+// search runs on scalar loads (the prefix is read-only while the kernel runs).
 __device__ inline void queue_locate(const uint32_t* prefix, uint32_t n_items, uint32_t tile,
                                     uint32_t& item, uint32_t& local) {
+    const UH_CONSTANT uint32_t* p = (const UH_CONSTANT uint32_t*)prefix;
     uint32_t lo = 0, hi = n_items;           // invariant: prefix[lo] <= tile < prefix[hi]
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= tile) lo = mid; else hi = mid;
+        if (p[mid] <= tile) lo = mid; else hi = mid;
     }
     item = lo;
-    local = tile - prefix[lo];
+    local = tile - p[lo];
+}
+
+// The same search over a copy of the prefix array in LDS (see stage_prefix).
+__device__ inline void queue_locate_lds(const uint32_t* s_prefix, uint32_t n_items, uint32_t tile,
+                                        uint32_t& item, uint32_t& local) {
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_prefix[mid] <= tile) lo = mid; else hi = mid;
+    }
+    item = __builtin_amdgcn_readfirstlane(lo);
+    local = tile - __builtin_amdgcn_readfirstlane(s_prefix[item]);
+}
+
+// Copies prefix[0..n_items] into LDS (the caller synchronises before use); false when it does
+// not fit in `cap` entries.
+__device__ inline bool stage_prefix(const uint32_t* prefix, uint32_t n_items, uint32_t* s_prefix, uint32_t cap) {
+    if (n_items + 1 > cap) return false;
+    for (uint32_t i = threadIdx.x; i <= n_items; i += blockDim.x) s_prefix[i] = as_global(prefix)[i];
+    return true;
 }
 
 }  // namespace uh
