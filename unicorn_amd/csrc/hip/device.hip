@@ -338,6 +338,7 @@ int uh_batch_create(void** handle, void* stream) {
     // device queries used by the persistent launchers, answered here and not during a capture
     (void)cu_count();
     (void)filter_wg_per_cu();
+    (void)gemm_wg_per_cu();
     Batch* b = new Batch();
     b->stream = (hipStream_t)stream;
     b->slot_bytes = 0;
@@ -433,21 +434,18 @@ int uh_batch_flush(void* handle, int use_graph) {
         h->total_tiles = acc;
         g_items += n;
     }
-    // Bytes to upload: up to the end of the last family present. A captured graph copies a
-    // power-of-two bucket of that (at least 4 KB, at most the slot), and the bucket is part of
-    // the graph's key, so one capture serves every batch of that family mix and size class.
+    // The upload (up to the end of the last family present) goes on the stream ahead of the
+    // kernels; the graphs hold only the kernel launches, so one capture per family mix serves
+    // every batch, and the timing events bracket the kernels alone.
     int last = 0;
     for (int f = 0; f < FAM_COUNT; ++f)
         if ((mask >> f) & 1) last = f;
-    size_t used = (size_t)last * family_stride + need_family[last];
-    int bucket = 12;
-    while (((size_t)1 << bucket) < used) ++bucket;
-    if (use_graph) used = std::min(b->slot_bytes, (size_t)1 << bucket);
+    const size_t used = (size_t)last * family_stride + need_family[last];
+    UH_HIP(hipMemcpyAsync(dev, host, used, hipMemcpyHostToDevice, b->stream));
     auto enqueue = [&](hipStream_t s) -> int {
-        UH_HIP(hipMemcpyAsync(dev, host, used, hipMemcpyHostToDevice, s));
         int rc = UH_OK;
         if ((mask >> FAM_FILTER) & 1) rc |= filter_persistent(dev + FAM_FILTER * family_stride, s, b->kinds[FAM_FILTER]);
-        if ((mask >> FAM_GEMM) & 1) rc |= gemm_persistent(dev + FAM_GEMM * family_stride, s);
+        if ((mask >> FAM_GEMM) & 1) rc |= gemm_persistent(dev + FAM_GEMM * family_stride, s, b->kinds[FAM_GEMM]);
         if ((mask >> FAM_PAGERANK) & 1) rc |= pagerank_persistent(dev + FAM_PAGERANK * family_stride, s);
         if ((mask >> FAM_TRSM) & 1) rc |= trsm_persistent(dev + FAM_TRSM * family_stride, s);
         return rc;
@@ -465,8 +463,9 @@ int uh_batch_flush(void* handle, int use_graph) {
         }
     }
     if (use_graph) {
-        const uint32_t key = (uint32_t)mask | ((uint32_t)bucket << 12) |
-            (((mask >> FAM_FILTER) & 1) ? (uint32_t)filter_variant_for(b->kinds[FAM_FILTER]) << 8 : 0u);
+        const uint32_t key = (uint32_t)mask |
+            (((mask >> FAM_FILTER) & 1) ? (uint32_t)filter_variant_for(b->kinds[FAM_FILTER]) << 8 : 0u) |
+            (((mask >> FAM_GEMM) & 1) ? (uint32_t)gemm_variant_for(b->kinds[FAM_GEMM]) << 16 : 0u);
         hipGraphExec_t& ge = b->graphs[pos][key];
         if (!ge) {
             UH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

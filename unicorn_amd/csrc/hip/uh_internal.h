@@ -111,7 +111,11 @@ void record_item(Batch* b, Family f, const void* desc, size_t bytes, uint32_t ti
 int filter_persistent(const uint8_t* queue, hipStream_t s, uint32_t kinds);
 // Filter kernel variant that a flush with these kind bits launches.
 int filter_variant_for(uint32_t kinds);
-int gemm_persistent(const uint8_t* queue, hipStream_t s);
+int gemm_persistent(const uint8_t* queue, hipStream_t s, uint32_t kinds);
+// GEMM kernel variant that a flush with these kind bits launches.
+int gemm_variant_for(uint32_t kinds);
+// Workgroups of the persistent GEMM kernels resident per CU (occupancy queries, cached).
+int gemm_wg_per_cu();
 int pagerank_persistent(const uint8_t* queue, hipStream_t s);
 int trsm_persistent(const uint8_t* queue, hipStream_t s);
 // Workgroups of the persistent filter kernel resident per CU (an occupancy query, cached).
