@@ -4,51 +4,74 @@
 // The reference pushes contributions with float atomics from every source page into a
 // full-length output vector per subtask, over a padded record format copied from pageable host
 // memory each iteration. Here the graph is resident in HBM as an in-link CSR (rows = destination
-// pages, sources ascending) and each destination row is reduced in a fixed order by a group of
-// PR_G lanes (strided partial sums, then a butterfly over the group): no atomics, deterministic,
-// one pass over the edges per iteration. The epilogue also produces the next iteration's
-// per-source increment w = (damping * y) / deg, so the division of the reference's
-// pageRank.cpp:262,367 happens once per page instead of once per edge.
+// pages, sources ascending) and rows are reduced CSR-stream style: a workgroup takes 256
+// consecutive rows, streams their in-links through LDS in chunks of 2048 (each lane loads 8
+// consecutive-in-CSR link ids, coalesced, then issues the 8 independent gathers of w), and each
+// lane then sums its own row from LDS in index order. No atomics, a fixed summation order
+// (deterministic), one pass over the edges per iteration, and many gathers in flight. The
+// epilogue also produces the next iteration's per-source increment w = (damping * y) / deg, so
+// the division of the reference's pageRank.cpp:262,367 happens once per page instead of once
+// per edge.
 #include "uh_internal.h"
 
 namespace uh {
 
 static constexpr int PR_THREADS = 256;
-static constexpr int PR_G = 4;                           // lanes per destination row
-static constexpr int PR_GROUPS = PR_THREADS / PR_G;      // rows in flight per workgroup
-static constexpr int PR_ROWS_PER_TILE = 16 * PR_GROUPS;  // rows per work-queue tile
+static constexpr int PR_ROWS_PER_TILE = PR_THREADS;          // one destination row per lane
+static constexpr int PR_GATHER = 8;                          // gathers in flight per lane
+static constexpr int PR_CHUNK = PR_THREADS * PR_GATHER;      // links staged in LDS per chunk
 
-__device__ __forceinline__ void pagerank_rows(const uh_pagerank_args& a, uint32_t tile) {
-    const uint32_t lrow = threadIdx.x / PR_G;
-    const uint32_t sub = threadIdx.x % PR_G;
+struct PrShared {
+    uint32_t rp[PR_ROWS_PER_TILE + 1];      // row pointers of the tile's rows
+    float val[PR_CHUNK];                    // w[col[k]] of the current chunk
+};
+
+__device__ __forceinline__ void pagerank_rows(const uh_pagerank_args& a, uint32_t tile, PrShared& sm) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r0 = tile * PR_ROWS_PER_TILE;                 // first row (item-relative)
+    const uint32_t nrows = min((uint32_t)PR_ROWS_PER_TILE, a.rows - r0);
+    const uint32_t t0 = a.row0 + r0;                             // first row (global page id)
     const UH_GLOBAL uint32_t* row_ptr = as_global(a.row_ptr);
     const UH_GLOBAL uint32_t* col = as_global(a.col);
-    const UH_GLOBAL uint32_t* deg = as_global(a.deg);
     const UH_GLOBAL float* w = as_global(a.w);
-    for (uint32_t rr = lrow; rr < (uint32_t)PR_ROWS_PER_TILE; rr += PR_GROUPS) {
-        const uint32_t r = tile * PR_ROWS_PER_TILE + rr;
-        if (r >= a.rows) break;                          // the group's remaining rows are past the end
-        const uint32_t t = a.row0 + r;
-        float s = 0.f;
-        const uint32_t b = row_ptr[t], e = row_ptr[t + 1];
-        for (uint32_t k = b + sub; k < e; k += PR_G) s += w[col[k]];
-        // butterfly over the PR_G lanes of the row (fixed order -> deterministic); the PR_G lanes
-        // of a group always reach it together
+    for (uint32_t i = tid; i <= nrows; i += PR_THREADS) sm.rp[i] = row_ptr[t0 + i];
+    __syncthreads();
+    const uint32_t b = __builtin_amdgcn_readfirstlane(sm.rp[0]);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(sm.rp[nrows]);
+    const uint32_t my_b = tid < nrows ? sm.rp[tid] : 0u;
+    const uint32_t my_e = tid < nrows ? sm.rp[tid + 1] : 0u;
+    float s = 0.f;
+    for (uint32_t c0 = b; c0 < e; c0 += PR_CHUNK) {
+        uint32_t src[PR_GATHER];
 #pragma unroll
-        for (int off = PR_G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, PR_G);
-        if (sub == 0) {
-            as_global(a.y)[t] = s;
-            const uint32_t d = deg[t];
-            as_global(a.w_next)[t] = d ? (a.damping * s) / (float)d : 0.f;
+        for (int j = 0; j < PR_GATHER; ++j) {
+            const uint32_t k = c0 + tid + j * PR_THREADS;
+            src[j] = k < e ? col[k] : 0xffffffffu;
         }
+#pragma unroll
+        for (int j = 0; j < PR_GATHER; ++j) sm.val[tid + j * PR_THREADS] = src[j] != 0xffffffffu ? w[src[j]] : 0.f;
+        __syncthreads();
+        // this lane's row, restricted to the chunk, in link order
+        const uint32_t lo = my_b > c0 ? my_b : c0;
+        const uint32_t hi = my_e < c0 + PR_CHUNK ? my_e : c0 + PR_CHUNK;
+        for (uint32_t k = lo; k < hi; ++k) s += sm.val[k - c0];
+        __syncthreads();          // the chunk buffer is refilled next
+    }
+    if (tid < nrows) {
+        const uint32_t t = t0 + tid;
+        as_global(a.y)[t] = s;
+        const uint32_t d = as_global(a.deg)[t];
+        as_global(a.w_next)[t] = d ? (a.damping * s) / (float)d : 0.f;
     }
 }
 
 __global__ void __launch_bounds__(PR_THREADS) k_pagerank_pull(uh_pagerank_args a) {
-    pagerank_rows(a, blockIdx.x);
+    __shared__ PrShared sm;
+    pagerank_rows(a, blockIdx.x, sm);
 }
 
 __global__ void __launch_bounds__(PR_THREADS) k_pagerank_persistent(const uint8_t* queue) {
+    __shared__ PrShared sm;
     __shared__ uint32_t s_tile;
     QueueHeader* h = (QueueHeader*)queue;
     const uh_pagerank_args* items = (const uh_pagerank_args*)(queue + h->items_offset);
@@ -62,7 +85,8 @@ __global__ void __launch_bounds__(PR_THREADS) k_pagerank_persistent(const uint8_
         if (t >= total) break;
         uint32_t item, local;
         queue_locate(prefix, n_items, t, item, local);
-        pagerank_rows(uniform_load(items + item), local);
+        pagerank_rows(uniform_load(items + item), local, sm);
+        __syncthreads();          // LDS is reused by the next tile
     }
 }
 
