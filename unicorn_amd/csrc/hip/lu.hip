@@ -8,116 +8,365 @@
 // unit upper.
 //
 // Panel: the reference issues 3 tiny launches per pivot (reciprocal, sscal, sger; 1,533 launches
-// per 512 block). Here one 1024-thread workgroup factors the block with a blocked right-looking
-// scheme over 32-column sub-panels: LU of the 32x32 diagonal sub-block in LDS (the reference's
-// step order), triangular solves for the sub-row and sub-column panels, and a rank-32 update of
-// the remainder through LDS tiles. The block stays in L2 (1 MiB).
+// per 512 block). Here the block is factored right-looking over 32-column sub-panels, two
+// launches each: LU of the 32x32 diagonal sub-block in registers (the reference's step order)
+// with the sub-row and sub-column triangular solves spread over workgroups, then the rank-32
+// update of the remainder over 64x64 tiles. The block stays in L2 (1 MiB).
 // TRSM: a workgroup owns a slab of 32 right-hand-side vectors held in LDS and walks the triangle
-// in 32x32 blocks: a block-dot update from LDS-staged triangle tiles, then a 32-step substitution.
+// in 32-row blocks: an fp32-MFMA block update from the LDS-staged row panel, then a 32-step
+// substitution in registers.
 #include "uh_internal.h"
 
 namespace uh {
 
 static constexpr int LU_NB = 32;
 
+// In-register lower solve of 32 unknowns, right-looking: for j = 0..31, x_j is final once
+// scaled by scale[j] (when SCALE), then x_i -= C_j[i] x_j for i > j, where column C_j is the 32
+// floats at cols + j * ld (16-byte aligned). Every x_i receives its terms in the order j = 0, 1,
+// .., i-1, the fma sequence of the row-by-row form. Column j+1 is read from LDS while column j
+// is applied. The steps are instantiated one by one (straight-line code: a loop around the
+// memory barrier would not be unrolled), and the compiler-only memory barrier after each step
+// keeps the loads from all being hoisted to the top, which would need ~600 registers.
+template <bool SCALE, int J>
+__device__ __forceinline__ void lower_solve32_step(float (&x)[32], const float* cols, int ld, const float (&sc)[32],
+                                                   float4 (&cur)[8], float4 (&nxt)[8]) {
+    if constexpr (J < 32) {
+        if constexpr (J + 1 < 32) {
+#pragma unroll
+            for (int q = (J + 2) / 4; q < 8; ++q) nxt[q] = *(const float4*)(cols + (J + 1) * ld + q * 4);
+        }
+        if constexpr (SCALE) x[J] = x[J] * sc[J];
+#pragma unroll
+        for (int q = (J + 1) / 4; q < 8; ++q) {
+            const float tv[4] = {cur[q].x, cur[q].y, cur[q].z, cur[q].w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (q * 4 + c > J) x[q * 4 + c] = fmaf(-tv[c], x[J], x[q * 4 + c]);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+        asm volatile("" ::: "memory");
+        lower_solve32_step<SCALE, J + 1>(x, cols, ld, sc, cur, nxt);
+    }
+}
+
+template <bool SCALE>
+__device__ __forceinline__ void lower_solve32(float (&x)[32], const float* cols, int ld, const float* scale) {
+    float sc[32];
+    if constexpr (SCALE) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 t = *(const float4*)(scale + q * 4);
+            sc[q * 4] = t.x; sc[q * 4 + 1] = t.y; sc[q * 4 + 2] = t.z; sc[q * 4 + 3] = t.w;
+        }
+    }
+    float4 cur[8], nxt[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        cur[q] = *(const float4*)(cols + q * 4);
+        nxt[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    lower_solve32_step<SCALE, 0>(x, cols, ld, sc, cur, nxt);
+}
+
 // ---------------------------------------------------------------- panel
 
-__global__ void __launch_bounds__(1024) k_lu_panel(float* a, uint64_t lda, int n) {
-    __shared__ float d[LU_NB][LU_NB + 1];          // diagonal sub-block
-    __shared__ float tl[64][LU_NB + 1];            // L21 tile (rows x 32)
-    __shared__ float tu[LU_NB][64 + 1];            // U12 tile (32 x cols)
-    __shared__ float rcp[LU_NB];
+// Sub-step kb of the panel, launch 1. Every workgroup loads the nb x nb diagonal sub-block
+// (padded to 32 x 32 with the identity) and factors it in registers of wave 0, lane c holding
+// column c: step i reads the pivot and column i across lanes, scales row i by RN(1/pivot) and
+// applies the rank-1 update, the reference's per-pivot order. Every workgroup computes the same
+// bits, so workgroup 0 stores its copy and the others use theirs:
+//   workgroups 1 .. nr:       256 columns each of the sub-row panel, U12 = L11^-1 A12;
+//   workgroups nr+1 .. 2 nr:  256 rows each of the sub-column panel, L21 = A21 U11^-1;
+// one column or row per thread, solved in registers. The panel slice is loaded (coalesced, into
+// registers) before the factorisation and passes through LDS on its way in and out.
+__global__ void __launch_bounds__(256) k_lu_step(float* a_, uint64_t lda, int n, int kb) {
+    __shared__ __attribute__((aligned(16))) float d[LU_NB][36];    // d[r][c]: row r of the factored sub-block
+    __shared__ __attribute__((aligned(16))) float dt[LU_NB][36];   // dt[c][r] = d[r][c]
+    __shared__ __attribute__((aligned(16))) float rcp[LU_NB];
+    __shared__ float px[256][LU_NB + 1];                           // px[t][i]: thread t's column or row
+    UH_GLOBAL float* a = as_global(a_);
     const int tid = threadIdx.x;
-    for (int kb = 0; kb < n; kb += LU_NB) {
-        const int nb = min(LU_NB, n - kb);
-        // 1. diagonal sub-block: load, factor (reference step order), store
-        for (int e = tid; e < nb * nb; e += 1024) {
-            const int r = e % nb, c = e / nb;
-            d[r][c] = a[(uint64_t)(kb + c) * lda + kb + r];
+    const int nb = min(LU_NB, n - kb);
+    const int rest = n - kb - nb;
+    const int nr = (rest + 255) / 256;
+    const int role = blockIdx.x == 0 ? 0 : (blockIdx.x <= (unsigned)nr ? 1 : 2);
+    const int base = kb + nb + ((int)blockIdx.x - 1 - (role == 2 ? nr : 0)) * 256;   // first column / row
+    const int cnt = min(256, n - base);                                                // columns / rows here
+    // element q of this thread's share: role 1 column base + q * 8 + tid / 32, row kb + tid % 32;
+    // role 2 row base + tid, column kb + q. Addresses are clamped into the block, values zeroed.
+    float ld[LU_NB];
+    if (role != 0) {
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) {
+            int c, r;
+            if (role == 1) { c = base + q * 8 + (tid >> 5); r = kb + (tid & 31); }
+            else { c = kb + q; r = base + tid; }
+            const bool ok = (role == 1) ? (q * 8 + (tid >> 5) < cnt && (tid & 31) < nb) : (tid < cnt && q < nb);
+            const float v = a[(uint64_t)min(c, n - 1) * lda + min(r, n - 1)];
+            ld[q] = ok ? v : 0.f;
         }
-        __syncthreads();
-        for (int i = 0; i < nb; ++i) {
-            if (tid == 0) rcp[i] = 1.0f / d[i][i];
-            __syncthreads();
-            const float ri = rcp[i];
-            for (int c = i + 1 + tid; c < nb; c += 1024) d[i][c] = d[i][c] * ri;
-            __syncthreads();
-            const int m = nb - i - 1;
-            for (int e = tid; e < m * m; e += 1024) {
-                const int r = i + 1 + e % m, c = i + 1 + e / m;
-                d[r][c] = fmaf(-d[r][i], d[i][c], d[r][c]);
+    }
+    if (tid < 64) {
+        const int c = tid & 31;
+        float col[LU_NB];
+#pragma unroll
+        for (int r = 0; r < LU_NB; ++r) {
+            const float v = a[(uint64_t)(kb + min(c, nb - 1)) * lda + kb + min(r, nb - 1)];
+            col[r] = (r < nb && c < nb) ? v : (r == c ? 1.f : 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < LU_NB; ++i) {
+            const float ri = 1.0f / __int_as_float(__builtin_amdgcn_readlane(__float_as_int(col[i]), i));
+            const bool act = c > i;
+            if (act) col[i] = col[i] * ri;
+#pragma unroll
+            for (int r = i + 1; r < LU_NB; ++r) {
+                const float l = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(col[r]), i));
+                if (act) col[r] = fmaf(-l, col[i], col[r]);
             }
-            __syncthreads();
+            if (tid == 0) rcp[i] = ri;
         }
-        for (int e = tid; e < nb * nb; e += 1024) {
+        if (tid < 32) {
+#pragma unroll
+            for (int r = 0; r < LU_NB; ++r) {
+                d[r][c] = col[r];
+                dt[c][r] = col[r];
+            }
+        }
+    }
+    if (role != 0) {
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) {
+            if (role == 1) px[q * 8 + (tid >> 5)][tid & 31] = ld[q];
+            else px[tid][q] = ld[q];
+        }
+    }
+    __syncthreads();
+    if (role == 0) {
+        for (int e = tid; e < nb * nb; e += 256) {
             const int r = e % nb, c = e / nb;
             a[(uint64_t)(kb + c) * lda + kb + r] = d[r][c];
         }
-        const int rest = n - kb - nb;
-        if (rest <= 0) break;
-        // 2. sub-row panel: U12 = L11^-1 A12 (one thread per column, multiply by RN(1/l_ii))
-        for (int c = kb + nb + tid; c < n; c += 1024) {
-            float* col = a + (uint64_t)c * lda + kb;
-            float x[LU_NB];
-            for (int i = 0; i < nb; ++i) x[i] = col[i];
-            for (int i = 0; i < nb; ++i) {
-                float s = x[i];
-                for (int j = 0; j < i; ++j) s = fmaf(-d[i][j], x[j], s);
-                x[i] = s * rcp[i];
-            }
-            for (int i = 0; i < nb; ++i) col[i] = x[i];
-        }
-        // 3. sub-column panel: L21 = A21 U11^-1, U11 unit upper (one thread per row)
-        for (int r = kb + nb + tid; r < n; r += 1024) {
-            float x[LU_NB];
-            for (int j = 0; j < nb; ++j) x[j] = a[(uint64_t)(kb + j) * lda + r];
-            for (int j = 0; j < nb; ++j) {
-                float s = x[j];
-                for (int i = 0; i < j; ++i) s = fmaf(-x[i], d[i][j], s);
-                x[j] = s;
-            }
-            for (int j = 0; j < nb; ++j) a[(uint64_t)(kb + j) * lda + r] = x[j];
-        }
-        __syncthreads();
-        // 4. trailing update A22 -= L21 U12 in 64x64 tiles (4 outputs per thread)
-        const int r0 = kb + nb;
-        const int tiles = (rest + 63) / 64;
-        for (int t = 0; t < tiles * tiles; ++t) {
-            const int tr = r0 + (t % tiles) * 64, tc = r0 + (t / tiles) * 64;
-            for (int e = tid; e < 64 * nb; e += 1024) {
-                const int r = e % 64, k = e / 64;
-                tl[r][k] = (tr + r < n) ? a[(uint64_t)(kb + k) * lda + tr + r] : 0.f;
-                const int c = e % 64, k2 = e / 64;
-                tu[k2][c] = (tc + c < n) ? a[(uint64_t)(tc + c) * lda + kb + k2] : 0.f;
-            }
-            __syncthreads();
-            const int r = tid % 64, cq = (tid / 64) * 4;
-            if (tr + r < n) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int c = cq + q;
-                    if (tc + c >= n) continue;
-                    float* p = a + (uint64_t)(tc + c) * lda + tr + r;
-                    float s = *p;
-                    for (int k = 0; k < nb; ++k) s = fmaf(-tl[r][k], tu[k][c], s);
-                    *p = s;
-                }
-            }
-            __syncthreads();
-        }
+        return;
     }
+    float x[LU_NB];
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) x[i] = px[tid][i];
+    if (role == 1)   // U12 column: x_i = (x_i - sum_{j<i} L11(i, j) x_j) * RN(1/L11(i, i)); column j of L11 is dt[j]
+        lower_solve32<true>(x, &dt[0][0], 36, rcp);
+    else             // L21 row: x_j = x_j - sum_{i<j} x_i U11(i, j), U11 unit upper; row i of U11 is d[i]
+        lower_solve32<false>(x, &d[0][0], 36, nullptr);
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) px[tid][i] = x[i];
+    __syncthreads();
+    if (role == 1) {
+        for (int e = tid; e < cnt * 32; e += 256) {
+            const int c = e >> 5, i = e & 31;
+            if (i < nb) a[(uint64_t)(base + c) * lda + kb + i] = px[c][i];
+        }
+    } else {
+        for (int q = 0; q < nb; ++q)
+            if (tid < cnt) a[(uint64_t)(kb + q) * lda + base + tid] = px[tid][q];
+    }
+}
+
+// Sub-step kb of the panel, launch 2: A22 -= L21 U12 (K = 32) over 64 x 64 tiles; a thread owns
+// 4 consecutive rows of 4 columns and accumulates k = 0..31 in order from the element's value.
+static constexpr int LU_TL = 68;   // LDS row stride of the 64-wide staged panels (16-byte rows)
+
+__global__ void __launch_bounds__(256) k_lu_rank32(float* a_, uint64_t lda, int n, int kb) {
+    __shared__ __attribute__((aligned(16))) float tl[LU_NB][LU_TL];   // tl[k][r] = L21(tr + r, k)
+    __shared__ __attribute__((aligned(16))) float tu[LU_NB][LU_TL];   // tu[k][c] = U12(k, tc + c)
+    UH_GLOBAL float* a = as_global(a_);
+    const int tid = threadIdx.x;
+    const int r0 = kb + LU_NB;
+    const int tiles = (n - r0 + 63) / 64;
+    const int tr = r0 + (int)(blockIdx.x % (unsigned)tiles) * 64;
+    const int tc = r0 + (int)(blockIdx.x / (unsigned)tiles) * 64;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + 256 * q;
+        const int r = e & 63, k = e >> 6;
+        tl[k][r] = tr + r < n ? a[(uint64_t)(kb + k) * lda + tr + r] : 0.f;
+        const int k2 = e & 31, c = e >> 5;
+        tu[k2][c] = tc + c < n ? a[(uint64_t)(tc + c) * lda + kb + k2] : 0.f;
+    }
+    __syncthreads();
+    const int rq = (tid & 15) * 4, cq = (tid >> 4) * 4;
+    float s[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const bool ok = tr + rq + r < n && tc + cq + c < n;
+            s[c][r] = ok ? a[(uint64_t)(tc + cq + c) * lda + tr + rq + r] : 0.f;
+        }
+#pragma unroll 8
+    for (int k = 0; k < LU_NB; ++k) {
+        const float4 l = *(const float4*)&tl[k][rq];
+        const float4 u = *(const float4*)&tu[k][cq];
+        const float lv[4] = {l.x, l.y, l.z, l.w}, uv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s[c][r] = fmaf(-lv[r], uv[c], s[c][r]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (tr + rq + r < n && tc + cq + c < n) a[(uint64_t)(tc + cq + c) * lda + tr + rq + r] = s[c][r];
 }
 
 // ---------------------------------------------------------------- TRSM
 
-// Solves 32 vectors in place. Vector v (0..31 of the slab starting at s0) has elements
-// x_v[i] = X[elem(v, i)], i = 0..n-1. The triangle T' is lower: x_i = (b_i - sum_{j<i} T'(i,j) x_j)
-// * rdiag_i with T'(i,j) = L(i,j), rdiag = RN(1/L(i,i)) for kind 0 (L^-1 X, X n x m col-major);
-// T'(i,j) = U(j,i), rdiag = 1 for kind 1 (X U^-1, X m x n col-major); and for kind 2 (U^-1 X,
-// U unit upper, X n x m col-major) the same lower form on reversed indices.
-__device__ __forceinline__ void trsm_slab(const TrsmItem& it, uint32_t slab, float* xs /* n x 33 */,
-                                          float (*tt)[33], float* rd) {
-    const int n = it.n, m = it.m, tid = threadIdx.x;
+// A workgroup solves a slab of 32 right-hand-side vectors in place. Vector v (0..31 of the slab
+// starting at s0) has elements x_v[i] = X[elem(v, i)], i = 0..n-1, and the triangle T' is lower:
+// x_i = (b_i - sum_{j<i} T'(i,j) x_j) * rdiag_i with
+//   kind 0 (L^-1 X, X n x m col-major):      T'(i,j) = L(i,j), rdiag = RN(1/L(i,i));
+//   kind 1 (X U^-1, X m x n col-major):      T'(i,j) = U(j,i), rdiag = 1;
+//   kind 2 (U^-1 X, U unit upper, X n x m):  the same lower form on reversed indices.
+// The slab lives in LDS. Rows go in blocks of 32. For block b the four waves each apply a
+// quarter of the block update sum_{j<b} T'(b+i, j) x_j with v_mfma_f32_32x32x2_f32 from the
+// row panel T'(b..b+31, 0..b-1) staged in LDS; then wave 0 adds the partials in a fixed order
+// and substitutes through the 32 x 32 diagonal block (lane v holding vector v's 32 values in
+// registers) while waves 1..3 store the next block's row panel, which they loaded into registers
+// one block earlier, and issue the loads of the one after. Two barriers per block.
+static constexpr int TRSM_MAX_N = 512;
+static constexpr int TS_LD = 33;   // LDS row stride of the slab
+static constexpr int TS_PU = 24;   // 16-byte panel units per prefetching thread (192 threads)
+
+typedef float ts_f32x16 __attribute__((ext_vector_type(16)));
+
+struct TrsmSmem {
+    float xs[TRSM_MAX_N * TS_LD];        // xs[i * TS_LD + v] = x_v[i]
+    float pt[TRSM_MAX_N * 32];           // T'(b + i, j), j < b, at pt_idx(j, i)
+    float red[3][16][64];                // partial block updates of waves 1..3 (MFMA register layout)
+    __attribute__((aligned(16))) float rd[TRSM_MAX_N];
+    __attribute__((aligned(16))) float dg[2][32][36];   // dg[(b / 32) & 1][j][i] = T'(b + i, b + j)
+    __attribute__((aligned(16))) float sink[192][4];    // panel units outside the block's columns
+};
+
+// Row j of the panel is 32 floats; its 16-byte groups are XOR-swizzled by (j / 4) % 8 so that
+// the transposed unit writes of kind 1 spread over the banks. A permutation within the row:
+// the MFMA operand reads (two whole rows per instruction) stay conflict-free.
+__device__ __forceinline__ int pt_idx(int j, int i) { return j * 32 + (i ^ (((j >> 2) & 7) << 2)); }
+
+__device__ __forceinline__ float trsm_tval(const TrsmItem& it, const UH_GLOBAL float* tri, int i, int j) {
+    if (it.kind == 0) return tri[(uint64_t)j * it.ldt + i];
+    if (it.kind == 1) return tri[(uint64_t)i * it.ldt + j];
+    return tri[(uint64_t)(it.n - 1 - j) * it.ldt + (it.n - 1 - i)];
+}
+
+// The row panel of block b, T'(b + i, j) for i < 32 and j < b + 32, moves in 16-byte units; the
+// 192 prefetching threads (tp = 0..191) each own TS_PU of them, all with rows 4 (tp % 8) .. +3:
+//   kinds 0 and 2 (T' columns contiguous in the triangle): unit q is column j = tp / 8 + 24 q;
+//   kind 1 (T' rows contiguous): units 4 q' .. 4 q' + 3 are the 4 rows at columns 4 j4 .. 4 j4 + 3,
+//     j4 = tp / 8 + 24 q', transposed in registers into 4 columns before the store.
+// With ldt, n and the triangle's address multiples of 4 floats the units are single 16-byte
+// buffer loads (32-bit offsets: the host checks the triangle spans less than 2 GiB); otherwise
+// each is four 4-byte loads. Elements outside the triangle read as zero.
+__device__ __forceinline__ void trsm_panel_load(const TrsmItem& it, int b, int tp, float4 (&pf)[TS_PU]) {
+    const int n = it.n, ld = (int)it.ldt, J = min(b + 32, n);
+    const int i0 = 4 * (tp & 7), jt = tp >> 3;
+    const int extent = (int)(((int64_t)(n - 1) * (int64_t)ld + n) * 4);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)it.tri, 0, extent, 0x00020000);
+    const bool vec = ((ld | n) & 3) == 0 && ((uintptr_t)it.tri & 15) == 0;
+    auto ld1 = [&](bool ok, int off) -> float {
+        const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, ok ? (uint32_t)off * 4u : 0u, 0, 0));
+        return ok ? v : 0.f;
+    };
+    auto ld4 = [&](bool ok, int off) -> float4 {
+        const float4 v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? (uint32_t)off * 4u : 0u, 0, 0));
+        return ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    if (it.kind != 1) {
+        const bool rev = it.kind == 2;
+#pragma unroll
+        for (int q = 0; q < TS_PU; ++q) {
+            const int j = jt + 24 * q;
+            // element (b + i0 + t, j): kind 0 at j ld + b + i0 + t; kind 2 at (n-1-j) ld + n-1-b-i0-t
+            const int o = rev ? (n - 1 - j) * ld + (n - 1 - b - i0) : j * ld + b + i0;
+            if (vec) {
+                const bool ok = j < J && b + i0 < n;
+                const float4 v = ld4(ok, rev ? o - 3 : o);
+                pf[q] = rev ? make_float4(v.w, v.z, v.y, v.x) : v;
+            } else {
+                const int s = rev ? -1 : 1;
+                pf[q] = make_float4(ld1(j < J && b + i0 < n, o), ld1(j < J && b + i0 + 1 < n, o + s),
+                                    ld1(j < J && b + i0 + 2 < n, o + 2 * s), ld1(j < J && b + i0 + 3 < n, o + 3 * s));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TS_PU / 4; ++q) {
+            const int j = 4 * (jt + 24 * q);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const bool rok = b + i0 + r < n;
+                const int o = (b + i0 + r) * ld + j;   // element (b + i0 + r, j .. j + 3)
+                if (vec) pf[4 * q + r] = ld4(rok && j < J, o);
+                else pf[4 * q + r] = make_float4(ld1(rok && j < J, o), ld1(rok && j + 1 < J, o + 1),
+                                                 ld1(rok && j + 2 < J, o + 2), ld1(rok && j + 3 < J, o + 3));
+            }
+        }
+    }
+}
+
+// Branch-free: a unit (rows i0 .. i0 + 3 of column j) lands in the panel (j < b), the diagonal
+// block or the thread's sink.
+__device__ __forceinline__ void trsm_unit_store(TrsmSmem& sm, int b, int j, int i0, int tp, float4 v) {
+    float* dg = &sm.dg[(b >> 5) & 1][0][0];
+    float* dst = j < b ? &sm.pt[pt_idx(j, i0)] : (j < b + 32 ? &dg[(j - b) * 36 + i0] : &sm.sink[tp][0]);
+    *(float4*)dst = v;
+}
+
+__device__ __forceinline__ void trsm_panel_store(int kind, int b, int tp, const float4 (&pf)[TS_PU], TrsmSmem& sm) {
+    const int i0 = 4 * (tp & 7), jt = tp >> 3;
+    if (kind != 1) {
+#pragma unroll
+        for (int q = 0; q < TS_PU; ++q) trsm_unit_store(sm, b, jt + 24 * q, i0, tp, pf[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < TS_PU / 4; ++q) {
+            const int j = 4 * (jt + 24 * q);
+            const float4 r0 = pf[4 * q], r1 = pf[4 * q + 1], r2 = pf[4 * q + 2], r3 = pf[4 * q + 3];
+            trsm_unit_store(sm, b, j, i0, tp, make_float4(r0.x, r1.x, r2.x, r3.x));
+            trsm_unit_store(sm, b, j + 1, i0, tp, make_float4(r0.y, r1.y, r2.y, r3.y));
+            trsm_unit_store(sm, b, j + 2, i0, tp, make_float4(r0.z, r1.z, r2.z, r3.z));
+            trsm_unit_store(sm, b, j + 3, i0, tp, make_float4(r0.w, r1.w, r2.w, r3.w));
+        }
+    }
+}
+
+// Quarter `wave` of the block update of block b: D(i, v) = sum T'(b+i, j) x_j[v] over
+// j in [wave b/4, (wave+1) b/4) (zero for b = 0).
+__device__ __forceinline__ ts_f32x16 trsm_block_update(const TrsmSmem& sm, int b, int wave, int lane) {
+    ts_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int q = b >> 2;   // a multiple of 8
+    const int jo = lane >> 5, c = lane & 31;
+    for (int j = wave * q; j < (wave + 1) * q; j += 8) {
+        float av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            av[u] = sm.pt[pt_idx(j + 2 * u + jo, c)];
+            bv[u] = sm.xs[(j + 2 * u + jo) * TS_LD + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void trsm_slab(const TrsmItem& it, uint32_t slab, TrsmSmem& sm) {
+    const int n = it.n, m = it.m, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: the role branches are scalar
+    const int tp = tid - 64;
+    const int np = (n + 31) & ~31;   // rows padded to whole blocks
     const int s0 = (int)slab * 32;
     const int nv = min(32, m - s0);
     UH_GLOBAL float* x = as_global(it.x);
@@ -127,79 +376,95 @@ __device__ __forceinline__ void trsm_slab(const TrsmItem& it, uint32_t slab, flo
         if (it.kind == 1) return x + (uint64_t)i * it.ldx + (s0 + v);          // row s0+v, column i
         return x + (uint64_t)(s0 + v) * it.ldx + (n - 1 - i);                  // column, reversed
     };
-    auto tval = [&](int i, int j) -> float {
-        if (it.kind == 0) return tri[(uint64_t)j * it.ldt + i];
-        if (it.kind == 1) return tri[(uint64_t)i * it.ldt + j];
-        return tri[(uint64_t)(n - 1 - j) * it.ldt + (n - 1 - i)];
-    };
-    // load the slab: xs[i * 33 + v]
+    // the slab, 16 loads in flight per thread
+    for (int e0 = 0; e0 < np * 32; e0 += 256 * 16) {
+        float v16[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = e0 + u * 256 + tid;
+            int i, v;
+            if (it.kind != 1) { i = e % np; v = e / np; } else { v = e % 32; i = e / 32; }
+            const bool ok = e < np * 32 && v < nv && i < n;
+            const UH_GLOBAL float* p = ok ? xel(v, i) : x;
+            const float t = *p;
+            v16[u] = ok ? t : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = e0 + u * 256 + tid;
+            int i, v;
+            if (it.kind != 1) { i = e % np; v = e / np; } else { v = e % 32; i = e / 32; }
+            if (e < np * 32) sm.xs[i * TS_LD + v] = v16[u];
+        }
+    }
+    for (int i = tid; i < np; i += 256) sm.rd[i] = (it.kind == 0 && i < n) ? 1.0f / trsm_tval(it, tri, i, i) : 1.0f;
+    // The two roles run separate loops with the same two barriers per block, so the prefetch
+    // registers of waves 1..3 do not exist on wave 0's path through the substitution.
+    if (wave == 0) {
+        __syncthreads();   // slab, rd and block 0's panel in LDS
+        for (int b = 0; b < n; b += 32) {
+            ts_f32x16 acc = trsm_block_update(sm, b, 0, lane);
+            __syncthreads();
+            if (b > 0) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    float t = acc[r];
+                    t += sm.red[0][r][lane];
+                    t += sm.red[1][r][lane];
+                    t += sm.red[2][r][lane];
+                    const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    sm.xs[(b + row) * TS_LD + (lane & 31)] -= t;
+                }
+            }
+            // lanes 32..63 wrote rows that lanes 0..31 read below
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < 32) {
+                const int v = lane;
+                float xv[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) xv[i] = sm.xs[(b + i) * TS_LD + v];
+                lower_solve32<true>(xv, &sm.dg[(b >> 5) & 1][0][0], 36, &sm.rd[b]);
+#pragma unroll
+                for (int i = 0; i < 32; ++i) sm.xs[(b + i) * TS_LD + v] = xv[i];
+            }
+            __syncthreads();
+        }
+    } else {
+        float4 pf[TS_PU];
+        trsm_panel_load(it, 0, tp, pf);
+        trsm_panel_store(it.kind, 0, tp, pf, sm);
+        if (32 < n) trsm_panel_load(it, 32, tp, pf);
+        __syncthreads();
+        for (int b = 0; b < n; b += 32) {
+            const ts_f32x16 acc = trsm_block_update(sm, b, wave, lane);
+            if (b > 0) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) sm.red[wave - 1][r][lane] = acc[r];
+            }
+            __syncthreads();
+            if (b + 32 < n) {
+                trsm_panel_store(it.kind, b + 32, tp, pf, sm);
+                if (b + 64 < n) trsm_panel_load(it, b + 64, tp, pf);
+            }
+            __syncthreads();
+        }
+    }
     for (int e = tid; e < n * 32; e += 256) {
         int i, v;
         if (it.kind != 1) { i = e % n; v = e / n; } else { v = e % 32; i = e / 32; }
-        xs[i * 33 + v] = v < nv ? *xel(v, i) : 0.f;
-    }
-    for (int i = tid; i < n; i += 256) rd[i] = it.kind == 0 ? 1.0f / tval(i, i) : 1.0f;
-    __syncthreads();
-    for (int b = 0; b < n; b += 32) {
-        const int bn = min(32, n - b);
-        // block-dot update: x[b+i] -= sum_{j<b} T'(b+i, j) x[j]
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        const int ui = tid % 32, uv = (tid / 32) * 4;   // row ui of the block, vectors uv..uv+3
-        for (int jb = 0; jb < b; jb += 32) {
-            for (int e = tid; e < 32 * 32; e += 256) {
-                const int i = e % 32, j = e / 32;
-                tt[i][j] = (i < bn) ? tval(b + i, jb + j) : 0.f;
-            }
-            __syncthreads();
-#pragma unroll 8
-            for (int j = 0; j < 32; ++j) {
-                const float tv = tt[ui][j];
-                const float* xr = xs + (jb + j) * 33 + uv;
-                acc[0] = fmaf(tv, xr[0], acc[0]);
-                acc[1] = fmaf(tv, xr[1], acc[1]);
-                acc[2] = fmaf(tv, xr[2], acc[2]);
-                acc[3] = fmaf(tv, xr[3], acc[3]);
-            }
-            __syncthreads();
-        }
-        if (ui < bn)
-            for (int q = 0; q < 4; ++q) xs[(b + ui) * 33 + uv + q] -= acc[q];
-        // diagonal block: stage T'(b+i, b+j) and substitute (one lane per vector)
-        for (int e = tid; e < 32 * 32; e += 256) {
-            const int i = e % 32, j = e / 32;
-            tt[i][j] = (i < bn && j < bn) ? tval(b + i, b + j) : 0.f;
-        }
-        __syncthreads();
-        if (tid < 32) {
-            const int v = tid;
-            for (int i = 0; i < bn; ++i) {
-                float s = xs[(b + i) * 33 + v];
-                for (int j = 0; j < i; ++j) s = fmaf(-tt[i][j], xs[(b + j) * 33 + v], s);
-                xs[(b + i) * 33 + v] = s * rd[b + i];
-            }
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < n * 32; e += 256) {
-        int i, v;
-        if (it.kind != 1) { i = e % n; v = e / n; } else { v = e % 32; i = e / 32; }
-        if (v < nv) *xel(v, i) = xs[i * 33 + v];
+        if (v < nv) *xel(v, i) = sm.xs[i * TS_LD + v];
     }
 }
 
-static constexpr int TRSM_MAX_N = 512;
-
 __global__ void __launch_bounds__(256) k_trsm(TrsmItem it) {
-    extern __shared__ float xs[];
-    __shared__ float tt[32][33];
-    __shared__ float rd[TRSM_MAX_N];
-    trsm_slab(it, blockIdx.x, xs, tt, rd);
+    __shared__ TrsmSmem sm;
+    trsm_slab(it, blockIdx.x, sm);
 }
 
 __global__ void __launch_bounds__(256) k_trsm_persistent(const uint8_t* queue) {
-    __shared__ float xs[TRSM_MAX_N * 33];
-    __shared__ float tt[32][33];
-    __shared__ float rd[TRSM_MAX_N];
+    __shared__ TrsmSmem sm;
     __shared__ uint32_t s_tile;
     QueueHeader* h = (QueueHeader*)queue;
     const TrsmItem* items = (const TrsmItem*)(queue + h->items_offset);
@@ -213,7 +478,7 @@ __global__ void __launch_bounds__(256) k_trsm_persistent(const uint8_t* queue) {
         if (t >= total) break;
         uint32_t item, local;
         queue_locate(prefix, n_items, t, item, local);
-        trsm_slab(uniform_load(items + item), local, xs, tt, rd);
+        trsm_slab(uniform_load(items + item), local, sm);
         __syncthreads();
     }
 }
@@ -221,7 +486,7 @@ __global__ void __launch_bounds__(256) k_trsm_persistent(const uint8_t* queue) {
 uint32_t trsm_tiles(const TrsmItem& a) { return (uint32_t)((a.m + 31) / 32); }
 
 int trsm_persistent(const uint8_t* queue, hipStream_t s) {
-    UH_LAUNCH(k_trsm_persistent, dim3(cu_count() * 2), dim3(256), 0, s, queue);
+    UH_LAUNCH(k_trsm_persistent, dim3(cu_count()), dim3(256), 0, s, queue);
     UH_LAUNCH_CHECK("k_trsm_persistent");
     g_persistent_launches++;
     return UH_OK;
@@ -270,24 +535,44 @@ __global__ void __launch_bounds__(256) k_residual_sum(const double* part, int ch
     }
 }
 
-// y(i, k) -= sum_{j in [j0, j1)} A(i, j) * z(j, k) for rows [i0, i1): fp32
-__global__ void __launch_bounds__(256) k_update_f32(const float* a, uint64_t lda, int i0, int i1, int j0,
-                                                    int j1, float* z, uint64_t ldz, int nrhs) {
+// y(i, k) -= sum_{j in [j0, j1)} A(i, j) * z(j, k) for rows [i0, i1): fp32. A workgroup owns 64
+// rows; wave w takes the columns j = j0 + w (mod 4) and the four partial sums are added in a
+// fixed order.
+__global__ void __launch_bounds__(256) k_update_f32(const float* a_, uint64_t lda, int i0, int i1, int j0,
+                                                    int j1, float* z_, uint64_t ldz, int nrhs) {
     __shared__ float zs[RES_CHUNK][8];
-    const int i = i0 + blockIdx.x * 256 + threadIdx.x;
+    __shared__ float part[3][8][64];
+    const UH_GLOBAL float* a = as_global(a_);
+    UH_GLOBAL float* z = as_global(z_);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = i0 + blockIdx.x * 64 + lane;
     for (int e = threadIdx.x; e < (j1 - j0) * 8; e += 256) {
         const int j = e / 8, k = e % 8;
         zs[j][k] = k < nrhs ? z[(uint64_t)k * ldz + j0 + j] : 0.f;
     }
     __syncthreads();
-    if (i >= i1) return;
     float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int j = j0; j < j1; ++j) {
-        const float av = a[(uint64_t)j * lda + i];
+    if (i < i1) {
+#pragma unroll 4
+        for (int j = j0 + w; j < j1; j += 4) {
+            const float av = a[(uint64_t)j * lda + i];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s[k] = fmaf(av, zs[j - j0][k], s[k]);
+            for (int k = 0; k < 8; ++k) s[k] = fmaf(av, zs[j - j0][k], s[k]);
+        }
     }
-    for (int k = 0; k < nrhs; ++k) z[(uint64_t)k * ldz + i] -= s[k];
+    if (w > 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) part[w - 1][k][lane] = s[k];
+    }
+    __syncthreads();
+    if (w == 0 && i < i1) {
+        for (int k = 0; k < nrhs; ++k) {
+            float t = s[k] + part[0][k][lane];
+            t += part[1][k][lane];
+            t += part[2][k][lane];
+            z[(uint64_t)k * ldz + i] -= t;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_refine_update(double* x, const float* z, int n, int nrhs) {
@@ -324,9 +609,18 @@ extern "C" int uh_lu_panel(float* a, uint64_t lda, int n, void* stream) {
     } else {
         s = (hipStream_t)stream;
     }
-    UH_LAUNCH(k_lu_panel, dim3(1), dim3(1024), 0, s, a, lda, n);
-    UH_LAUNCH_CHECK("k_lu_panel");
-    g_plain_launches++;
+    for (int kb = 0; kb < n; kb += LU_NB) {
+        const int rest = n - kb - min(LU_NB, n - kb);
+        UH_LAUNCH(k_lu_step, dim3(1 + 2 * ((rest + 255) / 256)), dim3(256), 0, s, a, lda, n, kb);
+        UH_LAUNCH_CHECK("k_lu_step");
+        g_plain_launches++;
+        if (rest > 0) {
+            const unsigned tiles = (unsigned)((rest + 63) / 64);
+            UH_LAUNCH(k_lu_rank32, dim3(tiles * tiles), dim3(256), 0, s, a, lda, n, kb);
+            UH_LAUNCH_CHECK("k_lu_rank32");
+            g_plain_launches++;
+        }
+    }
     return UH_OK;
 }
 
@@ -334,6 +628,9 @@ static int trsm_common(const float* t, uint64_t ldt, float* x, uint64_t ldx, int
     if (!t || !x || n <= 0 || m < 0) return fail(UH_ERR_ARG, "uh_trsm: bad arguments");
     if (n > TRSM_MAX_N) return fail(UH_ERR_ARG, "uh_trsm: n=%d above %d", n, TRSM_MAX_N);
     if (ldt < (uint64_t)n) return fail(UH_ERR_ARG, "uh_trsm: ldt too small");
+    if (((uint64_t)(n - 1) * ldt + n) * 4 >= (1ull << 31))
+        return fail(UH_ERR_ARG, "uh_trsm: the triangle spans 2 GiB or more (n=%d, ldt=%llu)", n,
+                    (unsigned long long)ldt);
     if (kind == 0 && ldx < (uint64_t)n) return fail(UH_ERR_ARG, "uh_trsm: ldx too small");
     if (kind == 1 && ldx < (uint64_t)m) return fail(UH_ERR_ARG, "uh_trsm: ldx too small");
     if (m == 0) return UH_OK;
@@ -342,8 +639,7 @@ static int trsm_common(const float* t, uint64_t ldt, float* x, uint64_t ldx, int
         record_item(b, FAM_TRSM, &it, sizeof(it), trsm_tiles(it));
         return UH_OK;
     }
-    const size_t shm = (size_t)n * 33 * sizeof(float);
-    UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), shm, (hipStream_t)stream, it);
+    UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), 0, (hipStream_t)stream, it);
     UH_LAUNCH_CHECK("k_trsm");
     g_plain_launches++;
     return UH_OK;
@@ -407,7 +703,7 @@ extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, 
         rc = uh_trsm_lln(lu + (uint64_t)b0 * ld + b0, ld, rhs + b0, (uint64_t)n, bn, nrhs, s);
         if (rc) return rc;
         if (b0 + bn < n) {
-            UH_LAUNCH(k_update_f32, dim3((n - b0 - bn + 255) / 256), dim3(256), 0, s, lu, ld, b0 + bn, n,
+            UH_LAUNCH(k_update_f32, dim3((n - b0 - bn + 63) / 64), dim3(256), 0, s, lu, ld, b0 + bn, n,
                                b0, b0 + bn, rhs, (uint64_t)n, nrhs);
             UH_LAUNCH_CHECK("k_update_f32");
             g_plain_launches++;
@@ -419,11 +715,11 @@ extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, 
         const int b0 = max(0, b1 - B);
         const int bn = b1 - b0;
         TrsmItem it{lu + (uint64_t)b0 * ld + b0, rhs + b0, ld, (uint64_t)n, bn, nrhs, 2, 0};
-        UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), (size_t)bn * 33 * sizeof(float), s, it);
+        UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), 0, s, it);
         UH_LAUNCH_CHECK("k_trsm");
         g_plain_launches++;
         if (b0 > 0) {
-            UH_LAUNCH(k_update_f32, dim3((b0 + 255) / 256), dim3(256), 0, s, lu, ld, 0, b0, b0, b1, rhs,
+            UH_LAUNCH(k_update_f32, dim3((b0 + 63) / 64), dim3(256), 0, s, lu, ld, 0, b0, b0, b1, rhs,
                                (uint64_t)n, nrhs);
             UH_LAUNCH_CHECK("k_update_f32");
             g_plain_launches++;
