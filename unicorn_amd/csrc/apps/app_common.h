@@ -108,6 +108,16 @@ inline void autotune_gemm(int M, int N, int K, int precision) {
     uh_gemm_autotune(M, N, K, 0, 5, 0, nullptr, nullptr);
 }
 
+// Tunes on the shape (M, N, K) and records the winner for the shape class of (m, n, k): for
+// products that run batched, where the batch rather than one product fills the device.
+inline void autotune_gemm_as(int M, int N, int K, int m, int n, int k, int precision) {
+    if (precision != 0 || !pmHasHipDevice()) return;
+    const char* e = getenv("UA_GEMM_AUTOTUNE");
+    if (e && atoi(e) == 0) return;
+    int tile = -1;
+    if (uh_gemm_autotune(M, N, K, 0, 5, 0, &tile, nullptr) == 0 && tile >= 0) uh_gemm_set_tile(m, n, k, 0, tile);
+}
+
 void set_error(const std::string& s);
 
 }  // namespace apps

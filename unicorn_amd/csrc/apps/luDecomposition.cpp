@@ -19,6 +19,7 @@
 #include "app_common.h"
 #include "unicorn_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -395,7 +396,12 @@ extern "C" void* ua_lu_create(int n, int block, int precision) {
         return nullptr;
     }
     (void)bytes;
-    apps::autotune_gemm(block, block, block, precision);   // the stage-3 trailing-update shape
+    // Stage 3 runs as batches of (nb-k-1)^2 independent B^3 products, so the tile is tuned on
+    // one batch seen as a single product (g B square, K = B, g = min(nb - 1, 8)) and recorded for
+    // the B^3 shape; with one trailing block the single product is the shape.
+    const int g = std::min(n / block - 1, 8);
+    if (g >= 2) apps::autotune_gemm_as(g * block, g * block, block, block, block, block, precision);
+    else apps::autotune_gemm(block, block, block, precision);
     return in;
 }
 

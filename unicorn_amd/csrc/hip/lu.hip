@@ -492,6 +492,151 @@ int trsm_persistent(const uint8_t* queue, hipStream_t s) {
     return UH_OK;
 }
 
+// ---------------------------------------------------------------- sync-free triangular solve
+
+// The refinement's solve L U z = r for nrhs <= 8 vectors, one launch per triangle. Rows are cut
+// into 32-row blocks and a workgroup claims the next block by ticket (forward: top down;
+// backward: bottom up), so a block depends only on blocks claimed earlier, by workgroups that
+// are already running: the lowest unfinished ticket can always proceed, whatever the number of
+// resident workgroups. A workgroup streams its row panel in chunks of 8 tiles of 32 x 32 (the
+// loads do not depend on other blocks and run one chunk ahead), waits for each tile's block flag
+// before reading that block's solution, substitutes through its diagonal tile and publishes its
+// block: stores, agent-scope release, flag. Every wait is bounded: past the limit the block is
+// written as NaN and published anyway, so every workgroup exits and the refinement sees a
+// non-finite residual.
+static constexpr int TV_CH = 8;   // tiles per chunk (one per group of 32 threads)
+
+struct TrsvArgs {
+    const float* lu;
+    uint64_t ld;
+    float* z;          // n x nrhs column-major, leading dimension n, solved in place
+    uint32_t* sync;    // [0] ticket, [1 + t] flag of ticket t; zero before the launch
+    int32_t n, nrhs, upper, spin_limit;
+};
+
+__global__ void __launch_bounds__(256) k_trsv_sync_free(TrsvArgs a) {
+    __shared__ float zs[TV_CH * 32][8];
+    __shared__ float red[TV_CH][32][9];
+    __shared__ uint32_t s_t;
+    __shared__ int s_fail;
+    const int tid = threadIdx.x, r = tid & 31, cc = tid >> 5, lane = tid & 63;
+    const int n = a.n, nb = (n + 31) / 32;
+    const int64_t ld = (int64_t)a.ld;
+    const UH_GLOBAL float* lu = as_global(a.lu);
+    UH_GLOBAL float* z = as_global(a.z);
+    for (;;) {
+        if (tid == 0) {
+            s_t = atomicAdd(a.sync, 1u);
+            s_fail = 0;
+        }
+        __syncthreads();
+        const int t = (int)__builtin_amdgcn_readfirstlane(s_t);
+        if (t >= nb) break;
+        const int p = a.upper ? nb - 1 - t : t;   // row block
+        const int R = p * 32 + r;                  // this thread's row
+        auto colblock = [&](int d) { return a.upper ? nb - 1 - d : d; };   // row block of ticket d
+        auto load_tile = [&](int d, float (&l)[32]) {   // l[j] = M(R, 32 colblock(d) + j)
+            const int c0 = colblock(d) * 32;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const bool ok = d < t && R < n && c0 + j < n;
+                const UH_GLOBAL float* q = ok ? lu + (int64_t)(c0 + j) * ld + R : lu;
+                const float v = *q;
+                l[j] = ok ? v : 0.f;
+            }
+        };
+        // the diagonal tile's row R and the pivot
+        float dl[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const bool ok = R < n && p * 32 + j < n;
+            const UH_GLOBAL float* q = ok ? lu + (int64_t)(p * 32 + j) * ld + R : lu;
+            const float v = *q;
+            dl[j] = ok ? v : 0.f;
+        }
+        const float piv = R < n ? lu[(int64_t)R * ld + R] : 1.f;
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float lc[32], ln[32];
+        load_tile(cc, lc);
+        for (int d0 = 0; d0 < t; d0 += TV_CH) {
+            if (d0 + TV_CH < t) load_tile(d0 + TV_CH + cc, ln);
+            if (tid < TV_CH && d0 + tid < t) {
+                uint32_t* f = a.sync + 1 + d0 + tid;
+                int spins = 0;
+                while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                    if (++spins > a.spin_limit) {
+                        s_fail = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {   // zs[32 dd + j][k] = z(32 colblock(d0 + dd) + j, k)
+                const int e = u * 256 + tid, k = e & 7, jj = e >> 3, dd = jj >> 5, j = jj & 31;
+                const int col = colblock(d0 + dd) * 32 + j;
+                const bool ok = d0 + dd < t && col < n && k < a.nrhs;
+                const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + col : z;
+                const float v = *q;
+                zs[jj][k] = ok ? v : 0.f;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const float4 z0 = *(const float4*)&zs[cc * 32 + j][0];
+                const float4 z1 = *(const float4*)&zs[cc * 32 + j][4];
+                acc[0] = fmaf(lc[j], z0.x, acc[0]);
+                acc[1] = fmaf(lc[j], z0.y, acc[1]);
+                acc[2] = fmaf(lc[j], z0.z, acc[2]);
+                acc[3] = fmaf(lc[j], z0.w, acc[3]);
+                acc[4] = fmaf(lc[j], z1.x, acc[4]);
+                acc[5] = fmaf(lc[j], z1.y, acc[5]);
+                acc[6] = fmaf(lc[j], z1.z, acc[6]);
+                acc[7] = fmaf(lc[j], z1.w, acc[7]);
+            }
+#pragma unroll
+            for (int j = 0; j < 32; ++j) lc[j] = ln[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) red[cc][r][k] = acc[k];
+        __syncthreads();
+        // thread (row r, vector k = cc): x = rhs - sum over the chunk groups in a fixed order
+        const int k = cc;
+        float sum = 0.f;
+#pragma unroll
+        for (int g = 0; g < TV_CH; ++g) sum += red[g][r][k];
+        const bool mine = R < n && k < a.nrhs;
+        float x = mine ? z[(int64_t)k * n + R] - sum : 0.f;
+        // substitution through the diagonal tile; lanes 0..31 hold vector 2w, 32..63 vector 2w+1
+        if (!a.upper) {
+            const float rd = 1.0f / piv;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                if (r == j) x = x * rd;
+                const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+                const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32 + j));
+                const float xj = lane < 32 ? x0 : x1;
+                if (r > j) x = fmaf(-dl[j], xj, x);
+            }
+        } else {
+#pragma unroll
+            for (int j = 31; j >= 0; --j) {
+                const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+                const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32 + j));
+                const float xj = lane < 32 ? x0 : x1;
+                if (r < j) x = fmaf(-dl[j], xj, x);
+            }
+        }
+        if (s_fail) x = __int_as_float(0x7fc00000);
+        if (mine) z[(int64_t)k * n + R] = x;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(a.sync + 1 + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ---------------------------------------------------------------- refinement helpers
 
 // part[c][i][k] = sum_{j in chunk c} A(i, j) * x(j, k), A fp32 column-major, x fp64; nrhs <= 8
@@ -532,46 +677,6 @@ __global__ void __launch_bounds__(256) k_residual_sum(const double* part, int ch
         const double v = b[(uint64_t)k * n + i] - s;
         r[(uint64_t)k * n + i] = v;
         if (rf) rf[(uint64_t)k * n + i] = (float)v;
-    }
-}
-
-// y(i, k) -= sum_{j in [j0, j1)} A(i, j) * z(j, k) for rows [i0, i1): fp32. A workgroup owns 64
-// rows; wave w takes the columns j = j0 + w (mod 4) and the four partial sums are added in a
-// fixed order.
-__global__ void __launch_bounds__(256) k_update_f32(const float* a_, uint64_t lda, int i0, int i1, int j0,
-                                                    int j1, float* z_, uint64_t ldz, int nrhs) {
-    __shared__ float zs[RES_CHUNK][8];
-    __shared__ float part[3][8][64];
-    const UH_GLOBAL float* a = as_global(a_);
-    UH_GLOBAL float* z = as_global(z_);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = i0 + blockIdx.x * 64 + lane;
-    for (int e = threadIdx.x; e < (j1 - j0) * 8; e += 256) {
-        const int j = e / 8, k = e % 8;
-        zs[j][k] = k < nrhs ? z[(uint64_t)k * ldz + j0 + j] : 0.f;
-    }
-    __syncthreads();
-    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < i1) {
-#pragma unroll 4
-        for (int j = j0 + w; j < j1; j += 4) {
-            const float av = a[(uint64_t)j * lda + i];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) s[k] = fmaf(av, zs[j - j0][k], s[k]);
-        }
-    }
-    if (w > 0) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) part[w - 1][k][lane] = s[k];
-    }
-    __syncthreads();
-    if (w == 0 && i < i1) {
-        for (int k = 0; k < nrhs; ++k) {
-            float t = s[k] + part[0][k][lane];
-            t += part[1][k][lane];
-            t += part[2][k][lane];
-            z[(uint64_t)k * ldz + i] -= t;
-        }
     }
 }
 
@@ -690,40 +795,38 @@ extern "C" int uh_residual_f64(const float* a, uint64_t lda, int n, const double
     return UH_OK;
 }
 
+// Per-stream sync words of the triangular solves: solves on one stream run in stream order and
+// share them; solves on different streams get their own.
+static std::map<hipStream_t, std::pair<uint32_t*, size_t>> g_trsv_sync;
+static std::mutex g_trsv_mtx;
+static constexpr int TV_SPIN_LIMIT = 1 << 22;   // polls of one flag, about a second
+
 extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, int nrhs, void* stream) {
     if (!lu || !rhs || n <= 0 || nrhs <= 0 || nrhs > 8 || ld < (uint64_t)n)
         return fail(UH_ERR_ARG, "uh_lu_solve_f32: bad arguments");
     int rc;
     hipStream_t s = plain_stream(stream, &rc);
     if (rc) return rc;
-    const int B = RES_CHUNK;
-    // forward: L z = r, block by block (diagonal solve, then update of the rows below)
-    for (int b0 = 0; b0 < n; b0 += B) {
-        const int bn = min(B, n - b0);
-        rc = uh_trsm_lln(lu + (uint64_t)b0 * ld + b0, ld, rhs + b0, (uint64_t)n, bn, nrhs, s);
-        if (rc) return rc;
-        if (b0 + bn < n) {
-            UH_LAUNCH(k_update_f32, dim3((n - b0 - bn + 63) / 64), dim3(256), 0, s, lu, ld, b0 + bn, n,
-                               b0, b0 + bn, rhs, (uint64_t)n, nrhs);
-            UH_LAUNCH_CHECK("k_update_f32");
-            g_plain_launches++;
+    const int nb = (n + 31) / 32;
+    const size_t words = 2 * (size_t)(nb + 1);   // forward then backward
+    uint32_t* sync;
+    {
+        std::lock_guard<std::mutex> lk(g_trsv_mtx);
+        auto& e = g_trsv_sync[s];
+        if (e.second < words) {
+            if (e.first) { UH_HIP(hipStreamSynchronize(s)); UH_HIP(hipFree(e.first)); e.first = nullptr; e.second = 0; }
+            UH_HIP(hipMalloc(&e.first, words * sizeof(uint32_t)));
+            e.second = words;
         }
+        sync = e.first;
     }
-    // backward: U x = z, U unit upper, blocks from the bottom (diagonal solve on reversed
-    // indices, then update of the rows above)
-    for (int b1 = n; b1 > 0; b1 -= B) {
-        const int b0 = max(0, b1 - B);
-        const int bn = b1 - b0;
-        TrsmItem it{lu + (uint64_t)b0 * ld + b0, rhs + b0, ld, (uint64_t)n, bn, nrhs, 2, 0};
-        UH_LAUNCH(k_trsm, dim3(trsm_tiles(it)), dim3(256), 0, s, it);
-        UH_LAUNCH_CHECK("k_trsm");
+    rc = uh_fill_u32(sync, 0u, words, s);
+    if (rc) return rc;
+    for (int upper = 0; upper < 2; ++upper) {
+        TrsvArgs a{lu, ld, rhs, sync + upper * (nb + 1), n, nrhs, upper, TV_SPIN_LIMIT};
+        UH_LAUNCH(k_trsv_sync_free, dim3((unsigned)nb), dim3(256), 0, s, a);
+        UH_LAUNCH_CHECK("k_trsv_sync_free");
         g_plain_launches++;
-        if (b0 > 0) {
-            UH_LAUNCH(k_update_f32, dim3((b0 + 63) / 64), dim3(256), 0, s, lu, ld, 0, b0, b0, b1, rhs,
-                               (uint64_t)n, nrhs);
-            UH_LAUNCH_CHECK("k_update_f32");
-            g_plain_launches++;
-        }
     }
     return UH_OK;
 }
