@@ -8,20 +8,18 @@
 //
 // Design. The reference stages every tile through a D2D copy into pitched texture memory and
 // reads three separate texels per pixel. Here each workgroup computes 256 x 16 output tiles
-// straight from the (HBM-resident) view, several tiles per workgroup:
-//   1. fetch: the (16+2r) x (256+8) input window is read into registers, 4 pixels (12 bytes)
-//      per group, each group with one 16-byte global load from the dword-aligned address below
-//      it; all loads of a tile are issued back to back;
-//   2. store: v_alignbyte extracts the 12 bytes, v_perm de-interleaves them into three channel
-//      planes, XOR 0x80 biases the bytes so that they can be used as signed int8, and the
-//      planes go to LDS;
+// straight from the (HBM-resident) view, several tiles per workgroup, as a pipeline:
+//   1. stage: the (16+2r)-row input window (264 pixels from x0 - 4) moves from HBM to LDS with
+//      16-byte direct-to-LDS buffer loads, one tile ahead of the compute, holding no registers;
+//   2. de-interleave: v_alignbyte extracts each 4-pixel group's 12 bytes, v_perm splits them into
+//      three channel planes, XOR 0x80 biases the bytes so that they can be used as signed int8;
 //   3. compute: each lane computes 4 consecutive pixels x 4 rows x 3 channels. For a filter row,
 //      the 2r+1 taps of pixel x+q lie in two aligned LDS dwords; v_dot4_i32_i8 against weight
 //      vectors pre-shifted for each of the 4 pixel phases gives 2 instructions per (pixel,
 //      channel, filter row) with no byte extraction. The accumulators start at 128 * sum(w),
 //      which undoes the bias; mode 0 keeps the low byte (the reference's char accumulators,
 //      i.e. arithmetic modulo 256), mode 1 shifts right arithmetically and clamps to [0, 255].
-// The per-item arguments and weight vectors are scalar loads (SGPRs); image loads and stores use
+// The per-item arguments and weight vectors are scalar loads (SGPRs); image stores use
 // global-space pointers, so LDS waits never wait for HBM traffic.
 // Radii 1 and 2 take this path; other radii use a direct per-pixel path.
 #include "uh_internal.h"
@@ -37,9 +35,6 @@ static constexpr int FT_RPT = 4;      // output rows per lane
 static constexpr int FT_THREADS = 256;
 #ifndef FT_WAVES_PER_EU
 #define FT_WAVES_PER_EU 4             // occupancy target: 4 waves per SIMD (<= 128 VGPRs)
-#endif
-#ifndef FT_PREFETCH
-#define FT_PREFETCH 0                 // 1: fetch tile n+1 into registers while tile n is computed
 #endif
 static constexpr int FT_LW = FT_W + 8;        // LDS plane width in pixels (4 either side)
 static constexpr int FT_LWD = FT_LW / 4;      // ... in dwords = 4-pixel groups per staged row
@@ -82,14 +77,6 @@ struct FilterView {
     int first_row, last_row, first_col, last_col;  // view extent in image coordinates
 };
 
-// One tile's input window in registers, between its global loads and its LDS stores. Group i of
-// a lane holds the four dwords that start at the dword-aligned address at or below the group's
-// first byte; bits 2i..2i+1 of `sh` give the group's byte offset within them.
-struct FilterFetch {
-    uint32_t d[FT_MAX_ITERS][4];
-    uint32_t sh;
-};
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ __forceinline__ FilterView filter_view(const FilterArgs& a) {
@@ -103,69 +90,112 @@ __device__ __forceinline__ FilterView filter_view(const FilterArgs& a) {
 }
 
 // Tile geometry shared by the plain and the persistent kernels.
+// Tiles are numbered down each 256-pixel column strip, so consecutive tiles are vertical
+// neighbours and share halo rows.
 __device__ __forceinline__ void filter_tile_origin(const FilterArgs& a, uint32_t tile, int& x0, int& y0) {
-    const int tiles_x = (a.cols + FT_W - 1) / FT_W;
-    x0 = a.col0 + (int)(tile % (uint32_t)tiles_x) * FT_W;
-    y0 = a.row0 + (int)(tile / (uint32_t)tiles_x) * FT_H;
+    const uint32_t tiles_y = (uint32_t)((a.rows + FT_H - 1) / FT_H);
+    x0 = a.col0 + (int)(tile / tiles_y) * FT_W;
+    y0 = a.row0 + (int)(tile % tiles_y) * FT_H;
 }
 
-// Fetch the input window of one tile. Group g (= staged row lr, group column gc) covers image
-// columns x0 - 4 + 4gc .. +3 of image row y0 - R + lr, both clamped to the view.
+// Staged raw input. Row lr of a tile's window is the 816 bytes (51 16-byte chunks) from the
+// 16-byte-aligned address at or below the window's first byte, pixel x0 - 4 of image row
+// y0 - R + lr (rows clamped to the view). The chunks go straight from HBM to LDS
+// (buffer_load_dwordx4 ... lds: no registers hold data in flight) through a buffer resource
+// spanning the view, so chunks before or past the view read as zero instead of touching memory;
+// the de-interleave below reads only pixels inside the view (columns clamped to it), which lie
+// in the staged bytes.
+static constexpr int FT_RAW_ROW = 816;
+static constexpr int FT_RAW_CHUNKS = FT_RAW_ROW / 16;
+// whole 1 KB wave destinations (64 lanes x 16 bytes), so a wave's last chunk group stays inside
+static constexpr int FT_RAW_DWORDS = (FtGeom<2>::LROWS * FT_RAW_ROW + 1023) / 1024 * 256;
+
+struct FilterLds {
+    uint32_t planes[FT_LDS_DWORDS];
+    uint32_t raw[FT_RAW_DWORDS];
+};
+
+// Offset from the view pointer of the first window byte of staged row lr (may be negative).
 template <int R>
-__device__ __forceinline__ void filter_fetch(const FilterArgs& a, uint32_t tile, FilterFetch& f) {
+__device__ __forceinline__ int64_t filter_row_start(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr) {
+    const int m = clampi(y0 - R + lr, v.first_row, v.last_row);
+    return (int64_t)(a.view_last_row - m) * (int64_t)a.in_stride + (int64_t)(x0 - 4 - v.first_col) * 3;
+}
+
+template <int R>
+__device__ __forceinline__ void filter_dma(const FilterArgs& a, uint32_t tile, uint32_t* raw) {
+    constexpr int CHUNKS = FtGeom<R>::LROWS * FT_RAW_CHUNKS;
+    int x0, y0;
+    filter_tile_origin(a, tile, x0, y0);
+    const FilterView v = filter_view(a);
+    // The view spans [in, in + extent) (rows in inverted order; the host keeps it below 2 GiB).
+    // The resource starts at the 16-byte boundary at or below `in` and ends at the one at or
+    // above the view's end: every 16-byte chunk holding a view byte is read whole (an aligned
+    // chunk never leaves the page of its bytes), chunks past the end read as zero, and a chunk
+    // wholly before the view (a window starting left of it) is redirected to offset 0.
+    const int64_t mis = (int64_t)((uintptr_t)a.in & 15u);
+    const int64_t extent = (int64_t)(a.view_last_row - v.first_row) * (int64_t)a.in_stride + (int64_t)a.pad * 3;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.in - mis), 0, (int)((mis + extent + 15) & ~(int64_t)15), 0x00020000);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
+        const int c0 = i * FT_THREADS + wave * 64;   // this wave's first chunk: LDS destination base
+        if (c0 >= CHUNKS) break;
+        const int c = c0 + lane;
+        if (c < CHUNKS) {
+            const int lr = c / FT_RAW_CHUNKS, ch = c - lr * FT_RAW_CHUNKS;
+            const int64_t st = filter_row_start<R>(a, v, x0, y0, lr) + mis;   // from the aligned base
+            const int64_t off = (st & ~(int64_t)15) + ch * 16;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)((uint8_t*)raw + c0 * 16), 16,
+                (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
+        }
+    }
+}
+
+// De-interleave the staged window into three biased channel planes: plane[k][lr][gc] holds
+// channel k of the 4 pixels x0 - 4 + 4 gc .. +3 of staged row lr (LDS dword lr * FT_LWD + gc == g).
+template <int R>
+__device__ __forceinline__ void filter_unpack(const FilterArgs& a, uint32_t tile, const uint32_t* raw, uint32_t* lds) {
     using G = FtGeom<R>;
     int x0, y0;
     filter_tile_origin(a, tile, x0, y0);
     const FilterView v = filter_view(a);
-    const UH_GLOBAL uint8_t* in = as_global(a.in);
-    f.sh = 0;
-#pragma unroll
+    const int64_t mis = (int64_t)((uintptr_t)a.in & 15u);
+    const uint8_t* rawb = (const uint8_t*)raw;
+#pragma unroll 2
     for (int i = 0; i < G::ITERS; ++i) {
         const int g = threadIdx.x + i * FT_THREADS;
         if (i == G::ITERS - 1 && g >= G::GROUPS) break;
         const int lr = g / FT_LWD;
         const int gc = g - lr * FT_LWD;
-        const int m = clampi(y0 - R + lr, v.first_row, v.last_row);
-        const UH_GLOBAL uint8_t* row = in + (uint64_t)(a.view_last_row - m) * a.in_stride;
-        const int p = x0 - 4 + gc * 4;             // first image column of the group
-        if (p - 1 >= v.first_col && p + 5 <= v.last_col) {
-            // Pixels p-1 .. p+5 are in the view row, so the 16 bytes from the dword-aligned
-            // address at or below pixel p (which start at most 3 bytes before it) stay inside.
-            const UH_GLOBAL uint8_t* src = row + (uint64_t)(p - v.first_col) * 3;
-            const uint32_t sh = (uint32_t)((uintptr_t)src & 3u);
-            const UH_GLOBAL uint32_t* base = (const UH_GLOBAL uint32_t*)(src - sh);
-            f.d[i][0] = base[0]; f.d[i][1] = base[1]; f.d[i][2] = base[2]; f.d[i][3] = base[3];
-            f.sh |= sh << (2 * i);
+        const int64_t st = filter_row_start<R>(a, v, x0, y0, lr);
+        const int rb = lr * FT_RAW_ROW + (int)((st + mis) & 15);   // raw byte of pixel x0 - 4
+        const int p = x0 - 4 + gc * 4;                            // first image column of the group
+        uint32_t p0, p1, p2;
+        if (p >= v.first_col && p + 3 <= v.last_col) {
+            // 12 bytes at rb + 12 gc, inside the staged row: the four dwords from the one below
+            const int o = rb + 12 * gc;
+            const uint32_t* q = raw + (o >> 2);
+            const uint32_t s = (uint32_t)o & 3u;
+            const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
+            p0 = __builtin_amdgcn_alignbyte(d1, d0, s);
+            p1 = __builtin_amdgcn_alignbyte(d2, d1, s);
+            p2 = __builtin_amdgcn_alignbyte(d3, d2, s);
         } else {
-            // Group at an image or view edge: clamped per-pixel byte loads, offset 0.
+            // group at a view edge: pixels clamped to the view, read byte by byte
             uint32_t b[12];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int n = clampi(p + k, v.first_col, v.last_col);
-                const UH_GLOBAL uint8_t* s = row + (uint64_t)(n - v.first_col) * 3;
-                b[3 * k + 0] = s[0]; b[3 * k + 1] = s[1]; b[3 * k + 2] = s[2];
+                const uint8_t* px = rawb + rb + (n - (x0 - 4)) * 3;
+                b[3 * k + 0] = px[0]; b[3 * k + 1] = px[1]; b[3 * k + 2] = px[2];
             }
-            f.d[i][0] = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
-            f.d[i][1] = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
-            f.d[i][2] = b[8] | (b[9] << 8) | (b[10] << 16) | (b[11] << 24);
-            f.d[i][3] = 0;
+            p0 = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+            p1 = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
+            p2 = b[8] | (b[9] << 8) | (b[10] << 16) | (b[11] << 24);
         }
-    }
-}
-
-// Store a fetched window as three biased channel planes: plane[k][lr][gc] holds channel k of
-// the group's 4 pixels (LDS dword index lr * FT_LWD + gc == g).
-template <int R>
-__device__ __forceinline__ void filter_store(const FilterFetch& f, uint32_t* lds) {
-    using G = FtGeom<R>;
-#pragma unroll
-    for (int i = 0; i < G::ITERS; ++i) {
-        const int g = threadIdx.x + i * FT_THREADS;
-        if (i == G::ITERS - 1 && g >= G::GROUPS) break;
-        const uint32_t s = (f.sh >> (2 * i)) & 3u;
-        const uint32_t p0 = __builtin_amdgcn_alignbyte(f.d[i][1], f.d[i][0], s);
-        const uint32_t p1 = __builtin_amdgcn_alignbyte(f.d[i][2], f.d[i][1], s);
-        const uint32_t p2 = __builtin_amdgcn_alignbyte(f.d[i][3], f.d[i][2], s);
         // v_perm selects bytes of {S0:S1} (S1 = bytes 0-3, S0 = bytes 4-7; 0x0c gives zero).
         // Plane k takes bytes k, k+3, k+6, k+9 of the 12-byte group p0:p1:p2.
         const uint32_t t0 = __builtin_amdgcn_perm(p1, p0, 0x0c060300u);   // p0.0 p0.3 p1.2
@@ -179,6 +209,7 @@ __device__ __forceinline__ void filter_store(const FilterFetch& f, uint32_t* lds
         lds[2 * G::LROWS * FT_LWD + g] = pl2 ^ 0x80808080u;
     }
 }
+
 
 // Weight vectors: pixel x+q of lane l (x = x0 + 4l, LDS dword l+1 holds pixels x..x+3) needs
 // LDS columns 4l+4+q-R .. 4l+4+q+R. Column c (relative to 4l) lives in dword c/4, byte c%4;
@@ -212,19 +243,17 @@ __device__ __forceinline__ uint32_t pack4(uint32_t b0, uint32_t b1, uint32_t b2,
     return __builtin_amdgcn_perm(b1, b0, 0x0c0c0400u) | __builtin_amdgcn_perm(b3, b2, 0x04000c0cu);
 }
 
-template <int R, int MODE>
-__device__ __forceinline__ void filter_compute(const FilterArgs& a, const FilterWeights& w, uint32_t tile,
-                                               const uint32_t* lds) {
+typedef int FilterAcc[FT_RPT][4][3];
+
+// Dot products of one tile: lane l of wave w, pixels x0 + 4 l .. +3 of rows y0 + 4 w .. +3.
+template <int R>
+__device__ __forceinline__ void filter_accumulate(const FilterWeights& w, const uint32_t* lds, FilterAcc& acc) {
     constexpr int D = 2 * R + 1;
     constexpr int LROWS = FtGeom<R>::LROWS;
-    int x0, y0;
-    filter_tile_origin(a, tile, x0, y0);
     const int bias = 128 * w.wsum;               // undoes the XOR-0x80 bias of the inputs
-
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ry0 = wave * FT_RPT;               // first output row of this lane (tile-relative)
-    int acc[FT_RPT][4][3];
 #pragma unroll
     for (int r = 0; r < FT_RPT; ++r)
 #pragma unroll
@@ -260,8 +289,15 @@ __device__ __forceinline__ void filter_compute(const FilterArgs& a, const Filter
             }
         }
     }
+}
 
-    // Epilogue: apply the mode, swap channels (out c <- in 2-c), store 12 bytes per row.
+// Epilogue: apply the mode, swap channels (out c <- in 2-c), store 12 bytes per row.
+template <int MODE>
+__device__ __forceinline__ void filter_epilogue(const FilterArgs& a, uint32_t tile, const FilterAcc& acc) {
+    int x0, y0;
+    filter_tile_origin(a, tile, x0, y0);
+    const int lane = threadIdx.x & 63;
+    const int ry0 = (threadIdx.x >> 6) * FT_RPT;
     const int shift = a.shift;
     const int x = x0 + 4 * lane;
     const int cend = a.col0 + a.cols;
@@ -294,6 +330,7 @@ __device__ __forceinline__ void filter_compute(const FilterArgs& a, const Filter
         }
     }
 }
+
 
 // Direct path for any radius <= 15: one output pixel per thread.
 __device__ __forceinline__ void filter_pixel_direct(const FilterArgs& a, const int8_t* weights, int i, int j) {
@@ -331,7 +368,7 @@ __device__ __forceinline__ void filter_tile_direct(const FilterArgs& a, const in
 }
 
 // Kernel variants. Variant 0 dispatches on each item's radius and mode (uniform per tile; the
-// direct path fetches and stages nothing); variants 1..4 are specialised for one radius and mode
+// direct path stages nothing); variants 1..4 are specialised for one radius and mode
 // (V = 1 + 2 (R - 1) + MODE) and are launched only on items of that kind, which keeps the
 // register allocation of the common case free of the other paths.
 __host__ __device__ constexpr int filter_variant(int radius, int mode) {
@@ -339,172 +376,140 @@ __host__ __device__ constexpr int filter_variant(int radius, int mode) {
 }
 
 template <int V>
-__device__ __forceinline__ void filter_fetch_v(const FilterArgs& a, uint32_t tile, FilterFetch& f) {
+__device__ __forceinline__ bool filter_staged(const FilterArgs& a) {
+    if constexpr (V == 0) return a.radius == 1 || a.radius == 2;
+    else return true;
+}
+
+template <int V>
+__device__ __forceinline__ void filter_dma_v(const FilterArgs& a, uint32_t tile, uint32_t* raw) {
     if constexpr (V == 0) {
-        if (a.radius == 2) filter_fetch<2>(a, tile, f);
-        else if (a.radius == 1) filter_fetch<1>(a, tile, f);
+        if (a.radius == 2) filter_dma<2>(a, tile, raw);
+        else if (a.radius == 1) filter_dma<1>(a, tile, raw);
     } else {
-        filter_fetch<(V + 1) / 2>(a, tile, f);
+        filter_dma<(V + 1) / 2>(a, tile, raw);
     }
 }
 
 template <int V>
-__device__ __forceinline__ void filter_store_v(const FilterArgs& a, const FilterFetch& f, uint32_t* lds) {
+__device__ __forceinline__ void filter_unpack_v(const FilterArgs& a, uint32_t tile, const uint32_t* raw, uint32_t* lds) {
     if constexpr (V == 0) {
-        if (a.radius == 2) filter_store<2>(f, lds);
-        else if (a.radius == 1) filter_store<1>(f, lds);
+        if (a.radius == 2) filter_unpack<2>(a, tile, raw, lds);
+        else if (a.radius == 1) filter_unpack<1>(a, tile, raw, lds);
     } else {
-        filter_store<(V + 1) / 2>(f, lds);
+        filter_unpack<(V + 1) / 2>(a, tile, raw, lds);
     }
 }
 
-// weights: the item's full weight table (used by the direct path only)
+// Dot products of a staged tile; a tile of another radius (variant 0 only) is computed and
+// stored here by the direct path. weights: the item's full weight table (direct path only).
 template <int V>
-__device__ __forceinline__ void filter_compute_v(const FilterArgs& a, const FilterWeights& w,
-                                                 const int8_t* weights, uint32_t tile, const uint32_t* lds) {
+__device__ __forceinline__ void filter_compute_v(const FilterArgs& a, const FilterWeights& w, const int8_t* weights,
+                                                 uint32_t tile, const uint32_t* lds, FilterAcc& acc) {
     if constexpr (V == 0) {
-        if (a.radius == 2) {
-            if (a.mode == 0) filter_compute<2, 0>(a, w, tile, lds);
-            else filter_compute<2, 1>(a, w, tile, lds);
-        } else if (a.radius == 1) {
-            if (a.mode == 0) filter_compute<1, 0>(a, w, tile, lds);
-            else filter_compute<1, 1>(a, w, tile, lds);
-        } else {
-            filter_tile_direct(a, weights, tile);
+        if (a.radius == 2) filter_accumulate<2>(w, lds, acc);
+        else if (a.radius == 1) filter_accumulate<1>(w, lds, acc);
+        else filter_tile_direct(a, weights, tile);
+    } else {
+        filter_accumulate<(V + 1) / 2>(w, lds, acc);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void filter_epilogue_v(const FilterArgs& a, uint32_t tile, const FilterAcc& acc) {
+    if constexpr (V == 0) {
+        if (!filter_staged<0>(a)) return;
+        if (a.mode == 0) filter_epilogue<0>(a, tile, acc);
+        else filter_epilogue<1>(a, tile, acc);
+    } else {
+        filter_epilogue<(V + 1) % 2>(a, tile, acc);
+    }
+}
+
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }   // vmcnt(0)
+
+// The tile pipeline shared by both kernels: tiles l0 .. l1-1 of one item, whose arguments and
+// weights stay in scalar registers. Per tile n: the dot products of n from the LDS planes, then
+// (once every wave has left the planes) the de-interleave of tile n+1, whose raw window was
+// requested while tile n was de-interleaved, then the request for tile n+2's window, then n's
+// stores. No register holds a tile's input in flight, and the wait for a window covers stores
+// issued a whole tile earlier.
+template <int V>
+__device__ __forceinline__ void filter_run(FilterLds& sm, const FilterArgs& a, const FilterWeights& w,
+                                           const int8_t* table, uint32_t l0, uint32_t l1) {
+    if (l0 >= l1) return;
+    const bool staged = filter_staged<V>(a);
+    if (staged) filter_dma_v<V>(a, l0, sm.raw);
+    wait_vm_all();
+    __syncthreads();          // the window is in LDS; every wave has left the planes
+    if (staged) filter_unpack_v<V>(a, l0, sm.raw, sm.planes);
+    __syncthreads();
+    if (staged && l0 + 1 < l1) filter_dma_v<V>(a, l0 + 1, sm.raw);
+    for (uint32_t t = l0;; ++t) {
+        FilterAcc acc;
+        filter_compute_v<V>(a, w, table, t, sm.planes, acc);
+        const bool more = t + 1 < l1;
+        if (more) {
+            __syncthreads();      // every wave is done with the planes
+            wait_vm_all();
+            __syncthreads();      // tile t+1's window is in LDS
+            if (staged) filter_unpack_v<V>(a, t + 1, sm.raw, sm.planes);
+            __syncthreads();      // planes hold tile t+1; the raw window is free
+            if (staged && t + 2 < l1) filter_dma_v<V>(a, t + 2, sm.raw);
         }
-    } else {
-        filter_compute<(V + 1) / 2, (V + 1) % 2>(a, w, tile, lds);
+        filter_epilogue_v<V>(a, t, acc);
+        if (!more) break;
     }
 }
 
-// Plain launch: workgroup b computes tiles b, b + grid, b + 2 grid, ... of one item.
+// Plain launch: workgroup b computes tiles 2b and 2b + 1 of one item.
 template <int V>
 __global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
 k_imagefilter(FilterItem it, uint32_t tiles) {
-    __shared__ uint32_t lds[FT_LDS_DWORDS];
-    const FilterArgs a = head_of(it.a);
-#if FT_PREFETCH
-    // tile n+1 is fetched into registers while tile n is computed
-    uint32_t t = blockIdx.x;
-    if (t >= tiles) return;
-    FilterFetch f;
-    filter_fetch_v<V>(a, t, f);
-    for (;;) {
-        filter_store_v<V>(a, f, lds);
-        __syncthreads();
-        const uint32_t tn = t + gridDim.x;
-        if (tn < tiles) filter_fetch_v<V>(a, tn, f);
-        filter_compute_v<V>(a, it.w, it.a.weights, t, lds);
-        if (tn >= tiles) break;
-        __syncthreads();          // LDS is restaged for the next tile
-        t = tn;
-    }
-#else
-    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        FilterFetch f;
-        filter_fetch_v<V>(a, t, f);
-        filter_store_v<V>(a, f, lds);
-        __syncthreads();
-        filter_compute_v<V>(a, it.w, it.a.weights, t, lds);
-        __syncthreads();          // LDS is restaged for the next tile
-    }
-#endif
+    __shared__ FilterLds sm;
+    const uint32_t l0 = blockIdx.x * 2u;
+    filter_run<V>(sm, head_of(it.a), it.w, it.a.weights, l0, l0 + 2u < tiles ? l0 + 2u : tiles);
 }
 
-// Persistent launch over a device work queue. Tiles are claimed from an atomic counter one claim
-// ahead (the claim for the tile after next is issued before a tile is computed and published to
-// the workgroup after it), and the next tile is looked up (item search over the LDS copy of the
-// tile prefix, scalar loads of its arguments) while the current one is computed. Every workgroup
-// leaves once a claim returns an index past the total, so the grid always drains.
+// Persistent launch over a device work queue with a static schedule: worker w runs a contiguous
+// range of the queue's tile sequence (no per-tile atomics: a single counter hit by every
+// workgroup for every tile serialises across the XCDs), in segments that each lie in one item.
+// Workgroups are numbered so that the workers of one XCD (hardware workgroup b runs on XCD
+// b mod 8) are consecutive: neighbouring strips share an L2. Every workgroup leaves after its
+// range, so the grid always drains.
 static constexpr uint32_t FT_PREFIX_LDS = 1024;   // items whose tile prefix is searched in LDS
 
 template <int V>
 __global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
 k_imagefilter_persistent(const uint8_t* queue) {
-    __shared__ uint32_t lds[FT_LDS_DWORDS];
-    __shared__ uint32_t s_tile[2];
+    __shared__ FilterLds sm;
     __shared__ uint32_t s_prefix[FT_PREFIX_LDS + 1];
-    QueueHeader* h = (QueueHeader*)queue;
-    const QueueHeader hd = uniform_load(h);       // n_items, total_tiles, offsets (not next_tile)
+    const QueueHeader hd = uniform_load((const QueueHeader*)queue);   // n_items, total_tiles, offsets
     const FilterItem* items = (const FilterItem*)(queue + hd.items_offset);
     const uint32_t* prefix = (const uint32_t*)(queue + hd.prefix_offset);
-    const uint32_t total = hd.total_tiles, n_items = hd.n_items;
-    const bool lds_prefix = stage_prefix(prefix, n_items, s_prefix, FT_PREFIX_LDS + 1);
-    if (threadIdx.x == 0) {
-        s_tile[0] = atomicAdd(&h->next_tile, 1u);
-        s_tile[1] = atomicAdd(&h->next_tile, 1u);
-    }
+    const bool lds_prefix = stage_prefix(prefix, hd.n_items, s_prefix, FT_PREFIX_LDS + 1);
     __syncthreads();
-    auto locate = [&](uint32_t tile, uint32_t& item, uint32_t& local) {
-        if (lds_prefix) queue_locate_lds(s_prefix, n_items, tile, item, local);
-        else queue_locate(prefix, n_items, tile, item, local);
-    };
-    uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[0]);
-    uint32_t tn = __builtin_amdgcn_readfirstlane(s_tile[1]);
-    if (t >= total) return;
+    const uint32_t g = gridDim.x, b = blockIdx.x, total = hd.total_tiles;
+    const uint32_t worker = (g % 8u == 0u) ? (b % 8u) * (g / 8u) + b / 8u : b;
+    const uint32_t base = total / g, rem = total % g;
+    uint32_t t = worker * base + (worker < rem ? worker : rem);
+    const uint32_t end = t + base + (worker < rem ? 1u : 0u);
+    if (t >= end) return;
     uint32_t item, local;
-    locate(t, item, local);
-    FilterArgs a = uniform_load((const FilterArgs*)&items[item].a);
-#if FT_PREFETCH
-    // the input of tile n+1 is fetched into registers while tile n is computed
-    FilterFetch f;
-    filter_fetch_v<V>(a, local, f);
-    for (;;) {
-        filter_store_v<V>(a, f, lds);
-        __syncthreads();
-        uint32_t item_n = 0, local_n = 0;
-        FilterArgs an = a;
-        if (tn < total) {
-            locate(tn, item_n, local_n);
-            an = uniform_load((const FilterArgs*)&items[item_n].a);
-            filter_fetch_v<V>(an, local_n, f);
-        }
-        uint32_t claim = 0;
-        if (threadIdx.x == 0 && tn < total) claim = atomicAdd(&h->next_tile, 1u);
-        {
+    if (lds_prefix) queue_locate_lds(s_prefix, hd.n_items, t, item, local);
+    else queue_locate(prefix, hd.n_items, t, item, local);
+    while (t < end) {
+        const uint32_t item_end = uniform_load(prefix + item + 1);   // first tile of the next item
+        const uint32_t n = (end < item_end ? end : item_end) - t;
+        if (n) {
+            const FilterArgs a = uniform_load((const FilterArgs*)&items[item].a);
             const FilterWeights w = uniform_load(&items[item].w);
-            filter_compute_v<V>(a, w, items[item].a.weights, local, lds);
+            filter_run<V>(sm, a, w, items[item].a.weights, local, local + n);
         }
-        if (tn >= total) break;
-        if (threadIdx.x == 0) s_tile[0] = claim;
-        __syncthreads();          // LDS is restaged and s_tile is read below
-        t = tn;
-        tn = __builtin_amdgcn_readfirstlane(s_tile[0]);
-        item = item_n;
-        local = local_n;
-        a = an;
+        t += n;
+        ++item;
+        local = 0;
     }
-#else
-    for (;;) {
-        {
-            FilterFetch f;
-            filter_fetch_v<V>(a, local, f);
-            filter_store_v<V>(a, f, lds);
-        }
-        uint32_t claim = total;
-        if (threadIdx.x == 0 && tn < total) claim = atomicAdd(&h->next_tile, 1u);
-        __syncthreads();
-        // look up the next tile now; its scalar loads complete while this tile is computed
-        uint32_t item_n = 0, local_n = 0;
-        FilterArgs an = a;
-        if (tn < total) {
-            locate(tn, item_n, local_n);
-            an = uniform_load((const FilterArgs*)&items[item_n].a);
-        }
-        {
-            const FilterWeights w = uniform_load(&items[item].w);
-            filter_compute_v<V>(a, w, items[item].a.weights, local, lds);
-        }
-        if (tn >= total) break;
-        if (threadIdx.x == 0) s_tile[0] = claim;
-        __syncthreads();          // LDS is restaged and s_tile is read below
-        t = tn;
-        tn = __builtin_amdgcn_readfirstlane(s_tile[0]);
-        item = item_n;
-        local = local_n;
-        a = an;
-    }
-#endif
 }
 
 uint32_t filter_tiles(const FilterItem& it) {
@@ -572,6 +577,10 @@ static int filter_validate(const uh_imagefilter_args* a, int* view_cols) {
     if ((uint64_t)(lc - fc + 1) * 3 > a->in_stride || (uint64_t)a->cols * 3 > a->out_stride)
         return fail(UH_ERR_ARG, "uh_imagefilter: stride smaller than a row");
     *view_cols = lc - fc + 1;
+    // the staged path addresses the view through a buffer resource with 32-bit offsets
+    const int fr = a->row0 - r < 0 ? 0 : a->row0 - r;
+    if ((r == 1 || r == 2) && (uint64_t)(lr - fr) * a->in_stride + (uint64_t)(lc - fc + 1) * 3 + 32 >= (1ull << 31))
+        return fail(UH_ERR_ARG, "uh_imagefilter: the view spans 2 GiB or more");
     // The weighted sum must fit int32 for mode 1 (always true for |w| <= 127, r <= 15).
     return UH_OK;
 }
