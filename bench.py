@@ -272,6 +272,8 @@ def bench_lu(args, U):
             fact.append(t1 - t0)
             solve.append(t2 - t1)
             infos.append(info)
+        print(f"lu: step {step + 1}/{args.warmup + args.steps} factor {t1 - t0:.4f} s solve {t2 - t1:.4f} s",
+              file=sys.stderr, flush=True)
     fwd = float(np.abs(x - x_true).max() / np.abs(x_true).max())
     lu.close()
     del M, A
@@ -286,6 +288,7 @@ def bench_lu(args, U):
     b2 = np.asfortranarray(M2.T.astype(np.float64) @ x2)
     small = {}
     for name, prec in (("split_bf16", 0), ("plain_fp32", 2)):
+        print(f"lu: N={n2} comparison, {name}", file=sys.stderr, flush=True)
         l2 = U.LU(n2, bs, prec)
         l2.set_matrix(M2.T)
         l2.prefetch()
@@ -307,6 +310,7 @@ def bench_lu(args, U):
     if not args.no_cpu_baseline:
         from oracle import oracle_lib as O
         n3 = 2048
+        print(f"lu: CPU baseline at N={n3}", file=sys.stderr, flush=True)
         M3 = np.asfortranarray(rng.random((n3, n3), dtype=np.float32) + n3 * np.eye(n3, dtype=np.float32))
         t0 = time.perf_counter()
         O.lu(M3)
