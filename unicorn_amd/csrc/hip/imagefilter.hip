@@ -115,28 +115,33 @@ struct FilterLds {
     uint32_t raw[FT_RAW_DWORDS];
 };
 
-// Offset from the view pointer of the first window byte of staged row lr (may be negative).
+// Offset from the view pointer of the first window byte of staged row lr (may be negative). The
+// host keeps the view below 2 GiB and the stride below 2^24, so one full-rate 24-bit multiply
+// gives the row offset.
 template <int R>
-__device__ __forceinline__ int64_t filter_row_start(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr) {
+__device__ __forceinline__ int filter_row_start(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr) {
     const int m = clampi(y0 - R + lr, v.first_row, v.last_row);
-    return (int64_t)(a.view_last_row - m) * (int64_t)a.in_stride + (int64_t)(x0 - 4 - v.first_col) * 3;
+    return (int)__umul24((uint32_t)(a.view_last_row - m), (uint32_t)a.in_stride) + (x0 - 4 - v.first_col) * 3;
+}
+
+// a * b for a, b < 2^24: the two full-rate 24-bit multiplies (low and high half of the product).
+__device__ __forceinline__ uint64_t mul24_wide(uint32_t a, uint32_t b) {
+    return (uint64_t)(a & 0xFFFFFFu) * (uint64_t)(b & 0xFFFFFFu);
 }
 
 template <int R>
-__device__ __forceinline__ void filter_dma(const FilterArgs& a, uint32_t tile, uint32_t* raw) {
+__device__ __forceinline__ void filter_dma(const FilterArgs& a, int x0, int y0, uint32_t* raw) {
     constexpr int CHUNKS = FtGeom<R>::LROWS * FT_RAW_CHUNKS;
-    int x0, y0;
-    filter_tile_origin(a, tile, x0, y0);
     const FilterView v = filter_view(a);
     // The view spans [in, in + extent) (rows in inverted order; the host keeps it below 2 GiB).
     // The resource starts at the 16-byte boundary at or below `in` and ends at the one at or
     // above the view's end: every 16-byte chunk holding a view byte is read whole (an aligned
     // chunk never leaves the page of its bytes), chunks past the end read as zero, and a chunk
     // wholly before the view (a window starting left of it) is redirected to offset 0.
-    const int64_t mis = (int64_t)((uintptr_t)a.in & 15u);
-    const int64_t extent = (int64_t)(a.view_last_row - v.first_row) * (int64_t)a.in_stride + (int64_t)a.pad * 3;
+    const int mis = (int)((uintptr_t)a.in & 15u);
+    const int extent = (int)__umul24((uint32_t)(a.view_last_row - v.first_row), (uint32_t)a.in_stride) + a.pad * 3;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a.in - mis), 0, (int)((mis + extent + 15) & ~(int64_t)15), 0x00020000);
+        (void*)(a.in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
@@ -145,8 +150,8 @@ __device__ __forceinline__ void filter_dma(const FilterArgs& a, uint32_t tile, u
         const int c = c0 + lane;
         if (c < CHUNKS) {
             const int lr = c / FT_RAW_CHUNKS, ch = c - lr * FT_RAW_CHUNKS;
-            const int64_t st = filter_row_start<R>(a, v, x0, y0, lr) + mis;   // from the aligned base
-            const int64_t off = (st & ~(int64_t)15) + ch * 16;
+            const int st = filter_row_start<R>(a, v, x0, y0, lr) + mis;   // from the aligned base
+            const int off = (st & ~15) + ch * 16;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(
                 rs, (__attribute__((address_space(3))) void*)((uint8_t*)raw + c0 * 16), 16,
                 (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
@@ -157,12 +162,10 @@ __device__ __forceinline__ void filter_dma(const FilterArgs& a, uint32_t tile, u
 // De-interleave the staged window into three biased channel planes: plane[k][lr][gc] holds
 // channel k of the 4 pixels x0 - 4 + 4 gc .. +3 of staged row lr (LDS dword lr * FT_LWD + gc == g).
 template <int R>
-__device__ __forceinline__ void filter_unpack(const FilterArgs& a, uint32_t tile, const uint32_t* raw, uint32_t* lds) {
+__device__ __forceinline__ void filter_unpack(const FilterArgs& a, int x0, int y0, const uint32_t* raw, uint32_t* lds) {
     using G = FtGeom<R>;
-    int x0, y0;
-    filter_tile_origin(a, tile, x0, y0);
     const FilterView v = filter_view(a);
-    const int64_t mis = (int64_t)((uintptr_t)a.in & 15u);
+    const int mis = (int)((uintptr_t)a.in & 15u);
     const uint8_t* rawb = (const uint8_t*)raw;
 #pragma unroll 2
     for (int i = 0; i < G::ITERS; ++i) {
@@ -170,8 +173,8 @@ __device__ __forceinline__ void filter_unpack(const FilterArgs& a, uint32_t tile
         if (i == G::ITERS - 1 && g >= G::GROUPS) break;
         const int lr = g / FT_LWD;
         const int gc = g - lr * FT_LWD;
-        const int64_t st = filter_row_start<R>(a, v, x0, y0, lr);
-        const int rb = lr * FT_RAW_ROW + (int)((st + mis) & 15);   // raw byte of pixel x0 - 4
+        const int st = filter_row_start<R>(a, v, x0, y0, lr);
+        const int rb = lr * FT_RAW_ROW + ((st + mis) & 15);   // raw byte of pixel x0 - 4
         const int p = x0 - 4 + gc * 4;                            // first image column of the group
         uint32_t p0, p1, p2;
         if (p >= v.first_col && p + 3 <= v.last_col) {
@@ -293,9 +296,7 @@ __device__ __forceinline__ void filter_accumulate(const FilterWeights& w, const 
 
 // Epilogue: apply the mode, swap channels (out c <- in 2-c), store 12 bytes per row.
 template <int MODE>
-__device__ __forceinline__ void filter_epilogue(const FilterArgs& a, uint32_t tile, const FilterAcc& acc) {
-    int x0, y0;
-    filter_tile_origin(a, tile, x0, y0);
+__device__ __forceinline__ void filter_epilogue(const FilterArgs& a, int x0, int y0, const FilterAcc& acc) {
     const int lane = threadIdx.x & 63;
     const int ry0 = (threadIdx.x >> 6) * FT_RPT;
     const int shift = a.shift;
@@ -316,7 +317,7 @@ __device__ __forceinline__ void filter_epilogue(const FilterArgs& a, uint32_t ti
                 ob[3 * q + c] = MODE == 0 ? (uint32_t)v : (uint32_t)clampi(v >> shift, 0, 255);
             }
         }
-        UH_GLOBAL uint8_t* dst = out + (uint64_t)(y - a.row0) * a.out_stride + (uint64_t)(x - a.col0) * 3;
+        UH_GLOBAL uint8_t* dst = out + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride) + (uint32_t)(x - a.col0) * 3u;
         if (x + 4 <= cend && (((uintptr_t)dst) & 3u) == 0) {
             UH_GLOBAL uint32_t* d32 = (UH_GLOBAL uint32_t*)dst;
             d32[0] = pack4(ob[0], ob[1], ob[2], ob[3]);
@@ -382,22 +383,22 @@ __device__ __forceinline__ bool filter_staged(const FilterArgs& a) {
 }
 
 template <int V>
-__device__ __forceinline__ void filter_dma_v(const FilterArgs& a, uint32_t tile, uint32_t* raw) {
+__device__ __forceinline__ void filter_dma_v(const FilterArgs& a, int x0, int y0, uint32_t* raw) {
     if constexpr (V == 0) {
-        if (a.radius == 2) filter_dma<2>(a, tile, raw);
-        else if (a.radius == 1) filter_dma<1>(a, tile, raw);
+        if (a.radius == 2) filter_dma<2>(a, x0, y0, raw);
+        else if (a.radius == 1) filter_dma<1>(a, x0, y0, raw);
     } else {
-        filter_dma<(V + 1) / 2>(a, tile, raw);
+        filter_dma<(V + 1) / 2>(a, x0, y0, raw);
     }
 }
 
 template <int V>
-__device__ __forceinline__ void filter_unpack_v(const FilterArgs& a, uint32_t tile, const uint32_t* raw, uint32_t* lds) {
+__device__ __forceinline__ void filter_unpack_v(const FilterArgs& a, int x0, int y0, const uint32_t* raw, uint32_t* lds) {
     if constexpr (V == 0) {
-        if (a.radius == 2) filter_unpack<2>(a, tile, raw, lds);
-        else if (a.radius == 1) filter_unpack<1>(a, tile, raw, lds);
+        if (a.radius == 2) filter_unpack<2>(a, x0, y0, raw, lds);
+        else if (a.radius == 1) filter_unpack<1>(a, x0, y0, raw, lds);
     } else {
-        filter_unpack<(V + 1) / 2>(a, tile, raw, lds);
+        filter_unpack<(V + 1) / 2>(a, x0, y0, raw, lds);
     }
 }
 
@@ -416,13 +417,13 @@ __device__ __forceinline__ void filter_compute_v(const FilterArgs& a, const Filt
 }
 
 template <int V>
-__device__ __forceinline__ void filter_epilogue_v(const FilterArgs& a, uint32_t tile, const FilterAcc& acc) {
+__device__ __forceinline__ void filter_epilogue_v(const FilterArgs& a, int x0, int y0, const FilterAcc& acc) {
     if constexpr (V == 0) {
         if (!filter_staged<0>(a)) return;
-        if (a.mode == 0) filter_epilogue<0>(a, tile, acc);
-        else filter_epilogue<1>(a, tile, acc);
+        if (a.mode == 0) filter_epilogue<0>(a, x0, y0, acc);
+        else filter_epilogue<1>(a, x0, y0, acc);
     } else {
-        filter_epilogue<(V + 1) % 2>(a, tile, acc);
+        filter_epilogue<(V + 1) % 2>(a, x0, y0, acc);
     }
 }
 
@@ -439,12 +440,25 @@ __device__ __forceinline__ void filter_run(FilterLds& sm, const FilterArgs& a, c
                                            const int8_t* table, uint32_t l0, uint32_t l1) {
     if (l0 >= l1) return;
     const bool staged = filter_staged<V>(a);
-    if (staged) filter_dma_v<V>(a, l0, sm.raw);
+    // tile origins advance down the column strip: one division per segment, none per tile
+    const uint32_t tiles_y = (uint32_t)((a.rows + FT_H - 1) / FT_H);
+    uint32_t tx = l0 / tiles_y, ty = l0 - tx * tiles_y;
+    auto origin = [&](int& x0, int& y0) {   // origin of tile (tx, ty), then step to the next tile
+        x0 = a.col0 + (int)tx * FT_W;
+        y0 = a.row0 + (int)ty * FT_H;
+        if (++ty == tiles_y) { ty = 0; ++tx; }
+    };
+    int x0, y0, x1 = 0, y1 = 0, x2 = 0, y2 = 0;
+    origin(x0, y0);
+    if (staged) filter_dma_v<V>(a, x0, y0, sm.raw);
     wait_vm_all();
     __syncthreads();          // the window is in LDS; every wave has left the planes
-    if (staged) filter_unpack_v<V>(a, l0, sm.raw, sm.planes);
+    if (staged) filter_unpack_v<V>(a, x0, y0, sm.raw, sm.planes);
     __syncthreads();
-    if (staged && l0 + 1 < l1) filter_dma_v<V>(a, l0 + 1, sm.raw);
+    if (l0 + 1 < l1) {
+        origin(x1, y1);
+        if (staged) filter_dma_v<V>(a, x1, y1, sm.raw);
+    }
     for (uint32_t t = l0;; ++t) {
         FilterAcc acc;
         filter_compute_v<V>(a, w, table, t, sm.planes, acc);
@@ -453,12 +467,17 @@ __device__ __forceinline__ void filter_run(FilterLds& sm, const FilterArgs& a, c
             __syncthreads();      // every wave is done with the planes
             wait_vm_all();
             __syncthreads();      // tile t+1's window is in LDS
-            if (staged) filter_unpack_v<V>(a, t + 1, sm.raw, sm.planes);
+            if (staged) filter_unpack_v<V>(a, x1, y1, sm.raw, sm.planes);
             __syncthreads();      // planes hold tile t+1; the raw window is free
-            if (staged && t + 2 < l1) filter_dma_v<V>(a, t + 2, sm.raw);
+            if (t + 2 < l1) {
+                origin(x2, y2);
+                if (staged) filter_dma_v<V>(a, x2, y2, sm.raw);
+            }
         }
-        filter_epilogue_v<V>(a, t, acc);
+        filter_epilogue_v<V>(a, x0, y0, acc);
         if (!more) break;
+        x0 = x1; y0 = y1;
+        x1 = x2; y1 = y2;
     }
 }
 
@@ -581,6 +600,9 @@ static int filter_validate(const uh_imagefilter_args* a, int* view_cols) {
     const int fr = a->row0 - r < 0 ? 0 : a->row0 - r;
     if ((r == 1 || r == 2) && (uint64_t)(lr - fr) * a->in_stride + (uint64_t)(lc - fc + 1) * 3 + 32 >= (1ull << 31))
         return fail(UH_ERR_ARG, "uh_imagefilter: the view spans 2 GiB or more");
+    // ... and computes row offsets with 24-bit multiplies
+    if ((r == 1 || r == 2) && (a->in_stride >= (1u << 24) || a->out_stride >= (1u << 24) || a->height >= (1 << 24)))
+        return fail(UH_ERR_ARG, "uh_imagefilter: row strides of 16 MiB or more");
     // The weighted sum must fit int32 for mode 1 (always true for |w| <= 127, r <= 15).
     return UH_OK;
 }
