@@ -47,6 +47,13 @@ typedef struct pmHipStubStats
     uint64_t d2d_bytes;         /* device gather/scatter bytes (COMPACT views of mirrors) */
     uint64_t cache_hits;        /* PLAIN: read-only device buffers reused */
     uint64_t evictions;         /* HBM mirrors evicted to honour the budget */
+    /* host-side phase times in nanoseconds, summed */
+    uint64_t ns_queue;          /* submit -> first subtask picked by a stub, per task */
+    uint64_t ns_stage;          /* BATCHED: distribution + GPU callbacks of a batch's subtasks */
+    uint64_t ns_flush;          /* BATCHED: flush host time (queue build, upload, launch) */
+    uint64_t ns_wait;           /* BATCHED: stub thread blocked on the oldest batch */
+    uint64_t ns_retire;         /* BATCHED: retiring completed batches (scatter, completion) */
+    uint64_t ns_wake;           /* task completion -> blocked waiter resumed, per wait */
 } pmHipStubStats;
 
 #ifdef __cplusplus
@@ -56,7 +63,8 @@ int pmGetHipStubMode();
 /* Force the view used for SUBSCRIPTION_OPTIMAL on the HIP stub in RESIDENT/BATCHED modes:
  * SUBSCRIPTION_NATURAL (default, zero-copy) or SUBSCRIPTION_COMPACT (device gather/scatter). */
 pmStatus pmSetHipOptimalView(pmSubscriptionVisibilityType pType);
-/* Use hipGraph replay for batch flushes (default on). */
+/* Use hipGraph replay for batch flushes that launch more than one kernel (default on); a
+ * one-kernel flush launches directly either way. */
 pmStatus pmSetHipGraphs(bool pEnable);
 /* Maximum subtasks in flight on the HIP stub (PLAIN/RESIDENT) and per batch (BATCHED). */
 pmStatus pmSetHipPipelineDepth(int pDepth);

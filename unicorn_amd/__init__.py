@@ -58,7 +58,8 @@ def apps_lib() -> C.CDLL:
 class ua_stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "subtasks", "batches", "h2d_bytes", "d2h_bytes", "d2d_bytes", "cache_hits", "evictions",
-        "plain_launches", "persistent_launches", "queue_items")]
+        "plain_launches", "persistent_launches", "queue_items",
+        "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake")]
 
 
 class ua_if_config(C.Structure):

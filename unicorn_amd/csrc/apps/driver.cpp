@@ -53,6 +53,12 @@ int ua_get_stats(ua_stats* s) {
     s->d2d_bytes = h.d2d_bytes;
     s->cache_hits = h.cache_hits;
     s->evictions = h.evictions;
+    s->ns_queue = h.ns_queue;
+    s->ns_stage = h.ns_stage;
+    s->ns_flush = h.ns_flush;
+    s->ns_wait = h.ns_wait;
+    s->ns_retire = h.ns_retire;
+    s->ns_wake = h.ns_wake;
     s->plain_launches = s->persistent_launches = s->queue_items = 0;
     if (pmHasHipDevice()) uh_launch_counters(&s->plain_launches, &s->persistent_launches, &s->queue_items);
     return 0;

@@ -33,7 +33,28 @@ double now_secs() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Bounded spin before a thread sleeps on a condition variable: a client that submits its next
+// task as soon as the previous one completes, and a waiter on a short task, are served without
+// a futex wake-up (tens of microseconds). PMLIB_SPIN_US sets the bound (0 disables spinning).
+static double spin_secs() {
+    static const double s = [] {
+        const char* e = getenv("PMLIB_SPIN_US");
+        return (e ? atof(e) : 1000.0) * 1e-6;
+    }();
+    return s;
+}
+
+static inline void cpu_relax() {
+#if defined(__x86_64__) || defined(__i386__)
+    __builtin_ia32_pause();
+#endif
+}
+
 [[noreturn]] void raise(pmStatus s, const std::string& what) { throw Error{s, what}; }
+
+namespace {
+void stat_add(uint64_t pmHipStubStats::*f, uint64_t v);
+}
 
 pmTaskInfo Task::info() {
     pmTaskInfo ti;
@@ -162,6 +183,7 @@ void Runtime::submit(Task* t) {
         t->seq = ++seq;
         if (t->subtask_count == 0) {
             t->state = TaskState::Done;
+            t->finished.store(true, std::memory_order_release);
             t->start_time = t->end_time = t->submit_time;
             return;
         }
@@ -171,12 +193,19 @@ void Runtime::submit(Task* t) {
         });
         active.insert(it, t);
     }
+    submitted.fetch_add(1, std::memory_order_release);
     q_cv.notify_all();
 }
 
 void Runtime::wait(Task* t) {
+    if (!t->finished.load(std::memory_order_acquire)) {
+        const double until = now_secs() + spin_secs();
+        while (!t->finished.load(std::memory_order_acquire) && now_secs() < until) cpu_relax();
+    }
     std::unique_lock<std::mutex> g(t->mtx);
+    if (t->state == TaskState::Done) return;
     t->cv.wait(g, [&] { return t->state == TaskState::Done; });
+    stat_add(&pmHipStubStats::ns_wake, (uint64_t)((now_secs() - t->end_time) * 1e9));
 }
 
 Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
@@ -211,6 +240,7 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
         if (t->state == TaskState::Queued) {
             t->state = TaskState::Running;
             t->start_time = now_secs();
+            stat_add(&pmHipStubStats::ns_queue, (uint64_t)((t->start_time - t->submit_time) * 1e9));
         }
         return t;
     }
@@ -332,6 +362,7 @@ void Runtime::complete(Task* t) {
     std::lock_guard<std::mutex> g(t->mtx);
     t->end_time = now_secs();
     t->state = TaskState::Done;
+    t->finished.store(true, std::memory_order_release);
     t->cv.notify_all();
 }
 
@@ -846,8 +877,9 @@ void Runtime::gpu_loop() {
     auto ensure_task_ready = [&](Task* t) {
         if (t->gpu_prepared) return;
         if (t->gpu_resident) {
-            for (AddressSpace* as : t->spaces) Memory::get().ensure_device(as, copy_stream);
-            hipchk(uh_stream_sync(copy_stream), "mirror upload");
+            bool copied = false;
+            for (AddressSpace* as : t->spaces) copied |= Memory::get().ensure_device(as, copy_stream);
+            if (copied) hipchk(uh_stream_sync(copy_stream), "mirror upload");
             for (AddressSpace* as : t->spaces) {
                 std::lock_guard<std::mutex> g(as->mtx);
                 as->gpu_pins++;
@@ -900,8 +932,12 @@ void Runtime::gpu_loop() {
     auto retire_batches = [&](bool block) {
         while (!batches.empty()) {
             BatchRun& b = batches.front();
-            if (block) uh_event_sync(b.event);
-            else if (uh_event_query(b.event) != 0) return;
+            if (block) {
+                const double w0 = now_secs();
+                uh_event_sync(b.event);
+                stat_add(&pmHipStubStats::ns_wait, (uint64_t)((now_secs() - w0) * 1e9));
+            } else if (uh_event_query(b.event) != 0) return;
+            const double r0 = now_secs();
             for (auto& sp : b.slots) {
                 GpuSlot& s = *sp;
                 Task* t = s.task;
@@ -914,6 +950,7 @@ void Runtime::gpu_loop() {
                 retire_slot(this, s, false);
             }
             batches.pop_front();
+            stat_add(&pmHipStubStats::ns_retire, (uint64_t)((now_secs() - r0) * 1e9));
         }
     };
 
@@ -953,6 +990,13 @@ void Runtime::gpu_loop() {
                 return nullptr;
             };
             if (inflight == 0 && batches.empty()) {
+                if (!stopping && !pending() && spin_secs() > 0) {
+                    const uint64_t seen = submitted.load(std::memory_order_acquire);
+                    g.unlock();
+                    const double until = now_secs() + spin_secs();
+                    while (submitted.load(std::memory_order_acquire) == seen && now_secs() < until) cpu_relax();
+                    g.lock();
+                }
                 q_cv.wait(g, [&] { return stopping || pending(); });
                 if (stopping && !pending()) break;
                 if (!front()) continue;   // admission runs first
@@ -996,6 +1040,7 @@ void Runtime::gpu_loop() {
                     taken += c2;
                 }
             }
+            const double s0 = now_secs();
             for (auto& cl : claims) {
                 Task* tt = cl.first;
                 try { ensure_task_ready(tt); } catch (Error& e) { fail_task(tt, e); }
@@ -1022,11 +1067,14 @@ void Runtime::gpu_loop() {
                 }
                 // the stub held one pick per claim; slots release one each
             }
+            const double f0 = now_secs();
             try {
                 hipchk(uh_batch_flush(batch, use_graphs.load()), "batch flush");
             } catch (Error& e) {
                 for (auto& cl : claims) fail_task(cl.first, e);
             }
+            stat_add(&pmHipStubStats::ns_stage, (uint64_t)((f0 - s0) * 1e9));
+            stat_add(&pmHipStubStats::ns_flush, (uint64_t)((now_secs() - f0) * 1e9));
             run.event = batch_events[batch_ev];
             batch_ev = (batch_ev + 1) % 4;
             hipchk(uh_event_record(run.event, batch_stream), "batch event");

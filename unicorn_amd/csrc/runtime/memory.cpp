@@ -167,9 +167,10 @@ void Memory::sync_host(AddressSpace* as) {
     as->host_valid = true;
 }
 
-void Memory::ensure_device(AddressSpace* as, void* stream) {
+bool Memory::ensure_device(AddressSpace* as, void* stream) {
     std::lock_guard<std::mutex> g(as->mtx);
     as->last_use = ++clock_;
+    bool copied = false;
     if (!as->dev) {
         as->dev = (uint8_t*)device_alloc(as->length ? as->length : 1);
         as->dev_valid = false;
@@ -178,11 +179,13 @@ void Memory::ensure_device(AddressSpace* as, void* stream) {
         if (as->host_valid && as->length) {
             if (uh_memcpy_h2d(as->dev, as->host, as->length, stream))
                 raise(pmGraphicsCardError, std::string("upload of address space: ") + uh_last_error());
+            copied = true;
             Runtime* r = Runtime::get();
             if (r) { std::lock_guard<std::mutex> s(r->stats_mtx); r->stats.h2d_bytes += as->length; }
         }
         as->dev_valid = true;
     }
+    return copied;
 }
 
 void Memory::mark_device_written(AddressSpace* as) {

@@ -107,7 +107,7 @@ public:
     // Make the host copy current (download from HBM if the mirror is newer).
     void sync_host(AddressSpace* as);
     // Make the HBM mirror current (allocate within the budget, upload if the host is newer).
-    void ensure_device(AddressSpace* as, void* stream);
+    bool ensure_device(AddressSpace* as, void* stream);   // true when it enqueued an upload
     void mark_device_written(AddressSpace* as);
     void mark_host_written(AddressSpace* as);
     // HBM budget accounting for mirrors and subtask buffers.
@@ -184,6 +184,7 @@ struct Task {
     std::mutex mtx;
     std::condition_variable cv;
     TaskState state = TaskState::Queued;
+    std::atomic<bool> finished{false};  // state == Done, readable without the mutex
     uint64_t seq = 0;
     // reduction: completed subtasks awaiting a partner
     std::unique_ptr<Subtask> pending_reduce;
@@ -225,6 +226,7 @@ public:
     std::vector<Task*> active;           // queued or running, ordered by (priority, seq)
     uint64_t seq = 0;
     bool stopping = false;
+    std::atomic<uint64_t> submitted{0};   // tasks submitted so far (the idle GPU thread spins on it)
     // stubs
     std::vector<std::thread> cpu_threads;
     std::thread gpu_thread;
