@@ -39,7 +39,7 @@ double now_secs() {
 static double spin_secs() {
     static const double s = [] {
         const char* e = getenv("PMLIB_SPIN_US");
-        return (e ? atof(e) : 1000.0) * 1e-6;
+        return (e ? atof(e) : 300.0) * 1e-6;
     }();
     return s;
 }
