@@ -191,13 +191,16 @@ def bench_matmul(args, U):
 def bench_pagerank(args, U):
     """pageRank on a 10M-page synthetic power-law web graph (out-degree P(k) ~ (k + k0)^-2 on
     [1, 100], mean 10, 5 % dangling, preferential targets), 20 iterations, 1M-page subtasks.
-    One step = one pageRank computation (init task + 20 iteration tasks). Roofline: per
-    iteration 12 N + 4 E bytes (row pointers, degree, rank in / out, in-link ids) over the
-    flush's device time."""
+    One step = one pageRank computation (init task + 20 iteration tasks) over the source-blocked
+    graph layout. Roofline: per iteration 12 N + 4 E bytes (row pointers, degree, rank in / out,
+    in-link ids) over the flush's device time (the blocked gather and the combine kernel). The
+    plain in-link CSR layout of the same graph is timed beside it."""
     pages, per, iters = 10_000_000, 1_000_000, 20
     rp, col = U.webgen(pages, 20240601)
     edges = int(col.size)
     pr = U.PageRank(rp, col, per)
+    if not pr.blocked:
+        raise RuntimeError("bench: the 10M-page graph did not take the source-blocked layout")
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
     for _ in range(args.warmup):
         pr.run(iters, U.DEVICE_GPU)
@@ -216,6 +219,20 @@ def bench_pagerank(args, U):
     comps["plain"] = {"tasks_per_s": round(1 / e, 4), "p50_ms": round(1e3 * l[0], 1)}
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED)
     pr.close()
+    # the same graph in the plain in-link CSR layout (CSR-stream pull, no source blocking)
+    U.pr_set_layout(False)
+    prc = U.PageRank(rp, col, per)
+    U.pr_set_layout(True)
+    prc.run(iters, U.DEVICE_GPU)
+    U.device_sync()
+    U.set_kernel_timing(True)
+    U.kernel_time(reset=True)
+    e, l = time_tasks(lambda: prc.run(iters, U.DEVICE_GPU), 1)
+    U.device_sync()
+    cms, cfl = U.kernel_time(reset=True)
+    U.set_kernel_timing(False)
+    comps["csr_layout"] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms / max(1, cfl), 4)}
+    prc.close()
     it_bytes = 12 * pages + 4 * edges
     achieved = it_bytes / (kernel_ms * 1e-3) / 1e9
     cpu, err = None, None
@@ -237,7 +254,8 @@ def bench_pagerank(args, U):
         "per_subtask_launch": comps, "rel_l1_vs_oracle": err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_pagerank_persistent", "kernel_ms": round(kernel_ms, 4),
+                     "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent",
+                     "kernel_ms": round(kernel_ms, 4),
                      "algorithmic_bytes_per_launch": it_bytes},
         "cpu_baseline": cpu,
     }

@@ -93,6 +93,18 @@ uint64_t ua_pr_edges(void* h);
 void* ua_pr_clone(void* h);                  /* same graph, own rank state */
 int ua_pr_fetch(void* h, uint64_t* checksum);
 int ua_pr_destroy(void* h);
+/* Graph address space (see uh_pagerank_args for the source-blocked layout). ua_pr_block_counts
+ * fills bcnt (ceil(pages / UH_PR_GROUP_ROWS) * UH_PR_BLOCKS * UH_PR_GROUP_ROWS bytes) and returns
+ * 1, or 0 when a page has more than 255 in-links from one source block. ua_pr_build_graph writes
+ * [in_row_ptr | outdeg | in_col (| bseg | bcnt)] into ua_pr_graph_bytes(pages, edges, bcnt != 0)
+ * bytes; in_col is source-blocked when bcnt is given. ua_pr_set_layout(0) makes later
+ * ua_pr_create calls keep the plain in-link CSR; ua_pr_blocked reports an instance's layout. */
+int ua_pr_block_counts(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col, uint8_t* bcnt);
+uint64_t ua_pr_graph_bytes(uint32_t pages, uint64_t edges, int blocked);
+int ua_pr_build_graph(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                      const uint8_t* bcnt, uint32_t* graph);
+void ua_pr_set_layout(int blocked);
+int ua_pr_blocked(void* h);
 
 /* ---- luDecomposition (reference testSuite/luDecomposition) + refinement solve ----
  * n x n float32 column-major, blocks of `block` (<= 512); precision 0 = split-BF16 trailing

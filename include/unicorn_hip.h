@@ -117,12 +117,25 @@ int uh_imagefilter(const uh_imagefilter_args* args, void* stream);
  *   y[t] = sum_{e in [row_ptr[t], row_ptr[t+1])} w[col[e]]
  *   w_next[t] = (deg[t] > 0) ? (damping * y[t]) / deg[t] : 0
  * where w[src] = (damping * x[src]) / deg[src] was produced by the previous iteration
- * (the reference's increment, pageRank.cpp:262,367). y and w_next are indexed by global page id. */
+ * (the reference's increment, pageRank.cpp:262,367). y and w_next are indexed by global page id.
+ *
+ * Source-blocked layout (bcnt != NULL). Pages form row groups of UH_PR_GROUP_ROWS destinations
+ * and UH_PR_BLOCKS equal source blocks (block size rounded up to 32 pages). Within a row group
+ * the in-links are ordered by (source block, destination, source):
+ *   bseg[g * UH_PR_BLOCKS + x]  first link of group g, block x (bseg[groups * UH_PR_BLOCKS] = E);
+ *   bcnt[(g * UH_PR_BLOCKS + x) * UH_PR_GROUP_ROWS + r]  links of row g*GROUP_ROWS + r in block x
+ *   (u8; rows past the last page count 0; bcnt 4-byte aligned).
+ * col is indexed as before (global link id) and row_ptr is not read. Each row's sum is taken per
+ * block in link order and the blocks are added in block order, so results are deterministic.
+ * With bcnt == NULL col is a plain in-link CSR (sources ascending within each row). */
+#define UH_PR_GROUP_ROWS 2048
+#define UH_PR_BLOCKS 8
 typedef struct uh_pagerank_args {
     const uint32_t* row_ptr; const uint32_t* col; const uint32_t* deg;
     const float* w; float* y; float* w_next;
     uint32_t row0, rows;
     float damping; uint32_t pad;
+    const uint8_t* bcnt; const uint32_t* bseg;   /* source-blocked layout, or NULL */
 } uh_pagerank_args;
 int uh_pagerank_pull(const uh_pagerank_args* args, void* stream);
 /* w[i] = (damping * x0) / deg[i] (or 0 for dangling pages) for the first iteration. */

@@ -4,6 +4,9 @@ Tolerance: relative L1 <= 1e-5 against the oracle's restatement of the reference
 subtask + REDUCE_ADD in subtask order); the error against the FP64 oracle is asserted too.
 Closed forms pin the recurrence: a ring gives 0.85 everywhere, a star gives the hub
 0.85 * leaves after the first iteration, and a graph of dangling pages gives zero.
+Both graph layouts run: the source-blocked one (the default) and the plain in-link CSR. In the
+blocked layout the GPU kernels and the CPU callback add in the same order, so their ranks are
+bitwise equal.
 """
 import numpy as np
 import pytest
@@ -17,13 +20,22 @@ def _rel_l1(a, b):
     return float(np.abs(a.astype(np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
+@pytest.fixture
+def layout(gpu_runtime, request):
+    gpu_runtime.pr_set_layout(request.param)
+    yield request.param
+    gpu_runtime.pr_set_layout(True)
+
+
+@pytest.mark.parametrize("layout", [True, False], indirect=True, ids=["blocked", "csr"])
 @pytest.mark.parametrize("hip_mode", [0, 1, 2])
-def test_pagerank_matches_oracle(gpu_runtime, hip_mode):
+def test_pagerank_matches_oracle(gpu_runtime, hip_mode, layout):
     U = gpu_runtime
     U.set_hip_mode(hip_mode)
     rp, col = U.webgen(200_000, 11)
     pr = U.PageRank(rp, col, 50_000)
     try:
+        assert pr.blocked == layout
         pr.run(20, U.DEVICE_GPU)
         y = pr.ranks()
     finally:
@@ -32,12 +44,14 @@ def test_pagerank_matches_oracle(gpu_runtime, hip_mode):
     assert _rel_l1(y, O.pagerank_f64(rp, col, 20)) <= 1e-5
 
 
-def test_pagerank_compact_views(gpu_runtime):
+@pytest.mark.parametrize("layout", [True, False], indirect=True, ids=["blocked", "csr"])
+def test_pagerank_compact_views(gpu_runtime, layout):
     U = gpu_runtime
     U.set_hip_mode(1, compact=True)
     rp, col = U.webgen(30_000, 5)
-    pr = U.PageRank(rp, col, 7_000)      # ragged last subtask
+    pr = U.PageRank(rp, col, 7_000)      # ragged last subtask; subtasks share row groups
     try:
+        assert pr.blocked == layout
         pr.run(4, U.DEVICE_GPU)
         y = pr.ranks()
     finally:
@@ -93,3 +107,40 @@ def test_pagerank_cpu_gpu_mix(gpu_runtime):
     finally:
         pr.close()
     assert _rel_l1(y, O.pagerank(rp, col, 5, 5_000)) <= 1e-5
+
+
+@pytest.mark.parametrize("hip_mode", [0, 2])
+def test_pagerank_blocked_cpu_gpu_bitwise(gpu_runtime, hip_mode):
+    U = gpu_runtime
+    U.set_hip_mode(hip_mode)
+    rp, col = U.webgen(100_000, 4)
+    ys = []
+    for device in (U.DEVICE_CPU, U.DEVICE_GPU):
+        pr = U.PageRank(rp, col, 30_000)
+        try:
+            assert pr.blocked
+            pr.run(3, device)
+            ys.append(pr.ranks())
+        finally:
+            pr.close()
+    assert np.array_equal(ys[0], ys[1])
+    assert _rel_l1(ys[1], O.pagerank(rp, col, 3, 30_000)) <= 1e-5
+
+
+def test_pagerank_csr_fallback_for_heavy_rows(gpu_runtime):
+    U = gpu_runtime
+    U.set_hip_mode(2)
+    n = 20_000
+    rp0, col0 = U.webgen(n, 9)
+    src = np.repeat(np.arange(n), np.diff(rp0.astype(np.int64)))
+    edges = set(zip(src.tolist(), col0.tolist()))
+    edges |= {(s, 12_345) for s in range(400)}   # 400 in-links from source block 0: > 255
+    rp, col = _csr(n, sorted(edges))
+    pr = U.PageRank(rp, col, 3000)
+    try:
+        assert not pr.blocked
+        pr.run(2, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 2, 3000)) <= 1e-5

@@ -99,6 +99,10 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_pr_run": (i32, [vp, i32, i32, C.POINTER(dbl)]),
         "ua_pr_get_ranks": (i32, [vp, vp]), "ua_pr_edges": (C.c_uint64, [vp]),
         "ua_pr_destroy": (i32, [vp]),
+        "ua_pr_block_counts": (i32, [C.c_uint32, vp, vp, vp]),
+        "ua_pr_graph_bytes": (C.c_uint64, [C.c_uint32, C.c_uint64, i32]),
+        "ua_pr_build_graph": (i32, [C.c_uint32, vp, vp, vp, vp]),
+        "ua_pr_set_layout": (None, [i32]), "ua_pr_blocked": (i32, [vp]),
         "ua_lu_create": (vp, [i32, i32, i32]), "ua_lu_set_matrix": (i32, [vp, vp]),
         "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
         "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
@@ -256,6 +260,11 @@ class PageRank:
         if not self._h:
             _check(-1, "ua_pr_create")
 
+    @property
+    def blocked(self) -> bool:
+        """True if the graph is held in the source-blocked layout."""
+        return bool(apps_lib().ua_pr_blocked(self._h))
+
     def run(self, iterations: int, device: int = DEVICE_GPU) -> float:
         t = C.c_double(0.0)
         _check(apps_lib().ua_pr_run(self._h, iterations, device, C.byref(t)), "pageRank tasks")
@@ -276,6 +285,37 @@ class PageRank:
             self.close()
         except Exception:
             pass
+
+
+def pr_set_layout(blocked: bool) -> None:
+    """Layout of the pageRank graphs created from now on (source-blocked where possible, or the
+    plain in-link CSR)."""
+    apps_lib().ua_pr_set_layout(1 if blocked else 0)
+
+
+def pr_graph_layout(row_ptr: np.ndarray, col: np.ndarray, blocked: bool = True):
+    """The graph address space that PageRank builds from an out-link CSR, as numpy views:
+    (in_row_ptr, outdeg, in_col, bseg, bcnt); bseg and bcnt are None for the plain CSR (or when
+    a page has more than 255 in-links from one source block)."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
+    n = row_ptr.size - 1
+    e = int(row_ptr[-1])
+    groups = (n + 2047) // 2048
+    L = apps_lib()
+    bcnt = np.zeros(groups * 8 * 2048, dtype=np.uint8)
+    if blocked and not L.ua_pr_block_counts(n, _ptr(row_ptr), _ptr(col), _ptr(bcnt)):
+        blocked = False
+    nbytes = int(L.ua_pr_graph_bytes(n, e, 1 if blocked else 0))
+    g = np.zeros((nbytes + 3) // 4, dtype=np.uint32)
+    _check(L.ua_pr_build_graph(n, _ptr(row_ptr), _ptr(col), _ptr(bcnt) if blocked else None, _ptr(g)),
+           "ua_pr_build_graph")
+    rp, deg, icol = g[:n + 1], g[n + 1:2 * n + 1], g[2 * n + 1:2 * n + 1 + e]
+    if not blocked:
+        return rp, deg, icol, None, None
+    bseg = g[2 * n + 1 + e:2 * n + 1 + e + groups * 8 + 1]
+    off = (2 * n + 1 + e + groups * 8 + 1) * 4
+    return rp, deg, icol, bseg, g.view(np.uint8)[off:off + groups * 8 * 2048]
 
 
 class LU:

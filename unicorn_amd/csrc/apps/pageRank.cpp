@@ -12,15 +12,23 @@
 // ranges, and each computes its rows by pulling from the sources' increments: no atomics, no
 // partial vectors, no reduction, and the graph stays resident in HBM across iterations.
 // Address spaces of one iteration task:
-//   0 graph  (READ_ONLY):  [in_row_ptr (N+1) | outdeg (N) | in_col (E)]   uint32
+//   0 graph  (READ_ONLY):  [in_row_ptr (N+1) | outdeg (N) | in_col (E) | bseg | bcnt]
+//            uint32 except bcnt (uint8). When every page has at most 255 in-links from each of
+//            the UH_PR_BLOCKS source blocks, in_col is kept in the source-blocked order of
+//            uh_pagerank_args (per row group of UH_PR_GROUP_ROWS pages: by source block, then
+//            destination, then source) with its segment starts bseg and per-row counts bcnt;
+//            otherwise in_col is the plain in-link CSR and bseg / bcnt are absent.
 //   1 state  (READ_ONLY):  [x (N) | w (N)] float, w[p] = (0.85 * x[p]) / outdeg(p)
 //   2 state' (WRITE_ONLY): [y (N) | w' (N)] float, the next state
 // Subtask s subscribes to rows [s0, s1) of the graph (row pointers, degrees and their in-link
-// range), to all of w, and writes y[s0:s1) and w'[s0:s1). An "init" task writes the iteration-0
-// state (x = 1, w = 0.85 / outdeg).
+// range; in the blocked layout the in-links, bseg and bcnt of the row groups that overlap
+// [s0, s1)), to all of w, and writes y[s0:s1) and w'[s0:s1). An "init" task writes the
+// iteration-0 state (x = 1, w = 0.85 / outdeg).
 #include "app_common.h"
 #include "unicorn_hip.h"
+#include "unicorn_apps.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace pageRank {
@@ -37,11 +45,35 @@ struct pageRankTaskConf {
     float initialPageRank;
     int kind;                        // 0 = init, 1 = iterate
     const uint32_t* hostRowPtr;      // in-link row pointers on the host (subscription sizing)
+    int blocked;                     // graph in the source-blocked layout
+    const uint32_t* hostBseg;        // its segment starts on the host (subscription sizing)
 };
+
+static constexpr uint32_t GR = UH_PR_GROUP_ROWS;
+static constexpr uint32_t NB = UH_PR_BLOCKS;
+
+static inline uint32_t groups_of(uint64_t pages) { return (uint32_t)((pages + GR - 1) / GR); }
+// Pages per source block: ceil(N / NB) rounded up to 32 pages (a 128-byte line of w).
+static inline uint32_t block_pages(uint64_t pages) { return (uint32_t)(((pages + NB - 1) / NB + 31) / 32 * 32); }
+static inline size_t bcnt_index(uint32_t t, uint32_t x) { return ((size_t)(t / GR) * NB + x) * GR + t % GR; }
 
 static inline size_t row_ptr_off(uint32_t t) { return (size_t)t * 4; }
 static inline size_t deg_off(const pageRankTaskConf* c, uint32_t t) { return ((size_t)c->totalWebPages + 1 + t) * 4; }
 static inline size_t col_off(const pageRankTaskConf* c, uint64_t e) { return ((size_t)2 * c->totalWebPages + 1 + e) * 4; }
+static inline size_t bseg_off(const pageRankTaskConf* c, size_t i) {
+    return ((size_t)2 * c->totalWebPages + 1 + c->totalLinks + i) * 4;
+}
+static inline size_t bcnt_off(const pageRankTaskConf* c, size_t j) {
+    return bseg_off(c, (size_t)groups_of(c->totalWebPages) * NB + 1) + j;
+}
+// Row groups overlapping [s0, s1) and their link range in the blocked layout.
+static inline void group_range(const pageRankTaskConf* c, uint32_t s0, uint32_t s1, uint32_t* g0, uint32_t* ng,
+                               uint64_t* e0, uint64_t* e1) {
+    *g0 = s0 / GR;
+    *ng = (s1 - 1) / GR - *g0 + 1;
+    *e0 = c->hostBseg[(size_t)*g0 * NB];
+    *e1 = c->hostBseg[(size_t)(*g0 + *ng) * NB];
+}
 
 static void range_of(const pageRankTaskConf* c, ulong id, uint32_t* s0, uint32_t* s1) {
     *s0 = (uint32_t)(id * c->webPagesPerSubtask);
@@ -59,11 +91,24 @@ static pmStatus prDataDistribution(pmTaskInfo ti, pmDeviceInfo di, pmSubtaskInfo
                         pmSubscriptionInfo(row_ptr_off(s0), (size_t)(s1 - s0 + 1) * 4));
     pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX, READ_SUBSCRIPTION,
                         pmSubscriptionInfo(deg_off(c, s0), (size_t)(s1 - s0) * 4));
-    if (c->kind == 1) {
+    if (c->kind == 1 && c->blocked) {
+        uint32_t g0, ng;
+        uint64_t e0, e1;
+        group_range(c, s0, s1, &g0, &ng, &e0, &e1);
+        if (e1 > e0)
+            pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX,
+                                READ_SUBSCRIPTION, pmSubscriptionInfo(col_off(c, e0), (size_t)(e1 - e0) * 4));
+        pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX, READ_SUBSCRIPTION,
+                            pmSubscriptionInfo(bseg_off(c, (size_t)g0 * NB), ((size_t)ng * NB + 1) * 4));
+        pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX, READ_SUBSCRIPTION,
+                            pmSubscriptionInfo(bcnt_off(c, (size_t)g0 * NB * GR), (size_t)ng * NB * GR));
+    } else if (c->kind == 1) {
         const uint64_t e0 = c->hostRowPtr[s0], e1 = c->hostRowPtr[s1];
         if (e1 > e0)
             pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX,
                                 READ_SUBSCRIPTION, pmSubscriptionInfo(col_off(c, e0), (size_t)(e1 - e0) * 4));
+    }
+    if (c->kind == 1) {
         pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_IN_MEM_INDEX,
                             READ_SUBSCRIPTION, pmSubscriptionInfo(N * 4, N * 4));
     }
@@ -78,6 +123,8 @@ struct Views {
     const uint32_t* row_ptr;
     const uint32_t* deg;
     const uint32_t* col;
+    const uint32_t* bseg;     // blocked layout: indexed by g * NB + x
+    const uint8_t* bcnt;      // blocked layout: indexed by (g * NB + x) * GR + r
     const float* w;
     float* y;
     float* w_next;
@@ -89,20 +136,35 @@ static Views views_of(const pageRankTaskConf* c, const pmSubtaskInfo& si, uint32
     const pmMemInfo& in = si.memInfo[STATE_IN_MEM_INDEX];
     const pmMemInfo& out = si.memInfo[STATE_OUT_MEM_INDEX];
     const size_t N = c->totalWebPages;
-    const uint64_t e0 = c->hostRowPtr[s0], e1 = c->hostRowPtr[s1];
+    const bool blk = c->kind == 1 && c->blocked;
+    uint32_t g0 = 0, ng = 0;
+    uint64_t e0 = c->hostRowPtr[s0], e1 = c->hostRowPtr[s1];
+    if (blk) group_range(c, s0, s1, &g0, &ng, &e0, &e1);
+    const size_t b0 = (size_t)g0 * NB;          // first bseg entry of the subtask's groups
     const uint8_t* gb = (const uint8_t*)g.ptr;
+    v.bseg = nullptr;
+    v.bcnt = nullptr;
     if (g.visibilityType == SUBSCRIPTION_NATURAL) {
         // ptr is the start of the first subscribed record (row_ptr[s0]); distances are preserved
+        const size_t o = row_ptr_off(s0);
         v.row_ptr = (const uint32_t*)gb - s0;
-        v.deg = (const uint32_t*)(gb + (deg_off(c, s0) - row_ptr_off(s0))) - s0;
-        v.col = (const uint32_t*)(gb + (col_off(c, e0) - row_ptr_off(s0))) - e0;
+        v.deg = (const uint32_t*)(gb + (deg_off(c, s0) - o)) - s0;
+        v.col = (const uint32_t*)(gb + (col_off(c, e0) - o)) - e0;
+        if (blk) {
+            v.bseg = (const uint32_t*)(gb + (bseg_off(c, b0) - o)) - b0;
+            v.bcnt = gb + (bcnt_off(c, b0 * GR) - o) - b0 * GR;
+        }
     } else {
-        // packed records in address order: row pointers, degrees, in-links
+        // packed records in address order: row pointers, degrees, in-links (, bseg, bcnt)
         const size_t rp = (size_t)(s1 - s0 + 1) * 4, dg = (size_t)(s1 - s0) * 4;
         v.row_ptr = (const uint32_t*)gb - s0;
         v.deg = (const uint32_t*)(gb + rp) - s0;
         v.col = (const uint32_t*)(gb + rp + dg) - e0;
-        (void)e1;
+        if (blk) {
+            const size_t cb = (size_t)(e1 - e0) * 4, sb = ((size_t)ng * NB + 1) * 4;
+            v.bseg = (const uint32_t*)(gb + rp + dg + cb) - b0;
+            v.bcnt = gb + rp + dg + cb + sb - b0 * GR;
+        }
     }
     v.w = nullptr;
     if (c->kind == 1) {
@@ -125,6 +187,29 @@ static pmStatus pr_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
     uint32_t s0, s1;
     range_of(c, si.subtaskId, &s0, &s1);
     Views v = views_of(c, si, s0, s1);
+    if (c->kind == 1 && c->blocked) {
+        // the GPU kernels' order: per row, each block's links in link order, blocks in order
+        std::vector<float> y(s1 - s0, 0.f);
+        for (uint32_t g = s0 / GR; g <= (s1 - 1) / GR; ++g) {
+            for (uint32_t x = 0; x < NB; ++x) {
+                const uint8_t* cnt = v.bcnt + ((size_t)g * NB + x) * GR;
+                uint32_t k = v.bseg[(size_t)g * NB + x];
+                for (uint32_t r = 0; r < GR; ++r) {
+                    const uint32_t t = g * GR + r;
+                    float p = 0.f;
+                    for (uint32_t q = 0; q < cnt[r]; ++q) p += v.w[v.col[k + q]];
+                    k += cnt[r];
+                    if (t >= s0 && t < s1) y[t - s0] += p;
+                }
+            }
+        }
+        for (uint32_t t = s0; t < s1; ++t) {
+            v.y[t] = y[t - s0];
+            const uint32_t d = v.deg[t];
+            v.w_next[t] = d ? (c->dampingFactor * y[t - s0]) / (float)d : 0.f;
+        }
+        return pmSuccess;
+    }
     for (uint32_t t = s0; t < s1; ++t) {
         float y;
         if (c->kind == 0) {
@@ -162,6 +247,8 @@ static pmStatus pr_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* stre
         a.rows = s1 - s0;
         a.damping = c->dampingFactor;
         a.pad = 0;
+        a.bcnt = v.bcnt;
+        a.bseg = v.bseg;
         rc = uh_pagerank_pull(&a, stream);
     }
     if (rc) {
@@ -176,6 +263,7 @@ struct Instance {
     uint64_t edges = 0;
     float damping = 0.85f;
     std::shared_ptr<std::vector<uint32_t>> in_row_ptr;   // host copy for subscription sizing
+    std::shared_ptr<std::vector<uint32_t>> bseg;         // host copy (blocked layout), or null
     std::shared_ptr<apps::SharedMem> inputs;             // the graph, shared with clones
     pmMemHandle graph = nullptr, state[2] = {nullptr, nullptr};
     int current = 0;                    // state holding the latest ranks
@@ -183,6 +271,7 @@ struct Instance {
 
 static apps::CallbackVariants g_cb;
 static std::mutex g_mtx;
+static int g_use_blocked = 1;       // layout of graphs created from now on
 
 static int run_kind(Instance* in, int kind, int device, double* secs) {
     pageRankTaskConf conf;
@@ -193,6 +282,8 @@ static int run_kind(Instance* in, int kind, int device, double* secs) {
     conf.initialPageRank = 1.0f;
     conf.kind = kind;
     conf.hostRowPtr = in->in_row_ptr->data();
+    conf.blocked = in->bseg ? 1 : 0;
+    conf.hostBseg = in->bseg ? in->bseg->data() : nullptr;
     const int src = in->current, dst = kind == 0 ? in->current : 1 - in->current;
     pmTaskMem mem[3];
     mem[GRAPH_MEM_INDEX] = pmTaskMem(in->graph, READ_ONLY, SUBSCRIPTION_OPTIMAL);
@@ -208,6 +299,83 @@ static int run_kind(Instance* in, int kind, int device, double* secs) {
 }  // namespace pageRank
 
 using namespace pageRank;
+
+// bcnt of the source-blocked layout (groups * NB * GR bytes) from the out-link CSR. Returns 1, or
+// 0 if some page has more than 255 in-links from one source block (the layout cannot hold it).
+extern "C" int ua_pr_block_counts(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                                  uint8_t* bcnt) {
+    const uint32_t bs = block_pages(pages);
+    memset(bcnt, 0, (size_t)groups_of(pages) * NB * GR);
+    for (uint32_t s = 0; s < pages; ++s) {
+        const uint32_t x = s / bs;
+        for (uint32_t e = out_row_ptr[s]; e < out_row_ptr[s + 1]; ++e) {
+            uint8_t& n = bcnt[bcnt_index(out_col[e], x)];
+            if (n == 255) return 0;
+            ++n;
+        }
+    }
+    return 1;
+}
+
+// Bytes of the graph address space.
+extern "C" uint64_t ua_pr_graph_bytes(uint32_t pages, uint64_t edges, int blocked) {
+    uint64_t b = (2ull * pages + 1 + edges) * 4;
+    if (blocked) b += ((uint64_t)groups_of(pages) * NB + 1) * 4 + (uint64_t)groups_of(pages) * NB * GR;
+    return b;
+}
+
+// Fills the graph address space from the out-link CSR: the in-link CSR by a counting sort over
+// targets (sources ascending within each row) and, if bcnt is given (from ua_pr_block_counts),
+// its source-blocked reordering with bseg and bcnt.
+extern "C" int ua_pr_build_graph(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                                 const uint8_t* bcnt, uint32_t* graph) {
+    const size_t N = pages;
+    const uint64_t E = out_row_ptr[pages];
+    uint32_t* rp = graph;
+    uint32_t* deg = graph + N + 1;
+    uint32_t* col = graph + 2 * N + 1;
+    std::vector<uint32_t> cnt(N + 1, 0);
+    for (uint64_t e = 0; e < E; ++e) cnt[out_col[e] + 1]++;
+    rp[0] = 0;
+    for (size_t t = 0; t < N; ++t) rp[t + 1] = rp[t] + cnt[t + 1];
+    std::vector<uint32_t> pos(rp, rp + N);
+    for (uint32_t s = 0; s < pages; ++s) {
+        deg[s] = out_row_ptr[s + 1] - out_row_ptr[s];
+        for (uint32_t e = out_row_ptr[s]; e < out_row_ptr[s + 1]; ++e) col[pos[out_col[e]]++] = s;
+    }
+    if (!bcnt) return 0;
+    const uint32_t G = groups_of(pages);
+    uint32_t* bseg = col + E;
+    memcpy((uint8_t*)(bseg + (size_t)G * NB + 1), bcnt, (size_t)G * NB * GR);
+    // within each row group: block by block, each row's block-x links (a run of its sorted
+    // sources) in row order
+    std::vector<uint32_t> tmp, cur(GR);
+    for (uint32_t g = 0; g < G; ++g) {
+        const uint32_t r0 = g * GR, r1 = (uint32_t)std::min<size_t>(N, (size_t)r0 + GR);
+        const uint32_t l0 = rp[r0], l1 = rp[r1];
+        tmp.assign(col + l0, col + l1);
+        for (uint32_t r = r0; r < r1; ++r) cur[r - r0] = rp[r] - l0;
+        uint32_t k = l0;
+        for (uint32_t x = 0; x < NB; ++x) {
+            bseg[(size_t)g * NB + x] = k;
+            const uint8_t* c = bcnt + ((size_t)g * NB + x) * GR;
+            for (uint32_t r = r0; r < r1; ++r) {
+                const uint32_t n = c[r - r0];
+                memcpy(col + k, tmp.data() + cur[r - r0], (size_t)n * 4);
+                cur[r - r0] += n;
+                k += n;
+            }
+        }
+        if (k != l1) return -1;
+    }
+    bseg[(size_t)G * NB] = (uint32_t)E;
+    return 0;
+}
+
+// Layout of the graphs that ua_pr_create builds from now on: 1 = source-blocked where possible
+// (the default), 0 = plain in-link CSR.
+extern "C" void ua_pr_set_layout(int blocked) { g_use_blocked = blocked ? 1 : 0; }
+extern "C" int ua_pr_blocked(void* h) { return ((Instance*)h)->bseg ? 1 : 0; }
 
 // out_row_ptr / out_col: out-link CSR (0-based targets), e.g. from ua_webgen.
 extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
@@ -233,7 +401,13 @@ extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const
     in->damping = damping;
     in->edges = out_row_ptr[pages];
     const size_t N = pages;
-    const size_t graph_bytes = (2 * N + 1 + in->edges) * 4;
+    std::vector<uint8_t> bcnt;
+    bool blocked = false;
+    if (g_use_blocked && in->edges < (1ull << 32)) {
+        bcnt.resize((size_t)groups_of(pages) * NB * GR);
+        blocked = ua_pr_block_counts(pages, out_row_ptr, out_col, bcnt.data()) == 1;
+    }
+    const size_t graph_bytes = ua_pr_graph_bytes(pages, in->edges, blocked);
     in->inputs = std::make_shared<apps::SharedMem>();
     if (pmCreateMemory(graph_bytes, &in->graph) != pmSuccess) in->graph = nullptr;
     in->inputs->handles.push_back(in->graph);
@@ -247,20 +421,16 @@ extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const
     pmRawMemPtr p;
     pmGetRawMemPtr(in->graph, &p);
     uint32_t* g = (uint32_t*)p;
-    uint32_t* rp = g;
-    uint32_t* deg = g + N + 1;
-    uint32_t* col = g + 2 * N + 1;
-    // in-link CSR by counting sort over targets; sources ascending within each row
-    std::vector<uint32_t> cnt(N + 1, 0);
-    for (uint64_t e = 0; e < in->edges; ++e) cnt[out_col[e] + 1]++;
-    rp[0] = 0;
-    for (size_t t = 0; t < N; ++t) rp[t + 1] = rp[t] + cnt[t + 1];
-    std::vector<uint32_t> pos(rp, rp + N);
-    for (uint32_t s = 0; s < pages; ++s) {
-        deg[s] = out_row_ptr[s + 1] - out_row_ptr[s];
-        for (uint32_t e = out_row_ptr[s]; e < out_row_ptr[s + 1]; ++e) col[pos[out_col[e]]++] = s;
+    if (ua_pr_build_graph(pages, out_row_ptr, out_col, blocked ? bcnt.data() : nullptr, g) != 0) {
+        apps::set_error("ua_pr_create: blocked layout does not add up");
+        ua_pr_destroy(in);
+        return nullptr;
     }
-    in->in_row_ptr = std::make_shared<std::vector<uint32_t>>(rp, rp + N + 1);
+    in->in_row_ptr = std::make_shared<std::vector<uint32_t>>(g, g + N + 1);
+    if (blocked) {
+        const uint32_t* bseg = g + 2 * N + 1 + in->edges;
+        in->bseg = std::make_shared<std::vector<uint32_t>>(bseg, bseg + (size_t)groups_of(pages) * NB + 1);
+    }
     return in;
 }
 
@@ -274,6 +444,7 @@ extern "C" void* ua_pr_clone(void* h) {
     in->edges = src->edges;
     in->damping = src->damping;
     in->in_row_ptr = src->in_row_ptr;
+    in->bseg = src->bseg;
     in->inputs = src->inputs;
     in->graph = src->graph;
     const size_t N = src->pages;

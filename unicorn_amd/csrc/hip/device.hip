@@ -357,6 +357,7 @@ int uh_batch_destroy(void* handle) {
         if (b->dev_slot[i]) hipFree(b->dev_slot[i]);
         if (b->slot_free[i]) hipEventDestroy(b->slot_free[i]);
     }
+    if (b->pr_scratch) hipFree(b->pr_scratch);
     delete b;
     return UH_OK;
 }
@@ -373,7 +374,8 @@ int uh_batch_pending(void* handle, int* n) {
 }
 
 static const size_t kItemBytes[FAM_COUNT] = {
-    sizeof(FilterItem), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem)};
+    sizeof(FilterItem), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem),
+    sizeof(uh_pagerank_args)};
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -412,6 +414,20 @@ int uh_batch_flush(void* handle, int use_graph) {
         b->slot_bytes = slot;
     }
     family_stride = b->slot_bytes / FAM_COUNT;
+    if ((mask >> FAM_PRBLK) & 1) {
+        size_t groups = 0;
+        for (uint32_t t : b->tiles[FAM_PRBLK]) groups += t;
+        const size_t need = groups * kPrGroupScratch;
+        if (need > b->pr_scratch_bytes) {
+            // earlier flushes on the stream may still read the old buffer
+            UH_HIP(hipStreamSynchronize(b->stream));
+            if (b->pr_scratch) UH_HIP(hipFree(b->pr_scratch));
+            b->pr_scratch = nullptr;
+            b->pr_scratch_bytes = 0;
+            UH_HIP(hipMalloc(&b->pr_scratch, need));
+            b->pr_scratch_bytes = need;
+        }
+    }
     int pos = b->ring_pos;
     b->ring_pos = (pos + 1) % Batch::RING;
     UH_HIP(hipEventSynchronize(b->slot_free[pos]));   // previous use of this slot has finished
@@ -426,6 +442,9 @@ int uh_batch_flush(void* handle, int use_graph) {
         h->items_offset = (uint32_t)sizeof(QueueHeader);
         h->prefix_offset = (uint32_t)align_up(sizeof(QueueHeader) + n * kItemBytes[f], 4);
         h->done = 0;
+        memset(h->next_blk, 0, sizeof(h->next_blk));
+        h->scratch = f == FAM_PRBLK ? (uint64_t)(uintptr_t)b->pr_scratch : 0;
+        h->pad2 = 0;
         if (n) memcpy(blk + h->items_offset, b->items[f].data(), n * kItemBytes[f]);
         uint32_t* prefix = (uint32_t*)(blk + h->prefix_offset);
         uint32_t acc = 0;
@@ -448,6 +467,7 @@ int uh_batch_flush(void* handle, int use_graph) {
         if ((mask >> FAM_GEMM) & 1) rc |= gemm_persistent(dev + FAM_GEMM * family_stride, s, b->kinds[FAM_GEMM]);
         if ((mask >> FAM_PAGERANK) & 1) rc |= pagerank_persistent(dev + FAM_PAGERANK * family_stride, s);
         if ((mask >> FAM_TRSM) & 1) rc |= trsm_persistent(dev + FAM_TRSM * family_stride, s);
+        if ((mask >> FAM_PRBLK) & 1) rc |= pagerank_blocked_persistent(dev + FAM_PRBLK * family_stride, s);
         return rc;
     };
     int rc;

@@ -90,6 +90,174 @@ __global__ void __launch_bounds__(PR_THREADS) k_pagerank_persistent(const uint8_
     }
 }
 
+// ---- source-blocked pull ----
+// The CSR pull above gathers w[src] for random sources over the whole vector (40 MB at 10 M
+// pages), far more than one XCD's 4 MB L2, so nearly every gather is a miss served from the
+// fabric. Over the source-blocked layout (see uh_pagerank_args) a unit of work is (row group g,
+// source block x): the group's links from block x form one contiguous col segment, and all the
+// gathers of the unit fall into w's block x (1/8 of the vector). The persistent kernel lets the
+// workgroups of each XCD claim the units of "their" block first, so each XCD's L2 holds one block
+// of w. A unit writes the partial sums of its group's 2048 rows; a second kernel adds the 8
+// partials of each row in block order and produces y and w_next.
+static constexpr uint32_t PR_GR = UH_PR_GROUP_ROWS;
+static constexpr uint32_t PR_NB = UH_PR_BLOCKS;
+static constexpr int PR_BCH = 4096;                          // links staged per chunk
+static constexpr int PR_RPL = PR_GR / PR_THREADS;            // consecutive rows per lane
+static_assert(PR_RPL == 8 && PR_NB == 8, "a lane owns 8 rows; 8 source blocks");
+
+struct PrBlkShared {
+    float val[PR_BCH];                      // w[col[k]] of the current chunk
+    uint32_t wsum[PR_THREADS / 64];         // link totals of each wave's rows
+};
+
+// XCD of the calling wave: hardware register XCC_ID (id 20), bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 7u;
+}
+
+// Unit (group g, block x): part[r] = sum of w[src] over the block-x links of row g*PR_GR + r,
+// in link order, for r in [0, PR_GR). Lane t owns rows 8t .. 8t+7, whose links are consecutive.
+__device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_t g, uint32_t x,
+                                              float* part_, PrBlkShared& sm) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t u = g * PR_NB + x;
+    const UH_GLOBAL uint32_t* cw = (const UH_GLOBAL uint32_t*)(a.bcnt + (size_t)u * PR_GR);
+    const uint32_t c03 = cw[2 * tid], c47 = cw[2 * tid + 1];
+    uint32_t c[PR_RPL];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        c[j] = (c03 >> (8 * j)) & 255u;
+        c[4 + j] = (c47 >> (8 * j)) & 255u;
+    }
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < PR_RPL; ++j) tot += c[j];
+    uint32_t inc = tot;                      // inclusive scan of the lane totals in the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += v;
+    }
+    if (lane == 63) sm.wsum[wave] = inc;
+    const UH_GLOBAL uint32_t* bseg = as_global(a.bseg);
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(bseg[u]);
+    const uint32_t s1 = __builtin_amdgcn_readfirstlane(bseg[u + 1]);
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < wave; ++k) base += sm.wsum[k];
+    const uint32_t first = s0 + base + inc - tot;     // this lane's first link
+    const UH_GLOBAL uint32_t* col = as_global(a.col);
+    const UH_GLOBAL float* w = as_global(a.w);
+    float acc[PR_RPL];
+#pragma unroll
+    for (int j = 0; j < PR_RPL; ++j) acc[j] = 0.f;
+    for (uint32_t c0 = s0; c0 < s1; c0 += PR_BCH) {
+        uint32_t src[PR_BCH / PR_THREADS];
+#pragma unroll
+        for (int j = 0; j < PR_BCH / PR_THREADS; ++j) {
+            const uint32_t k = c0 + tid + j * PR_THREADS;
+            src[j] = k < s1 ? col[k] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int j = 0; j < PR_BCH / PR_THREADS; ++j)
+            sm.val[tid + j * PR_THREADS] = src[j] != 0xffffffffu ? w[src[j]] : 0.f;
+        __syncthreads();
+        uint32_t k = first;
+#pragma unroll
+        for (int j = 0; j < PR_RPL; ++j) {
+            const uint32_t e = k + c[j];
+            const uint32_t lo = k > c0 ? k : c0;
+            const uint32_t hi = e < c0 + PR_BCH ? e : c0 + PR_BCH;
+            for (uint32_t q = lo; q < hi; ++q) acc[j] += sm.val[q - c0];
+            k = e;
+        }
+        __syncthreads();          // the chunk buffer is refilled next
+    }
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    UH_GLOBAL f32x4* part = (UH_GLOBAL f32x4*)as_global(part_ + (size_t)PR_RPL * tid);
+    part[0] = f32x4{acc[0], acc[1], acc[2], acc[3]};
+    part[1] = f32x4{acc[4], acc[5], acc[6], acc[7]};
+    __syncthreads();              // wsum is rewritten by the next unit
+}
+
+// Rows of group g inside the item: y = sum of the 8 block partials in block order, and w_next.
+// part points at the group's partials (block x at part + x * PR_GR).
+__device__ __forceinline__ void pagerank_combine(const uh_pagerank_args& a, uint32_t g, const float* part_) {
+    const UH_GLOBAL float* part = as_global(part_);
+    const UH_GLOBAL uint32_t* deg = as_global(a.deg);
+    UH_GLOBAL float* y = as_global(a.y);
+    UH_GLOBAL float* wn = as_global(a.w_next);
+    const uint32_t end = a.row0 + a.rows;
+#pragma unroll
+    for (int j = 0; j < PR_RPL; ++j) {
+        const uint32_t rl = threadIdx.x + j * PR_THREADS, r = g * PR_GR + rl;
+        float t = 0.f;
+#pragma unroll
+        for (uint32_t x = 0; x < PR_NB; ++x) t += part[x * PR_GR + rl];
+        if (r >= a.row0 && r < end) {
+            y[r] = t;
+            const uint32_t d = deg[r];
+            wn[r] = d ? (a.damping * t) / (float)d : 0.f;
+        }
+    }
+}
+
+// Plain launch of one item: block b = unit (group row0 / PR_GR + b / 8, block b % 8). Workgroups
+// are dispatched round-robin over the 8 XCDs, so block x runs on XCD x.
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_blocked(uh_pagerank_args a, float* scratch) {
+    __shared__ PrBlkShared sm;
+    const uint32_t b = blockIdx.x;
+    pagerank_unit(a, a.row0 / PR_GR + b / PR_NB, b % PR_NB, scratch + (size_t)b * PR_GR, sm);
+}
+
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_combine(uh_pagerank_args a, const float* scratch) {
+    const uint32_t gl = blockIdx.x;
+    pagerank_combine(a, a.row0 / PR_GR + gl, scratch + (size_t)gl * PR_NB * PR_GR);
+}
+
+// Batched: tiles of the queue are (item, local group) pairs, k = the global group index; unit
+// (k, x) writes its partials to scratch + (k * 8 + x) * PR_GR. A workgroup claims the units of
+// its own XCD's block first, then helps with the other blocks, so every unit is taken whatever
+// the placement of the workgroups.
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_blocked_persistent(const uint8_t* queue) {
+    __shared__ PrBlkShared sm;
+    __shared__ uint32_t s_k;
+    QueueHeader* h = (QueueHeader*)queue;
+    const uh_pagerank_args* items = (const uh_pagerank_args*)(queue + h->items_offset);
+    const uint32_t* prefix = (const uint32_t*)(queue + h->prefix_offset);
+    const uint32_t total = h->total_tiles, n_items = h->n_items;
+    float* scratch = (float*)h->scratch;
+    const uint32_t x0 = __builtin_amdgcn_readfirstlane(xcc_id());
+    for (uint32_t i = 0; i < PR_NB; ++i) {
+        const uint32_t x = (x0 + i) % PR_NB;
+        for (;;) {
+            if (threadIdx.x == 0) s_k = atomicAdd(&h->next_blk[x], 1u);
+            __syncthreads();
+            const uint32_t k = __builtin_amdgcn_readfirstlane(s_k);
+            __syncthreads();
+            if (k >= total) break;
+            uint32_t item, local;
+            queue_locate(prefix, n_items, k, item, local);
+            const uh_pagerank_args a = uniform_load(items + item);
+            pagerank_unit(a, a.row0 / PR_GR + local, x, scratch + ((size_t)k * PR_NB + x) * PR_GR, sm);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PR_THREADS) k_pagerank_combine_persistent(const uint8_t* queue) {
+    const QueueHeader* h = (const QueueHeader*)queue;
+    const uh_pagerank_args* items = (const uh_pagerank_args*)(queue + h->items_offset);
+    const uint32_t* prefix = (const uint32_t*)(queue + h->prefix_offset);
+    const uint32_t total = h->total_tiles, n_items = h->n_items;
+    const float* scratch = (const float*)h->scratch;
+    for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+        uint32_t item, local;
+        queue_locate(prefix, n_items, k, item, local);
+        const uh_pagerank_args a = uniform_load(items + item);
+        pagerank_combine(a, a.row0 / PR_GR + local, scratch + (size_t)k * PR_NB * PR_GR);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n,
                                                          float x0, float damping) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -116,6 +284,19 @@ uint32_t pagerank_tiles(const uh_pagerank_args& a) {
     return (a.rows + PR_ROWS_PER_TILE - 1) / PR_ROWS_PER_TILE;
 }
 
+uint32_t pagerank_groups(const uh_pagerank_args& a) {
+    return (a.row0 + a.rows - 1) / PR_GR - a.row0 / PR_GR + 1;
+}
+
+int pagerank_blocked_persistent(const uint8_t* queue, hipStream_t s) {
+    UH_LAUNCH(k_pagerank_blocked_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
+    UH_LAUNCH_CHECK("k_pagerank_blocked_persistent");
+    UH_LAUNCH(k_pagerank_combine_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
+    UH_LAUNCH_CHECK("k_pagerank_combine_persistent");
+    g_persistent_launches += 2;
+    return UH_OK;
+}
+
 int pagerank_persistent(const uint8_t* queue, hipStream_t s) {
     UH_LAUNCH(k_pagerank_persistent, dim3(cu_count() * 8), dim3(PR_THREADS), 0, s, queue);
     UH_LAUNCH_CHECK("k_pagerank_persistent");
@@ -127,11 +308,45 @@ int pagerank_persistent(const uint8_t* queue, hipStream_t s) {
 
 using namespace uh;
 
+// Partial sums of plain source-blocked launches, per stream: launches on one stream run in
+// stream order and share the buffer.
+static std::map<hipStream_t, std::pair<float*, size_t>> g_pr_scratch;
+static std::mutex g_pr_mtx;
+
 extern "C" int uh_pagerank_pull(const uh_pagerank_args* a, void* stream) {
     if (!a || !a->row_ptr || !a->w || !a->y || !a->w_next || !a->deg)
         return fail(UH_ERR_ARG, "uh_pagerank_pull: null pointer");
     if (a->rows == 0) return UH_OK;
     if (!a->col) return fail(UH_ERR_ARG, "uh_pagerank_pull: null col");
+    if (a->bcnt) {
+        if (!a->bseg) return fail(UH_ERR_ARG, "uh_pagerank_pull: bcnt without bseg");
+        if ((uintptr_t)a->bcnt & 3u) return fail(UH_ERR_ARG, "uh_pagerank_pull: bcnt not 4-byte aligned");
+        if ((uint64_t)a->row0 + a->rows > 0xffffffffull) return fail(UH_ERR_ARG, "uh_pagerank_pull: rows past 2^32");
+        const uint32_t groups = pagerank_groups(*a);
+        if (Batch* b = as_batch(stream)) {
+            record_item(b, FAM_PRBLK, a, sizeof(*a), groups);
+            return UH_OK;
+        }
+        hipStream_t s = (hipStream_t)stream;
+        const size_t need = groups * kPrGroupScratch;
+        float* scratch;
+        {
+            std::lock_guard<std::mutex> lk(g_pr_mtx);
+            auto& e = g_pr_scratch[s];
+            if (e.second < need) {
+                if (e.first) { UH_HIP(hipStreamSynchronize(s)); UH_HIP(hipFree(e.first)); e.first = nullptr; e.second = 0; }
+                UH_HIP(hipMalloc(&e.first, need));
+                e.second = need;
+            }
+            scratch = e.first;
+        }
+        UH_LAUNCH(k_pagerank_blocked, dim3(groups * PR_NB), dim3(PR_THREADS), 0, s, *a, scratch);
+        UH_LAUNCH_CHECK("k_pagerank_blocked");
+        UH_LAUNCH(k_pagerank_combine, dim3(groups), dim3(PR_THREADS), 0, s, *a, scratch);
+        UH_LAUNCH_CHECK("k_pagerank_combine");
+        g_plain_launches += 2;
+        return UH_OK;
+    }
     const uint32_t tiles = pagerank_tiles(*a);
     if (Batch* b = as_batch(stream)) {
         record_item(b, FAM_PAGERANK, a, sizeof(*a), tiles);

@@ -44,7 +44,8 @@ extern std::atomic<uint64_t> g_items;
 int cu_count();
 
 // Kernel families that a batch can hold. Each family records fixed-size descriptors.
-enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_COUNT = 4 };
+// FAM_PRBLK holds pageRank items over the source-blocked layout; their tiles are row groups.
+enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_PRBLK = 4, FAM_COUNT = 5 };
 
 // Work item of the filter family: the launcher's arguments (pad = view width in pixels) plus the
 // dot4 weight vectors of radius <= 2, packed once on the host (wv[dy][q][d], see imagefilter.hip).
@@ -74,7 +75,11 @@ struct QueueHeader {
     uint32_t prefix_offset;   // bytes from the header start
     uint32_t done;            // workgroups finished (for the last-one-out reset)
     uint32_t pad[2];
+    uint32_t next_blk[8];     // per source block claim counters (FAM_PRBLK), zeroed by the upload
+    uint64_t scratch;         // device scratch of the flush (FAM_PRBLK partial sums), or 0
+    uint64_t pad2;
 };
+static_assert(sizeof(QueueHeader) % 16 == 0, "work items follow the header");
 
 struct Batch {
     hipStream_t stream = nullptr;
@@ -96,6 +101,10 @@ struct Batch {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
     double timed_ms = 0;
     uint64_t timed_flushes = 0;
+    // partial sums of the FAM_PRBLK items of one flush (flushes on the batch stream run in
+    // order, so one buffer serves all of them); grown when a flush needs more
+    void* pr_scratch = nullptr;
+    size_t pr_scratch_bytes = 0;
 };
 
 Batch* as_batch(void* stream);   // nullptr if the stream is not a batch handle
@@ -117,6 +126,7 @@ int gemm_variant_for(uint32_t kinds);
 // Workgroups of the persistent GEMM kernels resident per CU (occupancy queries, cached).
 int gemm_wg_per_cu();
 int pagerank_persistent(const uint8_t* queue, hipStream_t s);
+int pagerank_blocked_persistent(const uint8_t* queue, hipStream_t s);
 int trsm_persistent(const uint8_t* queue, hipStream_t s);
 // Workgroups of the persistent filter kernel resident per CU (an occupancy query, cached).
 int filter_wg_per_cu();
@@ -125,6 +135,9 @@ int filter_wg_per_cu();
 uint32_t filter_tiles(const FilterItem& it);
 uint32_t gemm_tiles(const uh_gemm_args& a);
 uint32_t pagerank_tiles(const uh_pagerank_args& a);
+uint32_t pagerank_groups(const uh_pagerank_args& a);   // row groups of a source-blocked item
+// Bytes of partial sums that one row group of a source-blocked item needs.
+constexpr size_t kPrGroupScratch = (size_t)UH_PR_BLOCKS * UH_PR_GROUP_ROWS * sizeof(float);
 uint32_t trsm_tiles(const TrsmItem& a);
 
 // Address spaces. A pointer that the compiler cannot trace to a kernel argument (for example

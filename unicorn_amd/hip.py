@@ -22,7 +22,8 @@ class uh_gemm_args(C.Structure):
 
 class uh_pagerank_args(C.Structure):
     _fields_ = [("row_ptr", vp), ("col", vp), ("deg", vp), ("w", vp), ("y", vp), ("w_next", vp),
-                ("row0", u32), ("rows", u32), ("damping", f32), ("pad", u32)]
+                ("row0", u32), ("rows", u32), ("damping", f32), ("pad", u32),
+                ("bcnt", vp), ("bseg", vp)]
 
 
 _declared = False
