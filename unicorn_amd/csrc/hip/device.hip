@@ -444,7 +444,6 @@ int uh_batch_flush(void* handle, int use_graph) {
         h->done = 0;
         memset(h->next_blk, 0, sizeof(h->next_blk));
         h->scratch = f == FAM_PRBLK ? (uint64_t)(uintptr_t)b->pr_scratch : 0;
-        h->pad2 = 0;
         if (n) memcpy(blk + h->items_offset, b->items[f].data(), n * kItemBytes[f]);
         uint32_t* prefix = (uint32_t*)(blk + h->prefix_offset);
         uint32_t acc = 0;

@@ -1,6 +1,9 @@
 // pageRank kernels for gfx950 (replace pageRank_cuda / zeroInit,
 // testSuite/pageRank/source/code/src/pageRankCUDA.cu:46-164).
 //
+// Two pulls over the HBM-resident in-link graph: the source-blocked one (below the CSR one),
+// used whenever the graph's layout allows it, and the CSR-stream one described here.
+//
 // The reference pushes contributions with float atomics from every source page into a
 // full-length output vector per subtask, over a padded record format copied from pageable host
 // memory each iteration. Here the graph is resident in HBM as an in-link CSR (rows = destination
@@ -123,15 +126,11 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
     const uint32_t u = g * PR_NB + x;
     const UH_GLOBAL uint32_t* cw = (const UH_GLOBAL uint32_t*)(a.bcnt + (size_t)u * PR_GR);
     const uint32_t c03 = cw[2 * tid], c47 = cw[2 * tid + 1];
-    uint32_t c[PR_RPL];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        c[j] = (c03 >> (8 * j)) & 255u;
-        c[4 + j] = (c47 >> (8 * j)) & 255u;
-    }
+    // row j's count, unpacked where it is used (keeps 6 registers free)
+    auto cnt = [&](int j) { return ((j < 4 ? c03 : c47) >> (8 * (j & 3))) & 255u; };
     uint32_t tot = 0;
 #pragma unroll
-    for (int j = 0; j < PR_RPL; ++j) tot += c[j];
+    for (int j = 0; j < PR_RPL; ++j) tot += cnt(j);
     uint32_t inc = tot;                      // inclusive scan of the lane totals in the wave
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -146,26 +145,29 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
     uint32_t base = 0;
     for (uint32_t k = 0; k < wave; ++k) base += sm.wsum[k];
     const uint32_t first = s0 + base + inc - tot;     // this lane's first link
-    const UH_GLOBAL uint32_t* col = as_global(a.col);
-    const UH_GLOBAL float* w = as_global(a.w);
+    // Buffer loads: 32-bit offsets from a scalar base, so a gather costs one address register
+    // (source ids are below 2^30). The col resource ends at the segment end, where loads return
+    // source 0; the val entries past the segment are never summed.
+    const __amdgpu_buffer_rsrc_t w_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.w, 0, -1, 0x00020000);
     float acc[PR_RPL];
 #pragma unroll
     for (int j = 0; j < PR_RPL; ++j) acc[j] = 0.f;
     for (uint32_t c0 = s0; c0 < s1; c0 += PR_BCH) {
+        const uint32_t len = s1 - c0 < (uint32_t)PR_BCH ? s1 - c0 : (uint32_t)PR_BCH;
+        const __amdgpu_buffer_rsrc_t c_rsrc =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(a.col + c0), 0, (int)(len * 4), 0x00020000);
         uint32_t src[PR_BCH / PR_THREADS];
 #pragma unroll
-        for (int j = 0; j < PR_BCH / PR_THREADS; ++j) {
-            const uint32_t k = c0 + tid + j * PR_THREADS;
-            src[j] = k < s1 ? col[k] : 0xffffffffu;
-        }
+        for (int j = 0; j < PR_BCH / PR_THREADS; ++j)
+            src[j] = __builtin_amdgcn_raw_buffer_load_b32(c_rsrc, tid * 4u, j * PR_THREADS * 4, 0);
 #pragma unroll
         for (int j = 0; j < PR_BCH / PR_THREADS; ++j)
-            sm.val[tid + j * PR_THREADS] = src[j] != 0xffffffffu ? w[src[j]] : 0.f;
+            sm.val[tid + j * PR_THREADS] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(w_rsrc, src[j] * 4u, 0, 0));
         __syncthreads();
         uint32_t k = first;
 #pragma unroll
         for (int j = 0; j < PR_RPL; ++j) {
-            const uint32_t e = k + c[j];
+            const uint32_t e = k + cnt(j);
             const uint32_t lo = k > c0 ? k : c0;
             const uint32_t hi = e < c0 + PR_BCH ? e : c0 + PR_BCH;
             for (uint32_t q = lo; q < hi; ++q) acc[j] += sm.val[q - c0];
@@ -218,8 +220,10 @@ __global__ void __launch_bounds__(PR_THREADS) k_pagerank_combine(uh_pagerank_arg
 // Batched: tiles of the queue are (item, local group) pairs, k = the global group index; unit
 // (k, x) writes its partials to scratch + (k * 8 + x) * PR_GR. A workgroup claims the units of
 // its own XCD's block first, then helps with the other blocks, so every unit is taken whatever
-// the placement of the workgroups.
-__global__ void __launch_bounds__(PR_THREADS) k_pagerank_blocked_persistent(const uint8_t* queue) {
+// the placement of the workgroups. Held to 64 VGPRs (8 waves per SIMD, all 8 workgroups of a CU
+// resident), as the plain kernel is by itself: the gathers need every wave in flight.
+__global__ void __launch_bounds__(PR_THREADS)
+k_pagerank_blocked_persistent(const uint8_t* queue) {
     __shared__ PrBlkShared sm;
     __shared__ uint32_t s_k;
     QueueHeader* h = (QueueHeader*)queue;
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(PR_THREADS) k_pagerank_blocked_persistent(cons
     for (uint32_t i = 0; i < PR_NB; ++i) {
         const uint32_t x = (x0 + i) % PR_NB;
         for (;;) {
-            if (threadIdx.x == 0) s_k = atomicAdd(&h->next_blk[x], 1u);
+            if (threadIdx.x == 0) s_k = atomicAdd(&h->next_blk[x * 32], 1u);
             __syncthreads();
             const uint32_t k = __builtin_amdgcn_readfirstlane(s_k);
             __syncthreads();

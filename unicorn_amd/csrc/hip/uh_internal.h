@@ -75,11 +75,15 @@ struct QueueHeader {
     uint32_t prefix_offset;   // bytes from the header start
     uint32_t done;            // workgroups finished (for the last-one-out reset)
     uint32_t pad[2];
-    uint32_t next_blk[8];     // per source block claim counters (FAM_PRBLK), zeroed by the upload
     uint64_t scratch;         // device scratch of the flush (FAM_PRBLK partial sums), or 0
-    uint64_t pad2;
+    uint32_t pad2[22];
+    // Per source block claim counters of FAM_PRBLK (counter x at next_blk[x * 32]), zeroed by the
+    // upload. Each sits in its own 128-byte line: on one shared line the claims of the 8 XCDs
+    // contended and cost a fifth of the kernel's time.
+    uint32_t next_blk[8 * 32];
 };
-static_assert(sizeof(QueueHeader) % 16 == 0, "work items follow the header");
+static_assert(sizeof(QueueHeader) % 128 == 0 && offsetof(QueueHeader, next_blk) % 128 == 0,
+              "claim counters on their own lines; work items follow the header");
 
 struct Batch {
     hipStream_t stream = nullptr;
