@@ -85,6 +85,10 @@ pmStatus pmPrefetchToDevice(pmMemHandle pMemHandle);
  * read it on the GPU; writes through it must be followed by pmMarkDeviceWritten. */
 pmStatus pmGetDeviceMemPtr(pmMemHandle pMemHandle, void** pDevicePtr);
 pmStatus pmMarkDeviceWritten(pmMemHandle pMemHandle);
+/* Copies bytes [pOffset, pOffset + pLength) of an address space's current contents (from the HBM
+ * mirror when that is the newer copy) to pDst without changing where the space is valid: unlike
+ * pmFetchMemory + pmGetRawMemPtr, the HBM mirror stays current for the next task. */
+pmStatus pmReadMemoryRange(pmMemHandle pMemHandle, size_t pOffset, size_t pLength, void* pDst);
 /* Device time of the BATCHED mode's flushes (work-queue upload + persistent kernels), measured
  * with events on the stream the persistent kernels run on. */
 pmStatus pmSetHipKernelTiming(bool pEnable);

@@ -225,7 +225,8 @@ static pmStatus s3_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* stre
 // ---- solve with iterative refinement ----
 // Address spaces: 0 = LU factors (READ_ONLY), 1 = pristine A (READ_ONLY), 2 = solve state
 // (READ_WRITE): [b | x | r] n x nrhs float64 column-major each, [rf] n x nrhs float32, then
-// [norms] 2 * 8 float64 (max|r| per column, max|x| per column).
+// [norms] 2 * 8 float64 (max|r| per column, max|x| per column). Between steps the host reads
+// only the norms (pmReadMemoryRange), so the state stays resident in HBM until x is fetched.
 
 struct SolveLayout {
     size_t n, nrhs;
@@ -261,7 +262,6 @@ static pmStatus solve_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* s
     double* x = (double*)(st + L.x());
     double* r = (double*)(st + L.r());
     float* rf = (float*)(st + L.rf());
-    double* norms = (double*)(st + L.norms());
     if (c->taskStep == REFINE_INIT) {
         // x = 0
         if (uh_fill_u32(x, 0u, (size_t)N * k * 2, stream)) return gpu_error("fill");
@@ -269,6 +269,7 @@ static pmStatus solve_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* s
     }
     const float* lu = (const float*)si.memInfo[0].ptr;
     const float* a = (const float*)si.memInfo[1].ptr;
+    double* norms = (double*)(st + L.norms());
     if (uh_residual_f64(a, (uint64_t)N, N, x, b, k, r, rf, stream)) return gpu_error("uh_residual_f64");
     if (uh_lu_solve_f32(lu, (uint64_t)N, N, rf, k, stream)) return gpu_error("uh_lu_solve_f32");
     if (uh_refine_update(x, rf, N, k, stream)) return gpu_error("uh_refine_update");
@@ -288,13 +289,13 @@ static pmStatus solve_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
     double* x = (double*)(st + L.x());
     double* r = (double*)(st + L.r());
     float* rf = (float*)(st + L.rf());
-    double* norms = (double*)(st + L.norms());
     if (c->taskStep == REFINE_INIT) {
         memset(x, 0, N * k * 8);
         return pmSuccess;
     }
     const float* lu = (const float*)si.memInfo[0].ptr;
     const float* a = (const float*)si.memInfo[1].ptr;
+    double* norms = (double*)(st + L.norms());
     auto residual = [&]() {
         for (size_t q = 0; q < k; ++q) {
             for (size_t i = 0; i < N; ++i) r[q * N + i] = b[q * N + i];
@@ -511,10 +512,9 @@ extern "C" int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, in
         ++steps;
         if (rc) break;
         const double t0 = apps::now_secs();
-        st = pmFetchMemory(in->state);
+        double norms[16];
+        st = pmReadMemoryRange(in->state, L.norms(), sizeof(norms), norms);
         if (st != pmSuccess) { rc = (int)st; break; }
-        pmGetRawMemPtr(in->state, &p);
-        const double* norms = (const double*)((uint8_t*)p + L.norms());
         berr = 0;
         for (int q = 0; q < nrhs; ++q) berr = std::max(berr, norms[q] / (in->a_inf * norms[8 + q] + bmax));
         t += apps::now_secs() - t0;
@@ -522,6 +522,8 @@ extern "C" int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, in
         if (berr <= eps) break;
     }
     if (rc) return rc;
+    st = pmFetchMemory(in->state);
+    if (st != pmSuccess) return (int)st;
     pmGetRawMemPtr(in->state, &p);
     memcpy(x, (uint8_t*)p + L.x(), N * nrhs * 8);
     if (info) { info[0] = steps; info[1] = berr; info[2] = first; info[3] = in->a_inf; }

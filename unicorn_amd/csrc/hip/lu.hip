@@ -492,37 +492,77 @@ int trsm_persistent(const uint8_t* queue, hipStream_t s) {
     return UH_OK;
 }
 
-// ---------------------------------------------------------------- sync-free triangular solve
+// ---------------------------------------------------------------- blocked triangular solve
 
-// The refinement's solve L U z = r for nrhs <= 8 vectors, one launch per triangle. Rows are cut
-// into 32-row blocks and a workgroup claims the next block by ticket (forward: top down;
-// backward: bottom up), so a block depends only on blocks claimed earlier, by workgroups that
-// are already running: the lowest unfinished ticket can always proceed, whatever the number of
-// resident workgroups. A workgroup streams its row panel in chunks of 8 tiles of 32 x 32 (the
-// loads do not depend on other blocks and run one chunk ahead), waits for each tile's block flag
-// before reading that block's solution, substitutes through its diagonal tile and publishes its
-// block: stores, agent-scope release, flag. Every wait is bounded: past the limit the block is
-// written as NaN and published anyway, so every workgroup exits and the refinement sees a
-// non-finite residual.
-static constexpr int TV_CH = 8;   // tiles per chunk (one per group of 32 threads)
+// The refinement's solve L U z = r for nrhs <= 8 vectors: one launch inverting the 128 x 128
+// diagonal blocks of both triangles, then one launch per triangle. Rows are cut into 128-row
+// blocks and a workgroup claims the next block by ticket (forward: top down; backward: bottom
+// up), so a block depends only on blocks claimed earlier, by workgroups that are already
+// running: the lowest unfinished ticket can always proceed, whatever the number of resident
+// workgroups. A workgroup streams its row panel one 128 x 128 tile ahead (those loads depend on
+// no other block), waits for each tile's block flag before reading that block's solution,
+// multiplies its remaining right-hand side by the inverted diagonal block and publishes its
+// block: stores, agent-scope release, flag. The dependency chain is N/128 blocks long and its
+// diagonal step is a parallel product, not a 128-step substitution. Every wait is bounded: past
+// the limit the block is written as NaN and published anyway, so every workgroup exits and the
+// refinement sees a non-finite residual.
+static constexpr int TV_B = 128;          // rows per block
+static constexpr int TV_H = TV_B / 2;     // columns of a tile per thread (two column halves)
 
 struct TrsvArgs {
     const float* lu;
     uint64_t ld;
     float* z;          // n x nrhs column-major, leading dimension n, solved in place
+    const float* inv;  // inverted diagonal blocks, [block][triangle] 128 x 128 column-major
     uint32_t* sync;    // [0] ticket, [1 + t] flag of ticket t; zero before the launch
     int32_t n, nrhs, upper, spin_limit;
 };
 
-__global__ void __launch_bounds__(256) k_trsv_sync_free(TrsvArgs a) {
-    __shared__ float zs[TV_CH * 32][8];
-    __shared__ float red[TV_CH][32][9];
+// Inverse of diagonal block p of the lower non-unit (blockIdx.y = 0) or the upper unit (1)
+// triangle of the packed factors; rows and columns past n are taken as the identity. Thread j
+// computes column j by substitution against the block held in LDS.
+__global__ void __launch_bounds__(TV_B) k_trsv_diag_inv(const float* lu, uint64_t ld, int n, float* inv) {
+    __shared__ float T[TV_B][TV_B + 1];
+    __shared__ float X[TV_B][TV_B + 1];
+    const int p = blockIdx.x, upper = blockIdx.y, j = threadIdx.x;
+    const int b0 = p * TV_B;
+    const UH_GLOBAL float* A = as_global(lu);
+    for (int c = 0; c < TV_B; ++c) {          // column c, row j: coalesced over the threads
+        const int gi = b0 + j, gc = b0 + c;
+        const bool in = gi < n && gc < n && (upper ? c > j : c <= j);
+        const UH_GLOBAL float* q = in ? A + (uint64_t)gc * ld + gi : A;
+        const float v = *q;
+        T[j][c] = in ? v : (c == j ? 1.f : 0.f);
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int i = 0; i < TV_B; ++i) {      // L X = I, rows top down; X[i][j] = 0 for i < j
+            float s = i == j ? 1.f : 0.f;
+            for (int k = j; k < i; ++k) s = fmaf(-T[i][k], X[k][j], s);
+            X[i][j] = s / T[i][i];
+        }
+    } else {
+        for (int i = TV_B - 1; i >= 0; --i) { // U X = I, rows bottom up; X[i][j] = 0 for i > j
+            float s = i == j ? 1.f : 0.f;
+            for (int k = i + 1; k <= j; ++k) s = fmaf(-T[i][k], X[k][j], s);
+            X[i][j] = s;
+        }
+    }
+    __syncthreads();
+    UH_GLOBAL float* out = as_global(inv) + ((size_t)p * 2 + upper) * TV_B * TV_B;
+    for (int c = 0; c < TV_B; ++c) out[c * TV_B + j] = X[j][c];
+}
+
+__global__ void __launch_bounds__(256) k_trsv_blocked(TrsvArgs a) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    __shared__ __attribute__((aligned(16))) float xs[2][TV_B][8];   // a column block's solution
+    __shared__ __attribute__((aligned(16))) float red[2][TV_B][8];  // the column halves' sums
+    __shared__ __attribute__((aligned(16))) float rs[TV_B][8];      // right-hand side of the diagonal step
     __shared__ uint32_t s_t;
     __shared__ int s_fail;
-    const int tid = threadIdx.x, r = tid & 31, cc = tid >> 5, lane = tid & 63;
-    const int n = a.n, nb = (n + 31) / 32;
+    const int tid = threadIdx.x, i = tid & (TV_B - 1), h = tid >> 7;
+    const int n = a.n, nb = (n + TV_B - 1) / TV_B;
     const int64_t ld = (int64_t)a.ld;
-    const UH_GLOBAL float* lu = as_global(a.lu);
     UH_GLOBAL float* z = as_global(a.z);
     for (;;) {
         if (tid == 0) {
@@ -533,35 +573,40 @@ __global__ void __launch_bounds__(256) k_trsv_sync_free(TrsvArgs a) {
         const int t = (int)__builtin_amdgcn_readfirstlane(s_t);
         if (t >= nb) break;
         const int p = a.upper ? nb - 1 - t : t;   // row block
-        const int R = p * 32 + r;                  // this thread's row
+        const int R = p * TV_B + i;                // this thread's row
         auto colblock = [&](int d) { return a.upper ? nb - 1 - d : d; };   // row block of ticket d
-        auto load_tile = [&](int d, float (&l)[32]) {   // l[j] = M(R, 32 colblock(d) + j)
-            const int c0 = colblock(d) * 32;
+        // Tile loads through a buffer resource based at the tile's first column: the lane offset
+        // (row) is fixed and the column step is a scalar offset, so the 64 loads need no address
+        // registers. Columns past n fall outside the resource and read 0; rows past n are masked.
+        const uint32_t row_off = (uint32_t)R * 4u;
+        const uint32_t col_step = (uint32_t)__builtin_amdgcn_readfirstlane((int)(ld * 4));
+        auto load_tile = [&](int d, float (&m)[TV_H]) {   // m[c] = M(R, 128 colblock(d) + 64 h + c)
+            const int c0 = __builtin_amdgcn_readfirstlane(colblock(d) * TV_B + h * TV_H);
+            // bytes from column c0 to the end of the last column of the 64 that exist (none: 0)
+            const int cols = min(n - c0, TV_H);
+            const int bytes = cols > 0 ? (int)((uint32_t)(cols - 1) * col_step + (uint32_t)n * 4u) : 0;
+            const __amdgpu_buffer_rsrc_t rs_m = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(a.lu + (int64_t)c0 * ld), 0, bytes, 0x00020000);
+            // rows past n read finite values that only reach rows whose results are dropped
 #pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                const bool ok = d < t && R < n && c0 + j < n;
-                const UH_GLOBAL float* q = ok ? lu + (int64_t)(c0 + j) * ld + R : lu;
-                const float v = *q;
-                l[j] = ok ? v : 0.f;
-            }
+            for (int c = 0; c < TV_H; ++c)
+                m[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_m, row_off, c * col_step, 0));
         };
-        // the diagonal tile's row R and the pivot
-        float dl[32];
+        // row i of the inverted diagonal block, this half's columns
+        float dv[TV_H];
+        {
+            const __amdgpu_buffer_rsrc_t rs_i = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(a.inv + ((int64_t)p * 2 + a.upper) * TV_B * TV_B + (int64_t)h * TV_H * TV_B), 0,
+                TV_H * TV_B * 4, 0x00020000);
 #pragma unroll
-        for (int j = 0; j < 32; ++j) {
-            const bool ok = R < n && p * 32 + j < n;
-            const UH_GLOBAL float* q = ok ? lu + (int64_t)(p * 32 + j) * ld + R : lu;
-            const float v = *q;
-            dl[j] = ok ? v : 0.f;
+            for (int c = 0; c < TV_H; ++c)
+                dv[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_i, (uint32_t)i * 4u, c * TV_B * 4, 0));
         }
-        const float piv = R < n ? lu[(int64_t)R * ld + R] : 1.f;
         float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        float lc[32], ln[32];
-        load_tile(cc, lc);
-        for (int d0 = 0; d0 < t; d0 += TV_CH) {
-            if (d0 + TV_CH < t) load_tile(d0 + TV_CH + cc, ln);
-            if (tid < TV_CH && d0 + tid < t) {
-                uint32_t* f = a.sync + 1 + d0 + tid;
+        // tile d: wait for its block, stage that block's solution, accumulate M(R, .) x
+        auto step = [&](int d, const float (&m)[TV_H]) {
+            if (tid == 0) {
+                uint32_t* f = a.sync + 1 + d;
                 int spins = 0;
                 while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                     if (++spins > a.spin_limit) {
@@ -573,64 +618,92 @@ __global__ void __launch_bounds__(256) k_trsv_sync_free(TrsvArgs a) {
             }
             __syncthreads();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int buf = d & 1;
+            {   // xs[buf][c][4 kg .. 4 kg + 3] = z(128 colblock(d) + c, k): thread (c = i, kg = h)
+                const int col = colblock(d) * TV_B + i;
+                float v[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {   // zs[32 dd + j][k] = z(32 colblock(d0 + dd) + j, k)
-                const int e = u * 256 + tid, k = e & 7, jj = e >> 3, dd = jj >> 5, j = jj & 31;
-                const int col = colblock(d0 + dd) * 32 + j;
-                const bool ok = d0 + dd < t && col < n && k < a.nrhs;
-                const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + col : z;
-                const float v = *q;
-                zs[jj][k] = ok ? v : 0.f;
+                for (int e = 0; e < 4; ++e) {
+                    const int k = h * 4 + e;
+                    const bool ok = k < a.nrhs && col < n;
+                    const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + col : z;
+                    const float x = *q;
+                    v[e] = ok ? x : 0.f;
+                }
+                *(f32x4*)&xs[buf][i][h * 4] = f32x4{v[0], v[1], v[2], v[3]};
             }
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                const float4 z0 = *(const float4*)&zs[cc * 32 + j][0];
-                const float4 z1 = *(const float4*)&zs[cc * 32 + j][4];
-                acc[0] = fmaf(lc[j], z0.x, acc[0]);
-                acc[1] = fmaf(lc[j], z0.y, acc[1]);
-                acc[2] = fmaf(lc[j], z0.z, acc[2]);
-                acc[3] = fmaf(lc[j], z0.w, acc[3]);
-                acc[4] = fmaf(lc[j], z1.x, acc[4]);
-                acc[5] = fmaf(lc[j], z1.y, acc[5]);
-                acc[6] = fmaf(lc[j], z1.z, acc[6]);
-                acc[7] = fmaf(lc[j], z1.w, acc[7]);
+            for (int c = 0; c < TV_H; ++c) {
+                const f32x4 x0 = *(const f32x4*)&xs[buf][h * TV_H + c][0];
+                const f32x4 x1 = *(const f32x4*)&xs[buf][h * TV_H + c][4];
+                acc[0] = fmaf(m[c], x0.x, acc[0]);
+                acc[1] = fmaf(m[c], x0.y, acc[1]);
+                acc[2] = fmaf(m[c], x0.z, acc[2]);
+                acc[3] = fmaf(m[c], x0.w, acc[3]);
+                acc[4] = fmaf(m[c], x1.x, acc[4]);
+                acc[5] = fmaf(m[c], x1.y, acc[5]);
+                acc[6] = fmaf(m[c], x1.z, acc[6]);
+                acc[7] = fmaf(m[c], x1.w, acc[7]);
             }
-#pragma unroll
-            for (int j = 0; j < 32; ++j) lc[j] = ln[j];
+        };
+        // two tile buffers in turn, the next tile's loads in flight during a step
+        float ma[TV_H], mb[TV_H];
+        load_tile(0, ma);
+        for (int d = 0; d < t; d += 2) {
+            if (d + 1 < t) load_tile(d + 1, mb);
+            step(d, ma);
+            if (d + 1 >= t) break;
+            if (d + 2 < t) load_tile(d + 2, ma);
+            step(d + 1, mb);
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) red[cc][r][k] = acc[k];
+        *(f32x4*)&red[h][i][0] = f32x4{acc[0], acc[1], acc[2], acc[3]};
+        *(f32x4*)&red[h][i][4] = f32x4{acc[4], acc[5], acc[6], acc[7]};
         __syncthreads();
-        // thread (row r, vector k = cc): x = rhs - sum over the chunk groups in a fixed order
-        const int k = cc;
-        float sum = 0.f;
+        {   // thread (row i, vectors 4h .. 4h+3): rhs minus the two halves, in a fixed order
+            const f32x4 s0 = *(const f32x4*)&red[0][i][h * 4];
+            const f32x4 s1 = *(const f32x4*)&red[1][i][h * 4];
+            const float sum[4] = {s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w};
+            float r[4];
 #pragma unroll
-        for (int g = 0; g < TV_CH; ++g) sum += red[g][r][k];
-        const bool mine = R < n && k < a.nrhs;
-        float x = mine ? z[(int64_t)k * n + R] - sum : 0.f;
-        // substitution through the diagonal tile; lanes 0..31 hold vector 2w, 32..63 vector 2w+1
-        if (!a.upper) {
-            const float rd = 1.0f / piv;
-#pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                if (r == j) x = x * rd;
-                const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-                const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32 + j));
-                const float xj = lane < 32 ? x0 : x1;
-                if (r > j) x = fmaf(-dl[j], xj, x);
+            for (int e = 0; e < 4; ++e) {
+                const int k = h * 4 + e;
+                const bool ok = R < n && k < a.nrhs;
+                const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + R : z;
+                const float b = *q;
+                r[e] = ok ? b - sum[e] : 0.f;
             }
-        } else {
+            *(f32x4*)&rs[i][h * 4] = f32x4{r[0], r[1], r[2], r[3]};
+        }
+        __syncthreads();
+        float y[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int j = 31; j >= 0; --j) {
-                const float x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-                const float x1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32 + j));
-                const float xj = lane < 32 ? x0 : x1;
-                if (r < j) x = fmaf(-dl[j], xj, x);
+        for (int c = 0; c < TV_H; ++c) {
+            const f32x4 r0 = *(const f32x4*)&rs[h * TV_H + c][0];
+            const f32x4 r1 = *(const f32x4*)&rs[h * TV_H + c][4];
+            y[0] = fmaf(dv[c], r0.x, y[0]);
+            y[1] = fmaf(dv[c], r0.y, y[1]);
+            y[2] = fmaf(dv[c], r0.z, y[2]);
+            y[3] = fmaf(dv[c], r0.w, y[3]);
+            y[4] = fmaf(dv[c], r1.x, y[4]);
+            y[5] = fmaf(dv[c], r1.y, y[5]);
+            y[6] = fmaf(dv[c], r1.z, y[6]);
+            y[7] = fmaf(dv[c], r1.w, y[7]);
+        }
+        *(f32x4*)&red[h][i][0] = f32x4{y[0], y[1], y[2], y[3]};   // red was last read before the barrier above
+        *(f32x4*)&red[h][i][4] = f32x4{y[4], y[5], y[6], y[7]};
+        __syncthreads();
+        {
+            const f32x4 s0 = *(const f32x4*)&red[0][i][h * 4];
+            const f32x4 s1 = *(const f32x4*)&red[1][i][h * 4];
+            const float x[4] = {s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w};
+            const bool nan = s_fail != 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k = h * 4 + e;
+                if (R < n && k < a.nrhs) z[(int64_t)k * n + R] = nan ? __int_as_float(0x7fc00000) : x[e];
             }
         }
-        if (s_fail) x = __int_as_float(0x7fc00000);
-        if (mine) z[(int64_t)k * n + R] = x;
         __threadfence();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(a.sync + 1 + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -795,39 +868,57 @@ extern "C" int uh_residual_f64(const float* a, uint64_t lda, int n, const double
     return UH_OK;
 }
 
-// Per-stream sync words of the triangular solves: solves on one stream run in stream order and
-// share them; solves on different streams get their own.
-static std::map<hipStream_t, std::pair<uint32_t*, size_t>> g_trsv_sync;
+// Per-stream workspace of the triangular solves (sync words, inverted diagonal blocks): solves
+// on one stream run in stream order and share it; solves on different streams get their own.
+struct TrsvWorkspace {
+    uint32_t* sync = nullptr;
+    size_t words = 0;
+    float* inv = nullptr;
+    size_t inv_floats = 0;
+};
+static std::map<hipStream_t, TrsvWorkspace> g_trsv_ws;
 static std::mutex g_trsv_mtx;
 static constexpr int TV_SPIN_LIMIT = 1 << 22;   // polls of one flag, about a second
 
 extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, int nrhs, void* stream) {
     if (!lu || !rhs || n <= 0 || nrhs <= 0 || nrhs > 8 || ld < (uint64_t)n)
         return fail(UH_ERR_ARG, "uh_lu_solve_f32: bad arguments");
+    if (ld * 4 * TV_H + (uint64_t)n * 4 >= (1ull << 31))   // 64 columns of a tile under 2 GiB
+        return fail(UH_ERR_ARG, "uh_lu_solve_f32: leading dimension too large");
     int rc;
     hipStream_t s = plain_stream(stream, &rc);
     if (rc) return rc;
-    const int nb = (n + 31) / 32;
+    const int nb = (n + TV_B - 1) / TV_B;
     const size_t words = 2 * (size_t)(nb + 1);   // forward then backward
+    const size_t inv_floats = (size_t)nb * 2 * TV_B * TV_B;
     uint32_t* sync;
+    float* inv;
     {
         std::lock_guard<std::mutex> lk(g_trsv_mtx);
-        auto& e = g_trsv_sync[s];
-        if (e.second < words) {
-            if (e.first) { UH_HIP(hipStreamSynchronize(s)); UH_HIP(hipFree(e.first)); e.first = nullptr; e.second = 0; }
-            UH_HIP(hipMalloc(&e.first, words * sizeof(uint32_t)));
-            e.second = words;
+        auto& e = g_trsv_ws[s];
+        if (e.words < words || e.inv_floats < inv_floats) {
+            UH_HIP(hipStreamSynchronize(s));
+            if (e.sync) UH_HIP(hipFree(e.sync));
+            if (e.inv) UH_HIP(hipFree(e.inv));
+            e = TrsvWorkspace{};
+            UH_HIP(hipMalloc(&e.sync, words * sizeof(uint32_t)));
+            e.words = words;
+            UH_HIP(hipMalloc(&e.inv, inv_floats * sizeof(float)));
+            e.inv_floats = inv_floats;
         }
-        sync = e.first;
+        sync = e.sync;
+        inv = e.inv;
     }
+    UH_LAUNCH(k_trsv_diag_inv, dim3((unsigned)nb, 2), dim3(TV_B), 0, s, lu, ld, n, inv);
+    UH_LAUNCH_CHECK("k_trsv_diag_inv");
     rc = uh_fill_u32(sync, 0u, words, s);
     if (rc) return rc;
     for (int upper = 0; upper < 2; ++upper) {
-        TrsvArgs a{lu, ld, rhs, sync + upper * (nb + 1), n, nrhs, upper, TV_SPIN_LIMIT};
-        UH_LAUNCH(k_trsv_sync_free, dim3((unsigned)nb), dim3(256), 0, s, a);
-        UH_LAUNCH_CHECK("k_trsv_sync_free");
-        g_plain_launches++;
+        TrsvArgs a{lu, ld, rhs, inv, sync + upper * (nb + 1), n, nrhs, upper, TV_SPIN_LIMIT};
+        UH_LAUNCH(k_trsv_blocked, dim3((unsigned)nb), dim3(256), 0, s, a);
+        UH_LAUNCH_CHECK("k_trsv_blocked");
     }
+    g_plain_launches += 3;
     return UH_OK;
 }
 

@@ -527,6 +527,17 @@ pmStatus pmResetHipStubStats() {
     r->stats = pmHipStubStats{};
     return pmSuccess;
 }
+pmStatus pmReadMemoryRange(pmMemHandle h, size_t off, size_t len, void* dst) {
+    PM_API_BEGIN
+    need_runtime();
+    AddressSpace* as = (AddressSpace*)h;
+    if (!as) raise(pmUnrecognizedMemory, "null memory handle");
+    if (off > as->length || len > as->length - off) raise(pmInvalidOffset, "range beyond address space");
+    if (len && !dst) raise(pmInvalidKey, "null destination");
+    if (len) Memory::get().read_range(as, off, len, dst);
+    PM_API_END
+}
+
 pmStatus pmPrefetchToDevice(pmMemHandle h) {
     PM_API_BEGIN
     Runtime* r = need_runtime();

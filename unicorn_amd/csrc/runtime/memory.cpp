@@ -167,6 +167,18 @@ void Memory::sync_host(AddressSpace* as) {
     as->host_valid = true;
 }
 
+void Memory::read_range(AddressSpace* as, size_t off, size_t len, void* dst) {
+    std::lock_guard<std::mutex> g(as->mtx);
+    if (as->host_valid || !as->dev || !as->dev_valid) {
+        memcpy(dst, as->host + off, len);
+        return;
+    }
+    if (uh_memcpy_d2h(dst, as->dev + off, len, nullptr) || uh_stream_sync(nullptr))
+        raise(pmGraphicsCardError, std::string("read of address space range: ") + uh_last_error());
+    Runtime* r = Runtime::get();
+    if (r) { std::lock_guard<std::mutex> s(r->stats_mtx); r->stats.d2h_bytes += len; }
+}
+
 bool Memory::ensure_device(AddressSpace* as, void* stream) {
     std::lock_guard<std::mutex> g(as->mtx);
     as->last_use = ++clock_;

@@ -106,6 +106,8 @@ public:
     void release(AddressSpace* as);
     // Make the host copy current (download from HBM if the mirror is newer).
     void sync_host(AddressSpace* as);
+    // copies [off, off + len) of the newest copy to dst; validity unchanged
+    void read_range(AddressSpace* as, size_t off, size_t len, void* dst);
     // Make the HBM mirror current (allocate within the budget, upload if the host is newer).
     bool ensure_device(AddressSpace* as, void* stream);   // true when it enqueued an upload
     void mark_device_written(AddressSpace* as);
