@@ -520,9 +520,12 @@ struct TrsvArgs {
 
 // Inverse of diagonal block p of the lower non-unit (blockIdx.y = 0) or the upper unit (1)
 // triangle of the packed factors; rows and columns past n are taken as the identity. Thread j
-// computes column j by substitution against the block held in LDS.
+// computes column j, row by row: every lane runs the same trip count (the terms against the
+// zero part of its column add nothing), reads row i of the block as 4-wide broadcasts and keeps
+// four independent partial sums.
 __global__ void __launch_bounds__(TV_B) k_trsv_diag_inv(const float* lu, uint64_t ld, int n, float* inv) {
-    __shared__ float T[TV_B][TV_B + 1];
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    __shared__ __attribute__((aligned(16))) float T[TV_B][TV_B + 4];
     __shared__ float X[TV_B][TV_B + 1];
     const int p = blockIdx.x, upper = blockIdx.y, j = threadIdx.x;
     const int b0 = p * TV_B;
@@ -535,17 +538,30 @@ __global__ void __launch_bounds__(TV_B) k_trsv_diag_inv(const float* lu, uint64_
         T[j][c] = in ? v : (c == j ? 1.f : 0.f);
     }
     __syncthreads();
+    auto dot = [&](int i, int k0, int k1) {   // sum over k in [k0, k1) of T[i][k] X[k][j], k0 % 4 == 0
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            const f32x4 t = *(const f32x4*)&T[i][k];
+            s0 = fmaf(t.x, X[k][j], s0);
+            s1 = fmaf(t.y, X[k + 1][j], s1);
+            s2 = fmaf(t.z, X[k + 2][j], s2);
+            s3 = fmaf(t.w, X[k + 3][j], s3);
+        }
+        for (; k < k1; ++k) s0 = fmaf(T[i][k], X[k][j], s0);
+        return (s0 + s1) + (s2 + s3);
+    };
     if (!upper) {
-        for (int i = 0; i < TV_B; ++i) {      // L X = I, rows top down; X[i][j] = 0 for i < j
-            float s = i == j ? 1.f : 0.f;
-            for (int k = j; k < i; ++k) s = fmaf(-T[i][k], X[k][j], s);
+        for (int i = 0; i < TV_B; ++i) {      // L X = I, rows top down
+            const float s = (i == j ? 1.f : 0.f) - dot(i, 0, i);
             X[i][j] = s / T[i][i];
         }
     } else {
-        for (int i = TV_B - 1; i >= 0; --i) { // U X = I, rows bottom up; X[i][j] = 0 for i > j
-            float s = i == j ? 1.f : 0.f;
-            for (int k = i + 1; k <= j; ++k) s = fmaf(-T[i][k], X[k][j], s);
-            X[i][j] = s;
+        for (int i = TV_B - 1; i >= 0; --i) { // U X = I, rows bottom up (unit diagonal)
+            const int k0 = (i + 1 + 3) & ~3;  // the terms below k0 are summed one by one
+            float s = 0.f;
+            for (int k = i + 1; k < k0 && k < TV_B; ++k) s = fmaf(T[i][k], X[k][j], s);
+            X[i][j] = (i == j ? 1.f : 0.f) - (s + dot(i, k0 < TV_B ? k0 : TV_B, TV_B));
         }
     }
     __syncthreads();
@@ -601,6 +617,17 @@ __global__ void __launch_bounds__(256) k_trsv_blocked(TrsvArgs a) {
 #pragma unroll
             for (int c = 0; c < TV_H; ++c)
                 dv[c] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_i, (uint32_t)i * 4u, c * TV_B * 4, 0));
+        }
+        // this block's own right-hand side (thread (row i, vectors 4h .. 4h+3)), loaded before
+        // any wait: it depends on no other block
+        float bv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int k = h * 4 + e;
+            const bool ok = R < n && k < a.nrhs;
+            const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + R : z;
+            const float b = *q;
+            bv[e] = ok ? b : 0.f;
         }
         float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         // tile d: wait for its block, stage that block's solution, accumulate M(R, .) x
@@ -666,13 +693,7 @@ __global__ void __launch_bounds__(256) k_trsv_blocked(TrsvArgs a) {
             const float sum[4] = {s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w};
             float r[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int k = h * 4 + e;
-                const bool ok = R < n && k < a.nrhs;
-                const UH_GLOBAL float* q = ok ? z + (int64_t)k * n + R : z;
-                const float b = *q;
-                r[e] = ok ? b - sum[e] : 0.f;
-            }
+            for (int e = 0; e < 4; ++e) r[e] = R < n ? bv[e] - sum[e] : 0.f;
             *(f32x4*)&rs[i][h * 4] = f32x4{r[0], r[1], r[2], r[3]};
         }
         __syncthreads();
@@ -704,7 +725,8 @@ __global__ void __launch_bounds__(256) k_trsv_blocked(TrsvArgs a) {
                 if (R < n && k < a.nrhs) z[(int64_t)k * n + R] = nan ? __int_as_float(0x7fc00000) : x[e];
             }
         }
-        __threadfence();
+        // The barrier orders every thread's stores before thread 0's flag store, whose
+        // agent-scope release makes them visible to the other XCDs (release is cumulative).
         __syncthreads();
         if (tid == 0) __hip_atomic_store(a.sync + 1 + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
