@@ -185,6 +185,8 @@ int uh_residual_f64(const float* a, uint64_t lda, int n, const double* x, const 
                     int nrhs, double* r, float* rf, void* stream);
 /* Solve (L U) z = rhs in place for nrhs columns (fp32), with the factors of uh_lu_panel. */
 int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, int nrhs, void* stream);
+/* Start of a refinement: x = 0, r = b, rf = (float) b (n x nrhs each, column-major). */
+int uh_refine_init(const double* b, double* x, double* r, float* rf, int n, int nrhs, void* stream);
 /* x += z (z fp32, x fp64), and norms: out[0..nrhs) = max|r|, out[nrhs..2nrhs) = max|x|. */
 int uh_refine_update(double* x, const float* z, int n, int nrhs, void* stream);
 int uh_colmax_abs_f64(const double* v, int n, int ncols, double* out, void* stream);

@@ -248,6 +248,8 @@ static pmStatus solveDataDistribution(pmTaskInfo ti, pmDeviceInfo di, pmSubtaskI
         pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, 1, READ_SUBSCRIPTION,
                             pmSubscriptionInfo(0, N * N * 4));
     }
+    // REFINE_INIT: x = 0, r = b, rf = (float) b. REFINE_STEP: solve L U z = rf, x += z, then
+    // r = b - A x (also as rf, the next step's right-hand side) and the norms.
     pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, 2, READ_WRITE_SUBSCRIPTION,
                         pmSubscriptionInfo(0, L.bytes()));
     return pmSuccess;
@@ -263,18 +265,17 @@ static pmStatus solve_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* s
     double* r = (double*)(st + L.r());
     float* rf = (float*)(st + L.rf());
     if (c->taskStep == REFINE_INIT) {
-        // x = 0
-        if (uh_fill_u32(x, 0u, (size_t)N * k * 2, stream)) return gpu_error("fill");
+        if (uh_refine_init(b, x, r, rf, N, k, stream)) return gpu_error("uh_refine_init");
         return pmSuccess;
     }
     const float* lu = (const float*)si.memInfo[0].ptr;
     const float* a = (const float*)si.memInfo[1].ptr;
     double* norms = (double*)(st + L.norms());
-    if (uh_residual_f64(a, (uint64_t)N, N, x, b, k, r, rf, stream)) return gpu_error("uh_residual_f64");
     if (uh_lu_solve_f32(lu, (uint64_t)N, N, rf, k, stream)) return gpu_error("uh_lu_solve_f32");
     if (uh_refine_update(x, rf, N, k, stream)) return gpu_error("uh_refine_update");
-    // residual of the updated x, and the norms the host uses to decide convergence
-    if (uh_residual_f64(a, (uint64_t)N, N, x, b, k, r, nullptr, stream)) return gpu_error("uh_residual_f64");
+    // residual of the updated x (the next step's right-hand side), and the norms the host uses
+    // to decide convergence
+    if (uh_residual_f64(a, (uint64_t)N, N, x, b, k, r, rf, stream)) return gpu_error("uh_residual_f64");
     if (uh_colmax_abs_f64(r, N, k, norms, stream)) return gpu_error("norms");
     if (uh_colmax_abs_f64(x, N, k, norms + 8, stream)) return gpu_error("norms");
     return pmSuccess;
@@ -291,6 +292,10 @@ static pmStatus solve_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
     float* rf = (float*)(st + L.rf());
     if (c->taskStep == REFINE_INIT) {
         memset(x, 0, N * k * 8);
+        for (size_t i = 0; i < N * k; ++i) {
+            r[i] = b[i];
+            rf[i] = (float)b[i];
+        }
         return pmSuccess;
     }
     const float* lu = (const float*)si.memInfo[0].ptr;
@@ -303,12 +308,11 @@ static pmStatus solve_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
                 const double xj = x[q * N + j];
                 for (size_t i = 0; i < N; ++i) r[q * N + i] -= (double)a[i + j * N] * xj;
             }
+            for (size_t i = 0; i < N; ++i) rf[q * N + i] = (float)r[q * N + i];
         }
     };
-    residual();
     for (size_t q = 0; q < k; ++q) {
-        float* z = rf + q * N;
-        for (size_t i = 0; i < N; ++i) z[i] = (float)r[q * N + i];
+        float* z = rf + q * N;      // r of the previous step (or of x = 0), in fp32
         for (size_t i = 0; i < N; ++i) {
             float s = z[i];
             for (size_t j = 0; j < i; ++j) s -= lu[i + j * N] * z[j];

@@ -944,6 +944,28 @@ extern "C" int uh_lu_solve_f32(const float* lu, uint64_t ld, int n, float* rhs, 
     return UH_OK;
 }
 
+__global__ void __launch_bounds__(256) k_refine_init(const double* b, double* x, double* r, float* rf, uint64_t tot) {
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= tot) return;
+    const double v = b[e];
+    x[e] = 0.0;
+    r[e] = v;
+    rf[e] = (float)v;
+}
+
+extern "C" int uh_refine_init(const double* b, double* x, double* r, float* rf, int n, int nrhs, void* stream) {
+    if (!b || !x || !r || !rf || n < 0 || nrhs < 0) return fail(UH_ERR_ARG, "uh_refine_init: bad arguments");
+    const uint64_t tot = (uint64_t)n * nrhs;
+    if (!tot) return UH_OK;
+    int rc;
+    hipStream_t s = plain_stream(stream, &rc);
+    if (rc) return rc;
+    UH_LAUNCH(k_refine_init, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, b, x, r, rf, tot);
+    UH_LAUNCH_CHECK("k_refine_init");
+    g_plain_launches++;
+    return UH_OK;
+}
+
 extern "C" int uh_refine_update(double* x, const float* z, int n, int nrhs, void* stream) {
     int rc;
     hipStream_t s = plain_stream(stream, &rc);
