@@ -253,7 +253,7 @@ def bench_pagerank(args, U):
         "p50_ms": round(1e3 * percentile(lat, 50), 3), "p99_ms": round(1e3 * percentile(lat, 99), 3),
         "per_subtask_launch": comps, "rel_l1_vs_oracle": err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("pagerank_10m"),
                      "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent",
                      "kernel_ms": round(kernel_ms, 4),
                      "algorithmic_bytes_per_launch": it_bytes},

@@ -107,6 +107,7 @@ static constexpr uint32_t PR_NB = UH_PR_BLOCKS;
 static constexpr int PR_BCH = 4096;                          // links staged per chunk
 static constexpr int PR_RPL = PR_GR / PR_THREADS;            // consecutive rows per lane
 static_assert(PR_RPL == 8 && PR_NB == 8, "a lane owns 8 rows; 8 source blocks");
+static constexpr int PR_NT = 2;                              // buffer cache policy: non-temporal
 
 struct PrBlkShared {
     float val[PR_BCH];                      // w[col[k]] of the current chunk
@@ -124,8 +125,12 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
                                               float* part_, PrBlkShared& sm) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t u = g * PR_NB + x;
-    const UH_GLOBAL uint32_t* cw = (const UH_GLOBAL uint32_t*)(a.bcnt + (size_t)u * PR_GR);
-    const uint32_t c03 = cw[2 * tid], c47 = cw[2 * tid + 1];
+    // The counts and link ids are read once: non-temporal loads (and stores of the partial
+    // sums) leave the L2 to this XCD's block of w, which the gathers reuse.
+    const __amdgpu_buffer_rsrc_t cnt_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(a.bcnt + (size_t)u * PR_GR), 0, PR_GR, 0x00020000);
+    const uint32_t c03 = __builtin_amdgcn_raw_buffer_load_b32(cnt_rsrc, 8 * tid, 0, PR_NT);
+    const uint32_t c47 = __builtin_amdgcn_raw_buffer_load_b32(cnt_rsrc, 8 * tid + 4, 0, PR_NT);
     // row j's count, unpacked where it is used (keeps 6 registers free)
     auto cnt = [&](int j) { return ((j < 4 ? c03 : c47) >> (8 * (j & 3))) & 255u; };
     uint32_t tot = 0;
@@ -159,7 +164,7 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
         uint32_t src[PR_BCH / PR_THREADS];
 #pragma unroll
         for (int j = 0; j < PR_BCH / PR_THREADS; ++j)
-            src[j] = __builtin_amdgcn_raw_buffer_load_b32(c_rsrc, tid * 4u, j * PR_THREADS * 4, 0);
+            src[j] = __builtin_amdgcn_raw_buffer_load_b32(c_rsrc, tid * 4u, j * PR_THREADS * 4, PR_NT);
 #pragma unroll
         for (int j = 0; j < PR_BCH / PR_THREADS; ++j)
             sm.val[tid + j * PR_THREADS] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(w_rsrc, src[j] * 4u, 0, 0));
@@ -175,10 +180,9 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
         }
         __syncthreads();          // the chunk buffer is refilled next
     }
-    typedef float f32x4 __attribute__((ext_vector_type(4)));
-    UH_GLOBAL f32x4* part = (UH_GLOBAL f32x4*)as_global(part_ + (size_t)PR_RPL * tid);
-    part[0] = f32x4{acc[0], acc[1], acc[2], acc[3]};
-    part[1] = f32x4{acc[4], acc[5], acc[6], acc[7]};
+    UH_GLOBAL float* part = as_global(part_ + (size_t)PR_RPL * tid);
+#pragma unroll
+    for (int j = 0; j < PR_RPL; ++j) __builtin_nontemporal_store(acc[j], part + j);
     __syncthreads();              // wsum is rewritten by the next unit
 }
 

@@ -44,6 +44,9 @@ static double spin_secs() {
     return s;
 }
 
+// Subtasks of the first BATCHED flush onto an idle device (later flushes grow from it).
+static constexpr unsigned long kFirstBatch = 128;
+
 static inline void cpu_relax() {
 #if defined(__x86_64__) || defined(__i386__)
     __builtin_ia32_pause();
@@ -871,6 +874,16 @@ void Runtime::gpu_loop() {
         std::vector<std::unique_ptr<GpuSlot>> slots;
     };
     std::deque<BatchRun> batches;
+    // Subtasks one BATCHED flush may take. Onto an idle device the first flush is small, so the
+    // kernels start while the host stages the rest; each later flush may take twice the one in
+    // flight (staging a subtask costs the host less than the device spends on it, so the device
+    // stays busy), up to the batch limit.
+    size_t last_batch = 0;
+    auto batch_cap = [&]() -> ulong {
+        const ulong lim = (ulong)batch_limit.load();
+        const ulong grow = batches.empty() ? kFirstBatch : std::max<ulong>(kFirstBatch, 2 * (ulong)last_batch);
+        return std::min(lim, grow);
+    };
     void* batch_events[4];
     for (int i = 0; i < 4; ++i) uh_event_create(&batch_events[i]);
     int batch_ev = 0;
@@ -1012,7 +1025,7 @@ void Runtime::gpu_loop() {
             }
             // the mode is the one the task was submitted under
             mode = front()->hip_mode;
-            ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? (ulong)batch_limit.load() : 1;
+            ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? batch_cap() : 1;
             t = pick(true, first, count, max_take);
             if (!t) continue;
         }
@@ -1023,9 +1036,10 @@ void Runtime::gpu_loop() {
             std::vector<std::pair<Task*, std::pair<ulong, ulong>>> claims;
             claims.push_back({t, {first, count}});
             ulong taken = count;
+            const ulong cap = batch_cap();
             {
                 std::lock_guard<std::mutex> g(q_mtx);
-                while (taken < (ulong)batch_limit.load()) {
+                while (taken < cap) {
                     ulong f2 = 0, c2 = 0;
                     Task* t2 = nullptr;
                     // only tasks that can be batched (GPU-resident)
@@ -1034,7 +1048,7 @@ void Runtime::gpu_loop() {
                             x->next.load() < x->subtask_count) { t2 = x; break; }
                     }
                     if (!t2) break;
-                    Task* got = pick(true, f2, c2, (ulong)batch_limit.load() - taken);
+                    Task* got = pick(true, f2, c2, cap - taken);
                     if (!got) break;
                     claims.push_back({got, {f2, c2}});
                     taken += c2;
@@ -1085,6 +1099,7 @@ void Runtime::gpu_loop() {
             for (auto& sp : run.slots)
                 for (uint i = 0; i < sp->task->spaces.size(); ++i)
                     if (sp->task->writable(i) && !sp->task->is_reduced(i)) Memory::get().mark_device_written(sp->task->spaces[i]);
+            last_batch = run.slots.size();
             batches.push_back(std::move(run));
             continue;
         }
