@@ -105,6 +105,17 @@ int ua_pr_build_graph(uint32_t pages, const uint32_t* out_row_ptr, const uint32_
                       const uint8_t* bcnt, uint32_t* graph);
 void ua_pr_set_layout(int blocked);
 int ua_pr_blocked(void* h);
+/* The reference's on-disk web dump (pageRank.cpp:53-197, scripts/webGenerator.pl:121-274):
+ * <dir>/web_dump_metadata = {pages, max out-links, pages per file} (uint32); <dir>/web/page_<n>
+ * = fixed records {count, count 1-based targets, zero padding to max out-links}, file n holding
+ * pages n, n+1, ... ua_webdump_read works like ua_webgen (col == NULL: row_ptr and *edges; then
+ * col, 0-based); ua_webdump_write writes an out-link CSR in that format;
+ * ua_pr_create_from_webdump builds a pageRank instance from a dump. */
+int ua_webdump_meta(const char* dir, uint32_t* pages, uint32_t* max_out, uint32_t* per_file);
+int ua_webdump_read(const char* dir, uint32_t* row_ptr, uint32_t* col, uint64_t col_capacity, uint64_t* edges);
+int ua_webdump_write(const char* dir, uint32_t pages, uint32_t max_out, uint32_t per_file,
+                     const uint32_t* row_ptr, const uint32_t* col);
+void* ua_pr_create_from_webdump(const char* dir, uint32_t pages_per_subtask, float damping);
 
 /* ---- luDecomposition (reference testSuite/luDecomposition) + refinement solve ----
  * n x n float32 column-major, blocks of `block` (<= 512); precision 0 = split-BF16 trailing

@@ -144,3 +144,21 @@ def test_pagerank_csr_fallback_for_heavy_rows(gpu_runtime):
     finally:
         pr.close()
     assert _rel_l1(y, O.pagerank(rp, col, 2, 3000)) <= 1e-5
+
+
+@pytest.mark.parametrize("hip_mode", [0, 2])
+def test_pagerank_from_webdump(gpu_runtime, hip_mode, tmp_path):
+    """A graph written in the reference's on-disk web dump format and loaded back (pmMapFile'd
+    records → resident graph) gives the oracle's ranks."""
+    U = gpu_runtime
+    U.set_hip_mode(hip_mode)
+    rp, col = U.webgen(150_000, 12)
+    U.webdump_write(tmp_path, rp, col, 100, 40_000)
+    pr = U.PageRank.from_webdump(tmp_path, 50_000)
+    try:
+        assert pr.blocked
+        pr.run(10, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 10, 50_000)) <= 1e-5

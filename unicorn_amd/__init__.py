@@ -103,6 +103,10 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_pr_graph_bytes": (C.c_uint64, [C.c_uint32, C.c_uint64, i32]),
         "ua_pr_build_graph": (i32, [C.c_uint32, vp, vp, vp, vp]),
         "ua_pr_set_layout": (None, [i32]), "ua_pr_blocked": (i32, [vp]),
+        "ua_webdump_meta": (i32, [C.c_char_p, vp, vp, vp]),
+        "ua_webdump_read": (i32, [C.c_char_p, vp, vp, C.c_uint64, vp]),
+        "ua_webdump_write": (i32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]),
+        "ua_pr_create_from_webdump": (vp, [C.c_char_p, C.c_uint32, C.c_float]),
         "ua_lu_create": (vp, [i32, i32, i32]), "ua_lu_set_matrix": (i32, [vp, vp]),
         "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
         "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
@@ -260,6 +264,19 @@ class PageRank:
         if not self._h:
             _check(-1, "ua_pr_create")
 
+    @classmethod
+    def from_webdump(cls, path: str, pages_per_subtask: int, damping: float = 0.85) -> "PageRank":
+        """pageRank over a web dump in the reference's on-disk format (see webdump_write)."""
+        init()
+        self = cls.__new__(cls)
+        pages, max_out, per_file = webdump_meta(path)
+        self.pages = pages
+        self.row_ptr = self.col = None
+        self._h = apps_lib().ua_pr_create_from_webdump(str(path).encode(), pages_per_subtask, damping)
+        if not self._h:
+            _check(-1, "ua_pr_create_from_webdump")
+        return self
+
     @property
     def blocked(self) -> bool:
         """True if the graph is held in the source-blocked layout."""
@@ -285,6 +302,35 @@ class PageRank:
             self.close()
         except Exception:
             pass
+
+
+def webdump_meta(path: str):
+    """(pages, max out-links per page, pages per file) of a web dump."""
+    v = [C.c_uint32(0) for _ in range(3)]
+    _check(apps_lib().ua_webdump_meta(str(path).encode(), *[C.byref(x) for x in v]), "ua_webdump_meta")
+    return tuple(x.value for x in v)
+
+
+def webdump_read(path: str):
+    """The out-link CSR (row_ptr, col; 0-based targets) of a web dump in the reference's format."""
+    pages, _, _ = webdump_meta(path)
+    lib = apps_lib()
+    row_ptr = np.empty(pages + 1, dtype=np.uint32)
+    edges = C.c_uint64(0)
+    _check(lib.ua_webdump_read(str(path).encode(), _ptr(row_ptr), None, 0, C.byref(edges)), "ua_webdump_read")
+    col = np.empty(max(1, edges.value), dtype=np.uint32)
+    _check(lib.ua_webdump_read(str(path).encode(), _ptr(row_ptr), _ptr(col), col.size, C.byref(edges)),
+           "ua_webdump_read")
+    return row_ptr, col[:edges.value]
+
+
+def webdump_write(path: str, row_ptr: np.ndarray, col: np.ndarray, max_out: int, pages_per_file: int) -> None:
+    """Writes an out-link CSR as a web dump in the reference's format: <path>/web_dump_metadata
+    and <path>/web/page_<n> files of fixed records (count, 1-based targets, zero padding)."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
+    _check(apps_lib().ua_webdump_write(str(path).encode(), row_ptr.size - 1, max_out, pages_per_file,
+                                       _ptr(row_ptr), _ptr(col)), "ua_webdump_write")
 
 
 def pr_set_layout(blocked: bool) -> None:
