@@ -233,6 +233,16 @@ def bench_pagerank(args, U):
     U.set_kernel_timing(False)
     comps["csr_layout"] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms / max(1, cfl), 4)}
     prc.close()
+    # the reference's decomposition: source-page subtasks push into private full-length vectors
+    # that REDUCE_ADD folds in HBM (records padded to the largest out-degree, 101 uint32 per page)
+    prp = U.PageRankPush(rp, col, per)
+    prp.run(1, U.DEVICE_GPU)
+    U.device_sync()
+    e, l = time_tasks(lambda: prp.run(iters, U.DEVICE_GPU), 1)
+    yp = prp.ranks()
+    prp.close()
+    comps["push_reduce_add"] = {"tasks_per_s": round(1 / e, 4),
+                                "rel_l1_vs_pull": float(np.abs(yp.astype(np.float64) - y).sum() / np.abs(y).sum())}
     it_bytes = 12 * pages + 4 * edges
     achieved = it_bytes / (kernel_ms * 1e-3) / 1e9
     cpu, err = None, None

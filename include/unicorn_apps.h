@@ -116,6 +116,17 @@ int ua_webdump_read(const char* dir, uint32_t* row_ptr, uint32_t* col, uint64_t 
 int ua_webdump_write(const char* dir, uint32_t pages, uint32_t max_out, uint32_t per_file,
                      const uint32_t* row_ptr, const uint32_t* col);
 void* ua_pr_create_from_webdump(const char* dir, uint32_t pages_per_subtask, float damping);
+/* pageRank in the reference's decomposition (pageRank.cpp:237-378, 449-560): subtasks own
+ * source pages and push into private full-length vectors folded by the built-in REDUCE_ADD
+ * (in HBM on the HIP stub's resident modes). Records address space = the web dump's records;
+ * ua_prp_create pads an out-link CSR to its largest out-degree, ua_prp_create_from_webdump copies
+ * the dump's page files. ua_prp_run runs `iterations` iteration tasks from x0 = 1.0. */
+void* ua_prp_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                    uint32_t pages_per_subtask, float damping);
+void* ua_prp_create_from_webdump(const char* dir, uint32_t pages_per_subtask, float damping);
+int ua_prp_run(void* h, int iterations, int device, double* seconds);
+int ua_prp_get_ranks(void* h, float* ranks);
+int ua_prp_destroy(void* h);
 
 /* ---- luDecomposition (reference testSuite/luDecomposition) + refinement solve ----
  * n x n float32 column-major, blocks of `block` (<= 512); precision 0 = split-BF16 trailing

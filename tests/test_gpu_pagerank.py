@@ -162,3 +162,34 @@ def test_pagerank_from_webdump(gpu_runtime, hip_mode, tmp_path):
     finally:
         pr.close()
     assert _rel_l1(y, O.pagerank(rp, col, 10, 50_000)) <= 1e-5
+
+
+@pytest.mark.parametrize("hip_mode", [0, 1, 2])
+def test_pagerank_push_reference_decomposition(gpu_runtime, hip_mode):
+    """The reference's decomposition on the GPU: source-page subtasks push (float atomics) into
+    private full-length vectors; REDUCE_ADD folds them in HBM (resident modes) or on the host
+    after their D2H (PLAIN)."""
+    U = gpu_runtime
+    U.set_hip_mode(hip_mode)
+    rp, col = U.webgen(120_000, 14)
+    pr = U.PageRankPush(rp, col, 25_000)
+    try:
+        pr.run(8, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 8, 25_000)) <= 1e-5
+
+
+def test_pagerank_push_from_webdump(gpu_runtime, tmp_path):
+    U = gpu_runtime
+    U.set_hip_mode(2)
+    rp, col = U.webgen(80_000, 15)
+    U.webdump_write(tmp_path, rp, col, 100, 20_000)
+    pr = U.PageRankPush.from_webdump(tmp_path, 20_000)
+    try:
+        pr.run(6, U.DEVICE_GPU)
+        y = pr.ranks()
+    finally:
+        pr.close()
+    assert _rel_l1(y, O.pagerank(rp, col, 6, 20_000)) <= 1e-5

@@ -107,6 +107,10 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_webdump_read": (i32, [C.c_char_p, vp, vp, C.c_uint64, vp]),
         "ua_webdump_write": (i32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]),
         "ua_pr_create_from_webdump": (vp, [C.c_char_p, C.c_uint32, C.c_float]),
+        "ua_prp_create": (vp, [C.c_uint32, vp, vp, C.c_uint32, C.c_float]),
+        "ua_prp_create_from_webdump": (vp, [C.c_char_p, C.c_uint32, C.c_float]),
+        "ua_prp_run": (i32, [vp, i32, i32, C.POINTER(dbl)]),
+        "ua_prp_get_ranks": (i32, [vp, vp]), "ua_prp_destroy": (i32, [vp]),
         "ua_lu_create": (vp, [i32, i32, i32]), "ua_lu_set_matrix": (i32, [vp, vp]),
         "ua_lu_factor": (i32, [vp, i32, C.POINTER(dbl)]), "ua_lu_get_factors": (i32, [vp, vp]),
         "ua_lu_solve": (i32, [vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
@@ -295,6 +299,52 @@ class PageRank:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_pr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PageRankPush:
+    """pageRank in the reference's decomposition: source-page subtasks push into private
+    full-length vectors that the built-in REDUCE_ADD folds (on the device in the HIP stub's
+    resident modes). Build from an out-link CSR or, with from_webdump, from a web dump."""
+
+    def __init__(self, row_ptr: np.ndarray, col: np.ndarray, pages_per_subtask: int, damping: float = 0.85):
+        init()
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
+        self.pages = row_ptr.size - 1
+        self._h = apps_lib().ua_prp_create(self.pages, _ptr(row_ptr), _ptr(col), pages_per_subtask, damping)
+        if not self._h:
+            _check(-1, "ua_prp_create")
+
+    @classmethod
+    def from_webdump(cls, path: str, pages_per_subtask: int, damping: float = 0.85) -> "PageRankPush":
+        init()
+        self = cls.__new__(cls)
+        self.pages = webdump_meta(path)[0]
+        self._h = apps_lib().ua_prp_create_from_webdump(str(path).encode(), pages_per_subtask, damping)
+        if not self._h:
+            _check(-1, "ua_prp_create_from_webdump")
+        return self
+
+    def run(self, iterations: int, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_prp_run(self._h, iterations, device, C.byref(t)), "pageRank push tasks")
+        return t.value
+
+    def ranks(self) -> np.ndarray:
+        out = np.empty(self.pages, dtype=np.float32)
+        _check(apps_lib().ua_prp_get_ranks(self._h, _ptr(out)), "ua_prp_get_ranks")
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            apps_lib().ua_prp_destroy(self._h)
             self._h = None
 
     def __del__(self):
