@@ -341,6 +341,8 @@ __global__ void __launch_bounds__(G_THREADS) k_gemm(uh_gemm_args g, int cfg) {
 template <int V>
 __global__ void __launch_bounds__(G_THREADS) k_gemm_persistent(const uint8_t* queue) {
     __shared__ short lds[VariantLds<V>::SHORTS];
+    // the claim is passed through the tile buffer itself: configuration 4 uses exactly half
+    // of a CU's LDS, and any further shared word would leave room for one workgroup, not two
     uint32_t* s_tile = (uint32_t*)lds;
     QueueHeader* h = (QueueHeader*)queue;
     const QueueHeader hd = uniform_load(h);
