@@ -182,7 +182,8 @@ def bench_matmul(args, U):
         "per_subtask_launch": comps, "fp64_normwise_rel_err": err,
         "fp32_equivalent_tflops": round(flop / (kernel_ms * 1e-3) / 1e12, 1),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(achieved / 2500.0, 4), "traffic": None, "kernel": "k_gemm_persistent",
+                     "frac": round(achieved / 2500.0, 4), "traffic": traffic_from_profiles("matmul_2048"),
+                     "kernel": "k_gemm_persistent",
                      "kernel_ms": round(kernel_ms, 4), "algorithmic_flop_per_launch": issued},
         "cpu_baseline": cpu,
     }
