@@ -349,19 +349,28 @@ __global__ void __launch_bounds__(G_THREADS) k_gemm_persistent(const uint8_t* qu
     const uh_gemm_args* items = (const uh_gemm_args*)(queue + hd.items_offset);
     const uint32_t* prefix = (const uint32_t*)(queue + hd.prefix_offset);
     const uint32_t total = hd.total_tiles, n_items = hd.n_items;
-    for (;;) {
-        if (threadIdx.x == 0) *s_tile = atomicAdd(&h->next_tile, 1u);
-        __syncthreads();
-        const uint32_t t = __builtin_amdgcn_readfirstlane(*s_tile);
-        __syncthreads();
-        if (t >= total) break;
-        uint32_t item, local;
-        queue_locate(prefix, n_items, t, item, local);
-        uh_gemm_args g = uniform_load(items + item);
-        const int cfg = g.precision >> 8;
-        g.precision &= 0xff;
-        gemm_run<V>(g, cfg, local, lds);
-        __syncthreads();
+    // The tile range is cut into 8 contiguous parts, part x claimed through counter x (its own
+    // cache line of the header). A workgroup claims from its XCD's part first, so an XCD runs
+    // whole items and re-reads their A and B panels from its own L2; then it helps the other parts.
+    const uint32_t x0 = __builtin_amdgcn_readfirstlane(xcc_id());
+    for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t x = (x0 + i) & 7u;
+        const uint32_t lo = (uint32_t)(((uint64_t)total * x) / 8), hi = (uint32_t)(((uint64_t)total * (x + 1)) / 8);
+        for (;;) {
+            if (threadIdx.x == 0) *s_tile = atomicAdd(&h->next_blk[x * 32], 1u);
+            __syncthreads();
+            const uint32_t k = __builtin_amdgcn_readfirstlane(*s_tile);
+            __syncthreads();
+            if (k >= hi - lo) break;
+            const uint32_t t = lo + k;
+            uint32_t item, local;
+            queue_locate(prefix, n_items, t, item, local);
+            uh_gemm_args g = uniform_load(items + item);
+            const int cfg = g.precision >> 8;
+            g.precision &= 0xff;
+            gemm_run<V>(g, cfg, local, lds);
+            __syncthreads();
+        }
     }
 }
 

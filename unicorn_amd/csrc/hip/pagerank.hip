@@ -114,11 +114,6 @@ struct PrBlkShared {
     uint32_t wsum[PR_THREADS / 64];         // link totals of each wave's rows
 };
 
-// XCD of the calling wave: hardware register XCC_ID (id 20), bits [3:0].
-__device__ __forceinline__ uint32_t xcc_id() {
-    return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 7u;
-}
-
 // Unit (group g, block x): part[r] = sum of w[src] over the block-x links of row g*PR_GR + r,
 // in link order, for r in [0, PR_GR). Lane t owns rows 8t .. 8t+7, whose links are consecutive.
 __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_t g, uint32_t x,

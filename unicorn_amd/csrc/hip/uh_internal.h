@@ -185,6 +185,12 @@ __device__ inline void queue_locate(const uint32_t* prefix, uint32_t n_items, ui
     local = tile - p[lo];
 }
 
+// XCD of the calling wave: hardware register XCC_ID (id 20), bits [3:0]. Used only to choose
+// which work a workgroup claims first; every claim loop also takes the other XCDs' work.
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 7u;
+}
+
 // The same search over a copy of the prefix array in LDS (see stage_prefix).
 __device__ inline void queue_locate_lds(const uint32_t* s_prefix, uint32_t n_items, uint32_t tile,
                                         uint32_t& item, uint32_t& local) {
