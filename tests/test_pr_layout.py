@@ -9,7 +9,7 @@ block cannot be held by the u8 counts and falls back to the plain CSR.
 import numpy as np
 import pytest
 
-GR, NB = 2048, 8
+GR, NB = 2048, 8                     # UH_PR_GROUP_ROWS, UH_PR_BLOCKS
 
 
 @pytest.fixture(scope="module")
@@ -74,7 +74,7 @@ def test_blocked_layout_ragged_and_tiny(U):
 
 
 def test_blocked_layout_overflow_falls_back(U):
-    n = 3000                                # page 0 gets 2999 in-links, >255 from one block
+    n = 6000                                # page 0 gets 5999 in-links, >255 from one block
     rp = np.arange(n + 1, dtype=np.uint32)
     rp[1:] -= 1
     rp[0] = 0

@@ -389,6 +389,12 @@ def pr_set_layout(blocked: bool) -> None:
     apps_lib().ua_pr_set_layout(1 if blocked else 0)
 
 
+# Row group size and source block count of the source-blocked pageRank layout
+# (UH_PR_GROUP_ROWS and UH_PR_BLOCKS of unicorn_hip.h).
+PR_GROUP_ROWS = 2048
+PR_BLOCKS = 8
+
+
 def pr_graph_layout(row_ptr: np.ndarray, col: np.ndarray, blocked: bool = True):
     """The graph address space that PageRank builds from an out-link CSR, as numpy views:
     (in_row_ptr, outdeg, in_col, bseg, bcnt); bseg and bcnt are None for the plain CSR (or when
@@ -397,9 +403,10 @@ def pr_graph_layout(row_ptr: np.ndarray, col: np.ndarray, blocked: bool = True):
     col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
     n = row_ptr.size - 1
     e = int(row_ptr[-1])
-    groups = (n + 2047) // 2048
+    GR, NB = PR_GROUP_ROWS, PR_BLOCKS
+    groups = (n + GR - 1) // GR
     L = apps_lib()
-    bcnt = np.zeros(groups * 8 * 2048, dtype=np.uint8)
+    bcnt = np.zeros(groups * NB * GR, dtype=np.uint8)
     if blocked and not L.ua_pr_block_counts(n, _ptr(row_ptr), _ptr(col), _ptr(bcnt)):
         blocked = False
     nbytes = int(L.ua_pr_graph_bytes(n, e, 1 if blocked else 0))
@@ -409,9 +416,9 @@ def pr_graph_layout(row_ptr: np.ndarray, col: np.ndarray, blocked: bool = True):
     rp, deg, icol = g[:n + 1], g[n + 1:2 * n + 1], g[2 * n + 1:2 * n + 1 + e]
     if not blocked:
         return rp, deg, icol, None, None
-    bseg = g[2 * n + 1 + e:2 * n + 1 + e + groups * 8 + 1]
-    off = (2 * n + 1 + e + groups * 8 + 1) * 4
-    return rp, deg, icol, bseg, g.view(np.uint8)[off:off + groups * 8 * 2048]
+    bseg = g[2 * n + 1 + e:2 * n + 1 + e + groups * NB + 1]
+    off = (2 * n + 1 + e + groups * NB + 1) * 4
+    return rp, deg, icol, bseg, g.view(np.uint8)[off:off + groups * NB * GR]
 
 
 class LU:
