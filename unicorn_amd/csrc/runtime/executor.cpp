@@ -942,13 +942,16 @@ void Runtime::gpu_loop() {
         }
     };
 
-    auto retire_batches = [&](bool block) {
+    // Retires the finished batches in flight, oldest first; with wait_oldest, waits for the oldest
+    // one first (only that one: the next flush is staged while the younger batches run).
+    auto retire_batches = [&](bool wait_oldest) {
         while (!batches.empty()) {
             BatchRun& b = batches.front();
-            if (block) {
+            if (wait_oldest) {
                 const double w0 = now_secs();
                 uh_event_sync(b.event);
                 stat_add(&pmHipStubStats::ns_wait, (uint64_t)((now_secs() - w0) * 1e9));
+                wait_oldest = false;
             } else if (uh_event_query(b.event) != 0) return;
             const double r0 = now_secs();
             for (auto& sp : b.slots) {
@@ -1181,7 +1184,7 @@ void Runtime::gpu_loop() {
             inflight++;
         }
     }
-    retire_batches(true);
+    while (!batches.empty()) retire_batches(true);
     for (int i = 0; i < 4; ++i) uh_event_destroy(batch_events[i]);
     cache_purge(nullptr, true);
 }
