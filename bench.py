@@ -34,6 +34,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# The dominant kernel's duration comes from HIP events on its stream around every KT_EVERY-th
+# flush of the timed region: each event pair costs the device a few microseconds, which timing
+# every flush would add to the tasks being measured.
+KT_EVERY = 4
 
 
 def percentile(v, q):
@@ -48,6 +52,22 @@ def time_tasks(run, steps):
         lat.append(run())
     t1 = time.perf_counter()
     return t1 - t0, lat
+
+
+# Untimed tasks run before the warm-up steps, in seconds: under sustained load the GPU's clocks
+# settle within a few milliseconds (measured on the imageFiltering task: its kernel takes about
+# 0.167 ms over the first tens of tasks and 0.150 ms after), so the timed steps see the
+# steady state that a serving loop runs in.
+PREWARM_S = 0.3
+
+
+def prewarm(run, seconds=PREWARM_S):
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        run()
+        n += 1
+    return n
 
 
 def traffic_from_profiles(name):
@@ -72,7 +92,8 @@ def bench_imagefilter(args, U):
 
     # served + batched: HBM-resident address spaces, one persistent launch per task, hipGraph
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    U.set_kernel_timing(True)
+    U.set_kernel_timing(True, every=KT_EVERY)
+    warm = prewarm(lambda: f.run(U.DEVICE_GPU))
     for _ in range(args.warmup):
         f.run(U.DEVICE_GPU)
     U.device_sync()
@@ -101,7 +122,7 @@ def bench_imagefilter(args, U):
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("imagefilter_8192"),
-                "kernel": "k_imagefilter_persistent", "kernel_ms": round(kernel_ms, 4),
+                "kernel": "k_imagefilter_persistent", "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
                 "algorithmic_bytes_per_launch": algo_bytes}
 
     cpu = None
@@ -121,6 +142,7 @@ def bench_imagefilter(args, U):
         "per_subtask_launch": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in comps.items()},
         "speedup_vs_plain": round(batched["tasks_per_s"] / comps["plain"]["tasks_per_s"], 2),
         "roofline": roofline,
+        "prewarm": {"seconds": PREWARM_S, "tasks": warm},
         "cpu_baseline": cpu,
         "stub_stats": {k: st[k] for k in ("subtasks", "batches", "h2d_bytes", "d2h_bytes", "persistent_launches")},
     }
@@ -138,7 +160,8 @@ def bench_matmul(args, U):
     mm = U.MatMul(n, bs, 0)
     mm.set_inputs(A, B)
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    U.set_kernel_timing(True)
+    U.set_kernel_timing(True, every=KT_EVERY)
+    warm = prewarm(lambda: mm.run(U.DEVICE_GPU))
     for _ in range(args.warmup):
         mm.run(U.DEVICE_GPU)
     U.device_sync()
@@ -184,7 +207,8 @@ def bench_matmul(args, U):
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(achieved / 2500.0, 4), "traffic": traffic_from_profiles("matmul_2048"),
                      "kernel": "k_gemm_persistent",
-                     "kernel_ms": round(kernel_ms, 4), "algorithmic_flop_per_launch": issued},
+                     "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes, "algorithmic_flop_per_launch": issued},
+        "prewarm": {"seconds": PREWARM_S, "tasks": warm},
         "cpu_baseline": cpu,
     }
 
@@ -203,10 +227,11 @@ def bench_pagerank(args, U):
     if not pr.blocked:
         raise RuntimeError("bench: the 10M-page graph did not take the source-blocked layout")
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
+    warm = prewarm(lambda: pr.run(iters, U.DEVICE_GPU))
     for _ in range(args.warmup):
         pr.run(iters, U.DEVICE_GPU)
     U.device_sync()
-    U.set_kernel_timing(True)
+    U.set_kernel_timing(True, every=KT_EVERY)
     U.kernel_time(reset=True)
     elapsed, lat = time_tasks(lambda: pr.run(iters, U.DEVICE_GPU), args.steps)
     U.device_sync()
@@ -226,7 +251,7 @@ def bench_pagerank(args, U):
     U.pr_set_layout(True)
     prc.run(iters, U.DEVICE_GPU)
     U.device_sync()
-    U.set_kernel_timing(True)
+    U.set_kernel_timing(True, every=KT_EVERY)
     U.kernel_time(reset=True)
     e, l = time_tasks(lambda: prc.run(iters, U.DEVICE_GPU), 1)
     U.device_sync()
@@ -266,8 +291,9 @@ def bench_pagerank(args, U):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("pagerank_10m"),
                      "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent",
-                     "kernel_ms": round(kernel_ms, 4),
+                     "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
                      "algorithmic_bytes_per_launch": it_bytes},
+        "prewarm": {"seconds": PREWARM_S, "tasks": warm},
         "cpu_baseline": cpu,
     }
 

@@ -89,9 +89,13 @@ pmStatus pmMarkDeviceWritten(pmMemHandle pMemHandle);
  * mirror when that is the newer copy) to pDst without changing where the space is valid: unlike
  * pmFetchMemory + pmGetRawMemPtr, the HBM mirror stays current for the next task. */
 pmStatus pmReadMemoryRange(pmMemHandle pMemHandle, size_t pOffset, size_t pLength, void* pDst);
-/* Device time of the BATCHED mode's flushes (work-queue upload + persistent kernels), measured
- * with events on the stream the persistent kernels run on. */
+/* Device time of the BATCHED mode's flushes (their persistent kernels), measured with events on
+ * the stream the persistent kernels run on: around every flush, or with pmSetHipKernelTimingEvery
+ * around every pEvery-th one (0 turns timing off). Each timed flush costs a few microseconds of
+ * device time for the two event records, so a sample keeps the measurement from slowing the
+ * tasks it measures. pmGetHipKernelTime returns the time and the number of flushes timed. */
 pmStatus pmSetHipKernelTiming(bool pEnable);
+pmStatus pmSetHipKernelTimingEvery(unsigned int pEvery);
 pmStatus pmGetHipKernelTime(double* pMilliseconds, uint64_t* pFlushes, bool pReset);
 /* Whether a HIP device is available to the runtime. */
 bool pmHasHipDevice();

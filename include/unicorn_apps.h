@@ -41,7 +41,7 @@ int ua_set_batch_limit(int subtasks);
 int ua_set_hbm_budget(uint64_t bytes);
 int ua_get_stats(ua_stats* s);
 int ua_reset_stats(void);
-int ua_set_kernel_timing(int on);              /* events around BATCHED flushes */
+int ua_set_kernel_timing(int every);           /* events around every every-th BATCHED flush (0: off) */
 int ua_kernel_time(double* ms, uint64_t* flushes, int reset);
 int ua_device_sync(void);
 const char* ua_last_error(void);

@@ -201,10 +201,11 @@ int uh_batch_pending(void* batch_handle, int* n);
  * batch's stream. With use_graph != 0 the upload+launch sequence is captured once per batch
  * shape and replayed as a hipGraph. */
 int uh_batch_flush(void* batch_handle, int use_graph);
-/* Device time of flushes: when on, events are recorded on the batch stream around every flush
- * (upload of the work queue + the persistent kernels); uh_batch_timing waits for the pending
- * ones and returns the accumulated milliseconds and flush count. */
-int uh_batch_set_timing(void* batch_handle, int on);
+/* Device time of flushes: with every = N >= 1, events are recorded on the batch stream around the
+ * persistent kernels of every N-th flush (the first one after the call, then each N-th; the work
+ * queue's upload precedes the first event); every = 0 records none. uh_batch_timing waits for
+ * the pending ones and returns the accumulated milliseconds and the number of flushes timed. */
+int uh_batch_set_timing(void* batch_handle, int every);
 int uh_batch_timing(void* batch_handle, double* total_ms, uint64_t* flushes, int reset);
 /* Launch counters (kernels launched through this library since load). */
 int uh_launch_counters(uint64_t* plain_launches, uint64_t* persistent_launches, uint64_t* items);

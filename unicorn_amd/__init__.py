@@ -177,8 +177,9 @@ def reset_stats() -> None:
     _check(apps_lib().ua_reset_stats(), "reset stats")
 
 
-def set_kernel_timing(on: bool) -> None:
-    _check(apps_lib().ua_set_kernel_timing(1 if on else 0), "pmSetHipKernelTiming")
+def set_kernel_timing(on: bool, every: int = 1) -> None:
+    """Events around the persistent kernels of every `every`-th BATCHED flush (all when 1)."""
+    _check(apps_lib().ua_set_kernel_timing(every if on else 0), "pmSetHipKernelTimingEvery")
 
 
 def kernel_time(reset: bool = False):

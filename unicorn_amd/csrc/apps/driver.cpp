@@ -66,7 +66,7 @@ int ua_get_stats(ua_stats* s) {
 
 int ua_reset_stats(void) { return (int)pmResetHipStubStats(); }
 
-int ua_set_kernel_timing(int on) { return (int)pmSetHipKernelTiming(on != 0); }
+int ua_set_kernel_timing(int every) { return every < 0 ? -1 : (int)pmSetHipKernelTimingEvery((unsigned)every); }
 
 int ua_kernel_time(double* ms, uint64_t* flushes, int reset) {
     return (int)pmGetHipKernelTime(ms, flushes, reset != 0);

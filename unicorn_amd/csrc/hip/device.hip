@@ -472,7 +472,7 @@ int uh_batch_flush(void* handle, int use_graph) {
     int rc;
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     std::unique_lock<std::mutex> tlock(b->tmtx);
-    if (b->timing) {
+    if (b->timing && b->timing_seq++ % b->timing == 0) {
         if (b->spare.empty()) {
             UH_HIP(hipEventCreate(&ev.first));
             UH_HIP(hipEventCreate(&ev.second));
@@ -519,7 +519,8 @@ int uh_batch_set_timing(void* handle, int on) {
     Batch* b = as_batch(handle);
     if (!b) return fail(UH_ERR_ARG, "uh_batch_set_timing: not a batch");
     std::lock_guard<std::mutex> tlock(b->tmtx);
-    b->timing = on != 0;
+    b->timing = on > 0 ? (uint32_t)on : 0u;
+    b->timing_seq = 0;
     return UH_OK;
 }
 

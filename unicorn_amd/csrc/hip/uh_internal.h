@@ -100,7 +100,8 @@ struct Batch {
     // graphs captured per ring slot, keyed by family mask and the filter kernel variant
     std::map<uint32_t, hipGraphExec_t> graphs[RING];
     // device time of the flushes (events recorded on the batch stream around each flush)
-    bool timing = false;
+    uint32_t timing = 0;          // events around every timing-th flush (0: none)
+    uint64_t timing_seq = 0;      // flushes since timing was set
     std::mutex tmtx;                 // guards timed / spare / the totals
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
     double timed_ms = 0;

@@ -563,11 +563,14 @@ pmStatus pmMarkDeviceWritten(pmMemHandle h) {
 }
 bool pmHasHipDevice() { Runtime* r = Runtime::get(); return r && r->gpu_count > 0; }
 
-pmStatus pmSetHipKernelTiming(bool on) {
+pmStatus pmSetHipKernelTimingEvery(unsigned int every) {
     Runtime* r = Runtime::get();
     if (!r || !r->gpu_count) return pmNoCompatibleDevice;
-    return uh_batch_set_timing(r->batch, on ? 1 : 0) ? pmGraphicsCardError : pmSuccess;
+    if (every > (unsigned)INT_MAX) return pmInvalidIndex;
+    return uh_batch_set_timing(r->batch, (int)every) ? pmGraphicsCardError : pmSuccess;
 }
+
+pmStatus pmSetHipKernelTiming(bool on) { return pmSetHipKernelTimingEvery(on ? 1u : 0u); }
 
 pmStatus pmGetHipKernelTime(double* ms, uint64_t* flushes, bool reset) {
     Runtime* r = Runtime::get();
