@@ -379,6 +379,26 @@ static const size_t kItemBytes[FAM_COUNT] = {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// The queue blocks of a flush move from the pinned host slot to the device slot by a kernel on
+// the batch stream. A DMA-engine copy ahead of the persistent kernels made the kernels wait for
+// the other engine: 12-43 us from the copy's end to the kernel's start in a kernel trace, where
+// one kernel after another on the same queue starts within a few microseconds. Family f's block
+// is words [off[f], off[f] + words[f]) of both slots; the loads are system-scope atomics, so no
+// cache serves an old line of a host slot that the ring has reused.
+struct UploadRanges {
+    uint32_t off[FAM_COUNT];
+    uint32_t words[FAM_COUNT];
+};
+
+__global__ void __launch_bounds__(256) k_upload_queue(uint32_t* dst, const uint32_t* src, UploadRanges r) {
+    const uint32_t f = blockIdx.y;
+    const uint32_t n = r.words[f];
+    const uint32_t* s = src + r.off[f];
+    uint32_t* d = dst + r.off[f];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+        d[i] = __hip_atomic_load(s + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Layout of one upload: per family a QueueHeader block at a fixed offset of the slot
 // (family f's block begins at f * family_stride) so that captured graphs can be replayed
 // for any batch contents that fit the slot.
@@ -455,11 +475,16 @@ int uh_batch_flush(void* handle, int use_graph) {
     // The upload (up to the end of the last family present) goes on the stream ahead of the
     // kernels; the graphs hold only the kernel launches, so one capture per family mix serves
     // every batch, and the timing events bracket the kernels alone.
-    int last = 0;
-    for (int f = 0; f < FAM_COUNT; ++f)
-        if ((mask >> f) & 1) last = f;
-    const size_t used = (size_t)last * family_stride + need_family[last];
-    UH_HIP(hipMemcpyAsync(dev, host, used, hipMemcpyHostToDevice, b->stream));
+    UploadRanges ur;
+    uint32_t max_words = 0;
+    for (int f = 0; f < FAM_COUNT; ++f) {
+        ur.off[f] = (uint32_t)(f * family_stride / 4);
+        ur.words[f] = ((mask >> f) & 1) ? (uint32_t)(need_family[f] / 4) : 0u;   // need_family: 256-byte multiple
+        if (ur.words[f] > max_words) max_words = ur.words[f];
+    }
+    const uint32_t ugx = std::min<uint32_t>((max_words + 255) / 256, 64u);
+    UH_LAUNCH(k_upload_queue, dim3(ugx, FAM_COUNT), dim3(256), 0, b->stream, (uint32_t*)dev, (const uint32_t*)host, ur);
+    UH_LAUNCH_CHECK("k_upload_queue");
     auto enqueue = [&](hipStream_t s) -> int {
         int rc = UH_OK;
         if ((mask >> FAM_FILTER) & 1) rc |= filter_persistent(dev + FAM_FILTER * family_stride, s, b->kinds[FAM_FILTER]);
