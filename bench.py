@@ -18,6 +18,8 @@ requests (BATCHED vs PLAIN stub on the same stream) and reports tasks/s and p50/
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--serving-requests N] [--serving-rate R] [--serving-slots S]
+Defaults of K / W per workload: imagefilter and matmul 200 / 3 (a task takes 0.1-0.2 ms, so 200
+keep one slow task from moving the mean), pagerank 10 / 1, lu 5 / 1.
 """
 from __future__ import annotations
 
@@ -465,8 +467,8 @@ def cpu_baseline_imagefilter(img, weights, r, shift, tile):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="imagefilter", choices=["imagefilter", "matmul", "pagerank", "lu"])
     ap.add_argument("--comparator-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -476,6 +478,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run D2D copies of one image size (reference for HBM counters)")
     args = ap.parse_args()
+    steps, warmup = {"imagefilter": (200, 3), "matmul": (200, 3), "pagerank": (10, 1), "lu": (5, 1)}[args.workload]
+    if args.steps is None:
+        args.steps = steps
+    if args.warmup is None:
+        args.warmup = warmup
+    if args.steps < 1 or args.warmup < 0:
+        ap.error("--steps must be >= 1 and --warmup >= 0")
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}), flush=True)
         return 2
