@@ -95,6 +95,58 @@ def test_imagefilter_brute_force_small():
         assert np.array_equal(O.imagefilter(img, w, r, 0), want)
 
 
+def _raw_sums_by_padding(img_inv, w, r):
+    """Per-pixel int64 sums of the filter, computed by edge-padding the un-inverted image and
+    adding shifted slices (a second, vectorised formulation: no per-pixel clamp loop)."""
+    H, W, _ = img_inv.shape
+    D = 2 * r + 1
+    up = img_inv[::-1].astype(np.int64)                     # image rows in top-down order
+    pad = np.pad(up, ((r, r), (r, r), (0, 0)), mode="edge")
+    acc = np.zeros((H, W, 3), dtype=np.int64)
+    for dy in range(D):
+        for dx in range(D):
+            acc += int(w[dy * D + dx]) * pad[dy:dy + H, dx:dx + W]
+    return acc[:, :, ::-1]                                  # output channel c sums input 2 - c
+
+
+def test_imagefilter_mode1_brute_force_random():
+    """Shift + clamp mode on random images against the padded-slice formulation: floor shift of
+    negative sums (Python's >> floors), both clamp ends, radii 1..3, several shifts."""
+    rng = np.random.default_rng(31)
+    below = above = 0
+    for (H, W, r, lo, hi, shift) in [(9, 11, 1, -4, 5, 4), (17, 13, 2, -4, 5, 4), (12, 20, 2, -4, 5, 0),
+                                      (8, 8, 3, -128, 128, 7), (31, 5, 2, -16, 17, 2), (5, 40, 1, -4, 5, 1)]:
+        img = rng.integers(0, 256, (H, W, 3), dtype=np.uint8)
+        w = rng.integers(lo, hi, (2 * r + 1) ** 2).astype(np.int8)
+        acc = _raw_sums_by_padding(img, w, r)
+        want = np.clip(acc >> shift, 0, 255).astype(np.uint8)
+        got = O.imagefilter(img, w, r, 1, shift)
+        assert np.array_equal(got, want), (H, W, r, shift)
+        below += int((acc < 0).sum())
+        above += int((acc >> shift > 255).sum())
+        if shift:
+            trunc = np.clip(np.sign(acc) * (np.abs(acc) >> shift), 0, 255)
+            assert np.array_equal(trunc, want)  # truncation differs only below zero, clamped away
+            wrapped = ((acc >> shift) % 256).astype(np.uint8)
+            assert not np.array_equal(wrapped, want)                 # no clamp: caught
+    assert below > 0 and above > 0                              # both clamp ends exercised
+    # mode 0 against the same formulation (mod 256)
+    img = rng.integers(0, 256, (23, 19, 3), dtype=np.uint8)
+    w = rng.integers(-128, 128, 25).astype(np.int8)
+    assert np.array_equal(O.imagefilter(img, w, 2, 0), (_raw_sums_by_padding(img, w, 2) % 256).astype(np.uint8))
+
+
+def test_imagefilter_mode1_brute_force_catches_planted_mistakes():
+    rng = np.random.default_rng(32)
+    img = rng.integers(0, 256, (16, 16, 3), dtype=np.uint8)
+    w = rng.integers(-4, 5, 25).astype(np.int8)
+    want = np.clip(_raw_sums_by_padding(img, w, 2) >> 4, 0, 255).astype(np.uint8)
+    wt = w.reshape(5, 5).T.reshape(-1)
+    assert not np.array_equal(O.imagefilter(img, wt, 2, 1, 4), want)                  # transposed weights
+    assert not np.array_equal(O.imagefilter(img, w, 2, 1, 3), want)                   # wrong shift
+    assert not np.array_equal(O.imagefilter(img[::-1].copy(), w, 2, 1, 4), want)      # rows not inverted
+
+
 def test_imagefilter_tiles_equal_whole_image():
     rng = np.random.default_rng(4)
     img = rng.integers(0, 256, (70, 90, 3), dtype=np.uint8)
