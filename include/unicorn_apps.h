@@ -95,7 +95,8 @@ int ua_pr_fetch(void* h, uint64_t* checksum);
 int ua_pr_destroy(void* h);
 /* Graph address space (see uh_pagerank_args for the source-blocked layout). ua_pr_block_counts
  * fills bcnt (ceil(pages / UH_PR_GROUP_ROWS) * UH_PR_BLOCKS * UH_PR_GROUP_ROWS bytes) and returns
- * 1, or 0 when a page has more than 255 in-links from one source block. ua_pr_build_graph writes
+ * 1, or 0 when a page has more than 255 in-links from one source block or pages >= 2^30 (the
+ * blocked kernels address the rank vector with 32-bit byte offsets). ua_pr_build_graph writes
  * [in_row_ptr | outdeg | in_col (| bseg | bcnt)] into ua_pr_graph_bytes(pages, edges, bcnt != 0)
  * bytes; in_col is source-blocked when bcnt is given. ua_pr_set_layout(0) makes later
  * ua_pr_create calls keep the plain in-link CSR; ua_pr_blocked reports an instance's layout. */

@@ -81,3 +81,13 @@ def test_blocked_layout_overflow_falls_back(U):
     col = np.zeros(n - 1, dtype=np.uint32)
     _, _, _, bseg, bcnt = U.pr_graph_layout(rp, col, blocked=True)
     assert bseg is None and bcnt is None
+
+
+def test_blocked_layout_rejects_2_to_30_pages(U):
+    """The blocked kernels address the rank vector with 32-bit byte offsets, so a graph of 2^30
+    pages or more keeps the plain CSR (the builder answers before reading any array)."""
+    L = U.apps_lib()
+    dummy = np.zeros(4, dtype=np.uint32)
+    bcnt = np.zeros(4, dtype=np.uint8)
+    for pages in (1 << 30, (1 << 32) - 1):
+        assert L.ua_pr_block_counts(pages, dummy.ctypes.data, dummy.ctypes.data, bcnt.ctypes.data) == 0

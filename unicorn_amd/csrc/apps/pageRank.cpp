@@ -304,6 +304,9 @@ using namespace pageRank;
 // 0 if some page has more than 255 in-links from one source block (the layout cannot hold it).
 extern "C" int ua_pr_block_counts(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
                                   uint8_t* bcnt) {
+    // the blocked kernels address w with 32-bit byte offsets: graphs of 2^30 pages or more keep
+    // the plain CSR (checked before any array is read)
+    if (pages >= (1u << 30)) return 0;
     const uint32_t bs = block_pages(pages);
     memset(bcnt, 0, (size_t)groups_of(pages) * NB * GR);
     for (uint32_t s = 0; s < pages; ++s) {

@@ -146,8 +146,9 @@ __device__ __forceinline__ void pagerank_unit(const uh_pagerank_args& a, uint32_
     for (uint32_t k = 0; k < wave; ++k) base += sm.wsum[k];
     const uint32_t first = s0 + base + inc - tot;     // this lane's first link
     // Buffer loads: 32-bit offsets from a scalar base, so a gather costs one address register
-    // (source ids are below 2^30). The col resource ends at the segment end, where loads return
-    // source 0; the val entries past the segment are never summed.
+    // (source ids are below kPrBlockedMaxPages = 2^30, which the layout builder enforces). The
+    // col resource ends at the segment end, where loads return source 0; the val entries past
+    // the segment are never summed.
     const __amdgpu_buffer_rsrc_t w_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.w, 0, -1, 0x00020000);
     float acc[PR_RPL];
 #pragma unroll
@@ -324,7 +325,8 @@ extern "C" int uh_pagerank_pull(const uh_pagerank_args* a, void* stream) {
     if (a->bcnt) {
         if (!a->bseg) return fail(UH_ERR_ARG, "uh_pagerank_pull: bcnt without bseg");
         if ((uintptr_t)a->bcnt & 3u) return fail(UH_ERR_ARG, "uh_pagerank_pull: bcnt not 4-byte aligned");
-        if ((uint64_t)a->row0 + a->rows > 0xffffffffull) return fail(UH_ERR_ARG, "uh_pagerank_pull: rows past 2^32");
+        if ((uint64_t)a->row0 + a->rows > kPrBlockedMaxPages)
+            return fail(UH_ERR_ARG, "uh_pagerank_pull: the source-blocked layout holds pages below 2^30");
         const uint32_t groups = pagerank_groups(*a);
         if (Batch* b = as_batch(stream)) {
             record_item(b, FAM_PRBLK, a, sizeof(*a), groups);

@@ -143,6 +143,9 @@ uint32_t pagerank_tiles(const uh_pagerank_args& a);
 uint32_t pagerank_groups(const uh_pagerank_args& a);   // row groups of a source-blocked item
 // Bytes of partial sums that one row group of a source-blocked item needs.
 constexpr size_t kPrGroupScratch = (size_t)UH_PR_BLOCKS * UH_PR_GROUP_ROWS * sizeof(float);
+// Pages of a graph in the source-blocked layout stay below 2^30: the gathers address w with a
+// 32-bit byte offset (source id * 4) from the vector's base.
+constexpr uint64_t kPrBlockedMaxPages = 1ull << 30;
 uint32_t trsm_tiles(const TrsmItem& a);
 
 // Address spaces. A pointer that the compiler cannot trace to a kernel argument (for example
