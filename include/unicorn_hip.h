@@ -138,6 +138,16 @@ typedef struct uh_pagerank_args {
     const uint8_t* bcnt; const uint32_t* bseg;   /* source-blocked layout, or NULL */
 } uh_pagerank_args;
 int uh_pagerank_pull(const uh_pagerank_args* args, void* stream);
+/* Allocation-free variant of uh_pagerank_pull. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results as uh_pagerank_pull on a stream
+ * (a batch handle is rejected with UH_ERR_BATCH); it calls no hipMalloc/hipFree. The
+ * source-blocked layout's partial sums live in a static device array of 32 MiB per GPU (held until
+ * the process ends), which limits one call to 512 row groups of UH_PR_GROUP_ROWS (rows
+ * [row0, row0 + rows) may touch at most 512 groups: 1,048,576 rows when row0 is group-aligned);
+ * larger calls return UH_ERR_ARG. Every call on one GPU shares that array: calls that overlap in
+ * time on one GPU (on different streams) race. The CSR layout uses no array. */
+int uh_pagerank_pull_noalloc(const uh_pagerank_args* args, void* stream);
 /* w[i] = (damping * x0) / deg[i] (or 0 for dangling pages) for the first iteration. */
 int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, float x0, float damping,
                        void* stream);

@@ -363,6 +363,67 @@ extern "C" int uh_pagerank_pull(const uh_pagerank_args* a, void* stream) {
     return UH_OK;
 }
 
+// ---- allocation-free variant of uh_pagerank_pull ----
+// The partial sums of the source-blocked kernels live in a static device array instead of a
+// buffer allocated per stream. Each GPU the library runs on holds its own copy (32 MiB) until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in, not a device address).
+static constexpr uint32_t kNoallocGroups = 512;              // row groups: 1,048,576 rows
+static __device__ __align__(256) float pr_noalloc_partials[(size_t)kNoallocGroups * PR_NB * PR_GR];
+static constexpr int kMaxDevices = 64;
+static float* g_noalloc_ptr[kMaxDevices];
+static std::mutex g_noalloc_mtx;
+
+// Device address of the current device's copy, looked up once per device.
+static int noalloc_partials(float** out) {
+    int dev = 0;
+    UH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mtx);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        UH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(pr_noalloc_partials)));
+        if (!p) return fail(UH_ERR_HIP, "uh_pagerank_pull_noalloc: no device address for the partial-sum array");
+        g_noalloc_ptr[dev] = (float*)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return UH_OK;
+}
+
+extern "C" int uh_pagerank_pull_noalloc(const uh_pagerank_args* a, void* stream) {
+    if (!a || !a->row_ptr || !a->w || !a->y || !a->w_next || !a->deg)
+        return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: null pointer");
+    if (as_batch(stream))
+        return fail(UH_ERR_BATCH, "uh_pagerank_pull_noalloc: takes a stream, not a batch handle (a flush may allocate)");
+    if (a->rows == 0) return UH_OK;
+    if (!a->col) return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: null col");
+    hipStream_t s = (hipStream_t)stream;
+    if (!a->bcnt) {                      // the CSR pull keeps no partial sums
+        UH_LAUNCH(k_pagerank_pull, dim3(pagerank_tiles(*a)), dim3(PR_THREADS), 0, s, *a);
+        UH_LAUNCH_CHECK("k_pagerank_pull");
+        g_plain_launches++;
+        return UH_OK;
+    }
+    if (!a->bseg) return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: bcnt without bseg");
+    if ((uintptr_t)a->bcnt & 3u) return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: bcnt not 4-byte aligned");
+    if ((uint64_t)a->row0 + a->rows > kPrBlockedMaxPages)
+        return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: the source-blocked layout holds pages below 2^30");
+    const uint32_t groups = pagerank_groups(*a);
+    if (groups > kNoallocGroups)
+        return fail(UH_ERR_ARG, "uh_pagerank_pull_noalloc: rows [%u, %u) span %u row groups of %u; at most %u fit "
+                    "the static partial-sum array", a->row0, a->row0 + a->rows, groups, PR_GR, kNoallocGroups);
+    float* part = nullptr;
+    if (int rc = noalloc_partials(&part)) return rc;
+    // every unit writes all PR_GR partials of its (group, block) before the combine reads them,
+    // so the array needs no clearing between calls
+    UH_LAUNCH(k_pagerank_blocked, dim3(groups * PR_NB), dim3(PR_THREADS), 0, s, *a, part);
+    UH_LAUNCH_CHECK("k_pagerank_blocked");
+    UH_LAUNCH(k_pagerank_combine, dim3(groups), dim3(PR_THREADS), 0, s, *a, part);
+    UH_LAUNCH_CHECK("k_pagerank_combine");
+    g_plain_launches += 2;
+    return UH_OK;
+}
+
 extern "C" int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, float x0, float damping,
                                   void* stream) {
     if (!n) return UH_OK;
