@@ -4,22 +4,33 @@
 Metric: tasks/s (+ p50/p99 task latency) of the served + batched HIP stub, next to the
 per-subtask-launch paths on the same GPU and the reference CPU algorithm on the host cores.
 
-Workload (default `imagefilter`): imageFiltering of one 8192 x 8192 RGB uint8 image, 5 x 5
-integer weights in [-4, 4], sum >> 4 clamped to [0, 255], split into 16 subtasks of 2048 x 2048
-tiles with 2-pixel halos. Inputs are synthetic (seeded) and resident in HBM before the timed
-region. One step = one task (submit -> all 16 subtasks done).
+Headline workload: imageFiltering of one 8192 x 8192 RGB uint8 image, 5 x 5 integer weights in
+[-4, 4], sum >> 4 clamped to [0, 255], split into 16 subtasks of 2048 x 2048 tiles with 2-pixel
+halos. Inputs are synthetic (seeded) and resident in HBM before the timed region. One step = one
+task (submit -> all 16 subtasks done). The timed output is compared byte for byte with the
+oracle's output of the same image.
 
-Roofline: the dominant kernel is the persistent filter kernel of the batched flush; its
-algorithmic traffic per launch is 2 * W * H * 3 bytes (each input and output byte once), divided
-by the flush time measured with HIP events on the stream the kernel runs on.
+The default run (`--workload all`) also measures, each on its own configuration and under the
+same clock, with its own roofline, CPU baseline and parity check:
+  - matrixMultiply N=2048 float32, 16 subtasks of 512 x 512 (split-BF16 MFMA);
+  - pageRank on a 10M-page power-law web graph, 20 iterations, 1M-page subtasks;
+  - luDecomposition N=16384 (512 blocks, split-BF16 trailing update) + 8-RHS refined solve;
+  - the serving mix: seeded matrixMultiply / imageFiltering / pageRank requests served at
+    saturation (all requests queued at once, `--serving-slots` clients) by the BATCHED and the
+    PLAIN stub, then at two lower offered Poisson loads for latency.
 
-A second section serves a seeded Poisson mix of matrixMultiply / imageFiltering / pageRank
-requests (BATCHED vs PLAIN stub on the same stream) and reports tasks/s and p50/p99 latency.
+Roofline: for each row the dominant kernel's algorithmic work per launch (bytes or flop) over its
+device time, measured with HIP events on the stream the kernel runs on (BATCHED flushes that
+launch that kernel family alone).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                       [--serving-requests N] [--serving-rate R] [--serving-slots S]
-Defaults of K / W per workload: imagefilter and matmul 200 / 3 (a task takes 0.1-0.2 ms, so 200
-keep one slow task from moving the mean), pagerank 10 / 1, lu 5 / 1.
+CPU baseline: the same task on the pm:: runtime's CPU stubs (one thread per core of the box's CPU
+share), which run the apps' CPU callbacks (restatements of the reference's CPU callbacks) over
+the same subtask decomposition; luDecomposition is timed at N=4096.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|imagefilter|matmul|pagerank|lu]
+                       [--no-cpu-baseline] [--serving-requests N] [--serving-slots S]
+--steps / --warmup apply to the headline (or the one workload named); defaults per workload:
+imagefilter and matmul 200 / 3, pagerank 10 / 1, lu 3 / 1.
 """
 from __future__ import annotations
 
@@ -36,10 +47,27 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+SEED = 20240601
 # The dominant kernel's duration comes from HIP events on its stream around every KT_EVERY-th
 # flush of the timed region: each event pair costs the device a few microseconds, which timing
 # every flush would add to the tasks being measured.
 KT_EVERY = 4
+DEFAULTS = {"imagefilter": (200, 3), "matmul": (200, 3), "pagerank": (10, 1), "lu": (3, 1)}
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """Host cores of this GPU's CPU share (the box sets OMP_NUM_THREADS to it)."""
+    n = os.cpu_count() or 1
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    return max(1, min(n, share if share > 0 else 16))
 
 
 def percentile(v, q):
@@ -82,184 +110,219 @@ def traffic_from_profiles(name):
         return None
 
 
-def bench_imagefilter(args, U):
+def lat_stats(lat):
+    return {"p50_ms": round(1e3 * percentile(lat, 50), 4), "p99_ms": round(1e3 * percentile(lat, 99), 4)}
+
+
+def timed_batched(U, run, steps, warmup, family, every=KT_EVERY, warm=True):
+    """Pre-warm + warm-up + `steps` timed tasks in the BATCHED stub with graphs; returns
+    (elapsed s, latencies, kernel ms per one-family flush, flushes timed, work per flush)."""
+    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
+    n_warm = prewarm(run) if warm else 0
+    for _ in range(warmup):
+        run()
+    U.device_sync()
+    U.set_kernel_timing(True, every=every)
+    U.kernel_time(reset=True)
+    elapsed, lat = time_tasks(run, steps)
+    U.device_sync()
+    ms, fl, work = U.kernel_time_family(family, reset=True)
+    U.set_kernel_timing(False)
+    return elapsed, lat, ms / max(1, fl), fl, work / max(1, fl), n_warm
+
+
+def comparators(U, run, steps):
+    out = {}
+    for name, mode in (("plain", U.PM_HIP_MODE_PLAIN), ("resident", U.PM_HIP_MODE_RESIDENT)):
+        U.set_hip_mode(mode)
+        run()
+        e, l = time_tasks(run, steps)
+        out[name] = {"tasks_per_s": round(steps / e, 3), **lat_stats(l)}
+    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
+    return out
+
+
+# ---------------------------------------------------------------- imageFiltering (headline)
+
+def bench_imagefilter(args, U, steps, warmup, cores):
     H = W = 8192
     r, shift, tile = 2, 4, 2048
-    rng = np.random.default_rng(20240601)
+    rng = np.random.default_rng(SEED)
     img = rng.integers(0, 256, (H, W, 3), dtype=np.uint8)
     weights = rng.integers(-4, 5, (2 * r + 1) ** 2).astype(np.int8)
     f = U.ImageFilter(W, H, weights, r, mode=1, shift=shift, tile=tile)
     f.set_input(img)
     algo_bytes = 2 * W * H * 3
-
-    # served + batched: HBM-resident address spaces, one persistent launch per task, hipGraph
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    U.set_kernel_timing(True, every=KT_EVERY)
-    warm = prewarm(lambda: f.run(U.DEVICE_GPU))
-    for _ in range(args.warmup):
-        f.run(U.DEVICE_GPU)
-    U.device_sync()
-    U.kernel_time(reset=True)
-    U.reset_stats()
-    elapsed, lat = time_tasks(lambda: f.run(U.DEVICE_GPU), args.steps)
-    U.device_sync()
-    kms, flushes = U.kernel_time(reset=True)
+    log("imageFiltering 8192^2: batched")
+    run = lambda: f.run(U.DEVICE_GPU)
+    elapsed, lat, kernel_ms, flushes, work, warm = timed_batched(U, run, steps, warmup, U.FAMILY_FILTER)
+    got = f.output()
     st = U.stats()
-    U.set_kernel_timing(False)
-    batched = {"tasks_per_s": args.steps / elapsed, "p50_ms": 1e3 * percentile(lat, 50),
-               "p99_ms": 1e3 * percentile(lat, 99)}
-    kernel_ms = kms / max(1, flushes)
+    comps = comparators(U, run, max(2, min(steps, args.comparator_steps)))
 
-    # per-subtask-launch comparators on the same GPU
-    comps = {}
-    for name, mode in (("plain", U.PM_HIP_MODE_PLAIN), ("resident", U.PM_HIP_MODE_RESIDENT)):
-        U.set_hip_mode(mode)
-        f.run(U.DEVICE_GPU)
-        n = max(2, min(args.steps, args.comparator_steps))
-        e, l = time_tasks(lambda: f.run(U.DEVICE_GPU), n)
-        comps[name] = {"tasks_per_s": n / e, "p50_ms": 1e3 * percentile(l, 50), "p99_ms": 1e3 * percentile(l, 99)}
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED)
-    f.close()
+    # the same task alternating over two images (inputs and outputs 805 MB, three times the
+    # 256 MB last-level cache): the kernel time is not a cache artefact
+    img2 = rng.integers(0, 256, (H, W, 3), dtype=np.uint8)
+    f2 = U.ImageFilter(W, H, weights, r, mode=1, shift=shift, tile=tile)
+    f2.set_input(img2)
+    flip = [0]
 
-    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("imagefilter_8192"),
-                "kernel": "k_imagefilter_persistent", "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
-                "algorithmic_bytes_per_launch": algo_bytes}
+    def run_alt():
+        flip[0] ^= 1
+        return (f if flip[0] else f2).run(U.DEVICE_GPU)
+    e2, l2, k2, fl2, _, _ = timed_batched(U, run_alt, steps, warmup, U.FAMILY_FILTER, warm=False)
+    comps["batched_two_images_alternating"] = {"tasks_per_s": round(steps / e2, 3), **lat_stats(l2),
+                                               "kernel_ms": round(k2, 4), "timed_flushes": fl2}
+    f2.close()
 
-    cpu = None
+    cpu, parity = None, {"checked": False}
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_imagefilter(img, weights, r, shift, tile)
-
+        cpu, parity = cpu_leg_imagefilter(U, f, img, weights, r, shift, tile, got, cores)
+    f.close()
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     return {
         "metric": "tasks/s (served+batched, imageFiltering 8192^2 task)",
-        "value": round(batched["tasks_per_s"], 3),
-        "unit": "tasks/s",
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-        "dtype": "u8",
+        "value": round(steps / elapsed, 3), "unit": "tasks/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 4), "dtype": "u8",
         "config": {"workload": "imageFiltering 8192x8192 RGB u8, 5x5 int weights in [-4,4], >>4 + clamp, "
                                "16 subtasks of 2048x2048 tiles with 2-px halo",
-                   "hip_mode": "batched+graph", "tile": tile, "radius": r},
-        "p50_ms": round(batched["p50_ms"], 3), "p99_ms": round(batched["p99_ms"], 3),
-        "per_subtask_launch": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in comps.items()},
-        "speedup_vs_plain": round(batched["tasks_per_s"] / comps["plain"]["tasks_per_s"], 2),
-        "roofline": roofline,
+                   "hip_mode": "batched (one-family flushes launch directly)", "tile": tile, "radius": r},
+        **lat_stats(lat),
+        "bit_exact": parity.get("bit_exact"),
+        "parity": parity,
+        "per_subtask_launch": comps,
+        "speedup_vs_plain": round((steps / elapsed) / comps["plain"]["tasks_per_s"], 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("imagefilter_8192"),
+                     "kernel": "k_imagefilter_persistent", "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
+                     "algorithmic_bytes_per_launch": algo_bytes, "library_counted_bytes_per_launch": work},
         "prewarm": {"seconds": PREWARM_S, "tasks": warm},
         "cpu_baseline": cpu,
         "stub_stats": {k: st[k] for k in ("subtasks", "batches", "h2d_bytes", "d2h_bytes", "persistent_launches")},
     }
 
 
-def bench_matmul(args, U):
+def cpu_leg_imagefilter(U, f, img, weights, r, shift, tile, got, cores):
+    """CPU baseline: the task on the pm:: CPU stubs. Checker: the oracle's output of the same
+    image (the C restatement of the reference callback), compared byte for byte with the GPU's."""
+    from oracle import oracle_lib as O
+    log("imageFiltering 8192^2: CPU stubs and oracle")
+    t0 = time.perf_counter()
+    f.run(U.DEVICE_CPU)
+    dt = time.perf_counter() - t0
+    cpu_out = f.output()
+    H, W, _ = img.shape
+    ntiles = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    want = np.zeros_like(img)
+    O.imagefilter_tiled(img, weights, r, 1, shift, tile, 0, ntiles, cores, out=want)
+    bad = int(np.count_nonzero(got != want))
+    parity = {"checked": True, "bit_exact": bad == 0, "mismatched_bytes": bad, "against": "oracle (C restatement)",
+              "cpu_stubs_bit_exact": bool(np.array_equal(cpu_out, want))}
+    cpu = {"value": round(1.0 / dt, 4), "unit": "tasks/s", "cores": cores, "kind": "port",
+           "sample": f"one full task ({ntiles} tiles of {tile}x{tile}) on the pm:: CPU stubs, {cores} threads, "
+                     f"{dt:.2f} s"}
+    return cpu, parity
+
+
+# ---------------------------------------------------------------- matrixMultiply
+
+def bench_matmul(args, U, steps, warmup, cores):
     """matrixMultiply N=2048 float32, A and B uniform in [-1, 1) from the seed, 16 subtasks of
     512 x 512 output blocks; split-BF16 MFMA products (3 per fp32 product) with fp32 accumulation.
-    One step = one task. Roofline: MFMA-issued bf16 flop (3 * 2 N^3) per batched flush over the
-    flush's device time, against the dense bf16 peak."""
+    Roofline: MFMA-issued bf16 flop (3 * 2 N^3) per batched flush over the flush's device time."""
     n, bs = 2048, 512
-    rng = np.random.default_rng(20240601)
+    rng = np.random.default_rng(SEED)
     A = rng.uniform(-1, 1, (n, n)).astype(np.float32)
     B = rng.uniform(-1, 1, (n, n)).astype(np.float32)
     mm = U.MatMul(n, bs, 0)
     mm.set_inputs(A, B)
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    U.set_kernel_timing(True, every=KT_EVERY)
-    warm = prewarm(lambda: mm.run(U.DEVICE_GPU))
-    for _ in range(args.warmup):
-        mm.run(U.DEVICE_GPU)
-    U.device_sync()
-    U.kernel_time(reset=True)
-    elapsed, lat = time_tasks(lambda: mm.run(U.DEVICE_GPU), args.steps)
-    U.device_sync()
-    kms, flushes = U.kernel_time(reset=True)
-    U.set_kernel_timing(False)
-    kernel_ms = kms / max(1, flushes)
+    log("matrixMultiply 2048: batched")
+    run = lambda: mm.run(U.DEVICE_GPU)
+    elapsed, lat, kernel_ms, flushes, issued, warm = timed_batched(U, run, steps, warmup, U.FAMILY_GEMM)
+    Cg = mm.output()
+    comps = comparators(U, run, max(2, min(steps, args.comparator_steps)))
     C64 = A.astype(np.float64) @ B.astype(np.float64)
-    err = float(np.linalg.norm(mm.output() - C64) / np.linalg.norm(C64))
-    comps = {}
-    for name, mode in (("plain", U.PM_HIP_MODE_PLAIN), ("resident", U.PM_HIP_MODE_RESIDENT)):
-        U.set_hip_mode(mode)
-        mm.run(U.DEVICE_GPU)
-        k = max(2, min(args.steps, args.comparator_steps))
-        e, l = time_tasks(lambda: mm.run(U.DEVICE_GPU), k)
-        comps[name] = {"tasks_per_s": round(k / e, 3), "p50_ms": round(1e3 * percentile(l, 50), 3)}
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED)
-    mm.close()
-    flop = 2.0 * n ** 3
-    issued = 3 * flop
-    achieved = issued / (kernel_ms * 1e-3) / 1e12
+    err = float(np.linalg.norm(Cg - C64) / np.linalg.norm(C64))
+    comp = float(np.abs(Cg - C64).max() / (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)).max())
+    parity = {"checked": True, "fp64_normwise_rel_err": err, "max_componentwise_err_over_absAabsB": comp,
+              "tolerance": 1e-5, "pass": err <= 1e-5, "against": "FP64 product"}
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import oracle_lib as O
-        cores = os.cpu_count() or 1
-        k = min(16, cores)
-        C = np.zeros((n, n), dtype=np.float32)
+        log("matrixMultiply 2048: CPU stubs")
         t0 = time.perf_counter()
-        O.matmul_blocks(A, B, bs, 0, k, threads=k, C_out=C)
+        mm.run(U.DEVICE_CPU)
         dt = time.perf_counter() - t0
-        cpu = {"value": round((k / 16) / dt, 4), "unit": "tasks/s", "cores": k, "kind": "port",
-               "sample": f"{k} of the 16 subtasks (plain i-j-k float loop, one thread each), {dt:.2f} s"}
+        ec = float(np.linalg.norm(mm.output() - C64) / np.linalg.norm(C64))
+        cpu = {"value": round(1.0 / dt, 4), "unit": "tasks/s", "cores": cores, "kind": "port",
+               "sample": f"one full task (16 subtasks, plain i-j-k float loop per 512x512 block) on the pm:: CPU "
+                         f"stubs, {cores} threads, {dt:.2f} s; FP64 normwise error {ec:.1e}"}
+    mm.close()
+    flop = 2.0 * n ** 3
+    achieved = issued / (kernel_ms * 1e-3) / 1e12
     return {
         "metric": "tasks/s (served+batched, matrixMultiply N=2048 fp32 task)",
-        "value": round(args.steps / elapsed, 3), "unit": "tasks/s",
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "dtype": "f32 (split-bf16 x3 MFMA)",
-        "config": {"workload": "matrixMultiply N=2048 float32, 16 subtasks of 512x512", "hip_mode": "batched+graph"},
-        "p50_ms": round(1e3 * percentile(lat, 50), 3), "p99_ms": round(1e3 * percentile(lat, 99), 3),
-        "per_subtask_launch": comps, "fp64_normwise_rel_err": err,
+        "value": round(steps / elapsed, 3), "unit": "tasks/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 4), "dtype": "f32 (split-bf16 x3 MFMA, fp32 accumulate)",
+        "config": {"workload": "matrixMultiply N=2048 float32, 16 subtasks of 512x512", "hip_mode": "batched"},
+        **lat_stats(lat), "parity": parity, "per_subtask_launch": comps,
         "fp32_equivalent_tflops": round(flop / (kernel_ms * 1e-3) / 1e12, 1),
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(achieved / 2500.0, 4), "traffic": traffic_from_profiles("matmul_2048"),
-                     "kernel": "k_gemm_persistent",
-                     "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes, "algorithmic_flop_per_launch": issued},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles("matmul_2048"),
+                     "kernel": "k_gemm_persistent", "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
+                     "algorithmic_flop_per_launch": issued},
         "prewarm": {"seconds": PREWARM_S, "tasks": warm},
         "cpu_baseline": cpu,
     }
 
 
-def bench_pagerank(args, U):
+# ---------------------------------------------------------------- pageRank
+
+def bench_pagerank(args, U, steps, warmup, cores):
     """pageRank on a 10M-page synthetic power-law web graph (out-degree P(k) ~ (k + k0)^-2 on
     [1, 100], mean 10, 5 % dangling, preferential targets), 20 iterations, 1M-page subtasks.
-    One step = one pageRank computation (init task + 20 iteration tasks) over the source-blocked
-    graph layout. Roofline: per iteration 12 N + 4 E bytes (row pointers, degree, rank in / out,
-    in-link ids) over the flush's device time (the blocked gather and the combine kernel). The
-    plain in-link CSR layout of the same graph is timed beside it."""
+    One step = one pageRank computation (init task + 20 iteration tasks). Roofline: per iteration
+    12 N + 4 E bytes (row pointers, degree, rank in / out, in-link ids) over one iteration's
+    flush time."""
     pages, per, iters = 10_000_000, 1_000_000, 20
-    rp, col = U.webgen(pages, 20240601)
+    log("pageRank 10M: graph")
+    rp, col = U.webgen(pages, SEED)
     edges = int(col.size)
     pr = U.PageRank(rp, col, per)
     if not pr.blocked:
         raise RuntimeError("bench: the 10M-page graph did not take the source-blocked layout")
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    warm = prewarm(lambda: pr.run(iters, U.DEVICE_GPU))
-    for _ in range(args.warmup):
-        pr.run(iters, U.DEVICE_GPU)
-    U.device_sync()
-    U.set_kernel_timing(True, every=KT_EVERY)
-    U.kernel_time(reset=True)
-    elapsed, lat = time_tasks(lambda: pr.run(iters, U.DEVICE_GPU), args.steps)
-    U.device_sync()
-    kms, flushes = U.kernel_time(reset=True)
-    U.set_kernel_timing(False)
+    log("pageRank 10M: batched")
+    run = lambda: pr.run(iters, U.DEVICE_GPU)
+    elapsed, lat, kernel_ms, flushes, _, warm = timed_batched(U, run, steps, warmup, U.FAMILY_PAGERANK_BLOCKED)
     y = pr.ranks()
-    kernel_ms = kms / max(1, flushes)
     comps = {}
     U.set_hip_mode(U.PM_HIP_MODE_PLAIN)
-    e, l = time_tasks(lambda: pr.run(iters, U.DEVICE_GPU), 1)
+    e, l = time_tasks(run, 1)
     comps["plain"] = {"tasks_per_s": round(1 / e, 4), "p50_ms": round(1e3 * l[0], 1)}
-    U.set_hip_mode(U.PM_HIP_MODE_BATCHED)
+    U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
+    cpu, parity = None, {"checked": False}
+    if not args.no_cpu_baseline:
+        log("pageRank 10M: CPU stubs and oracle")
+        t0 = time.perf_counter()
+        pr.run(iters, U.DEVICE_CPU)
+        dt = time.perf_counter() - t0
+        yc = pr.ranks()
+        from oracle import oracle_lib as O
+        yo = O.pagerank(rp, col, iters, per)
+        err = float(np.abs(y.astype(np.float64) - yo).sum() / np.abs(yo).sum())
+        parity = {"checked": True, "rel_l1_vs_oracle": err, "tolerance": 1e-5, "pass": err <= 1e-5,
+                  "against": "oracle (push per subtask + REDUCE_ADD, fp32)",
+                  "cpu_stubs_rel_l1_vs_oracle": float(np.abs(yc.astype(np.float64) - yo).sum() / np.abs(yo).sum())}
+        cpu = {"value": round(1.0 / dt, 4), "unit": "tasks/s", "cores": cores, "kind": "port",
+               "sample": f"one full task (20 iterations x 10 subtasks) on the pm:: CPU stubs, {cores} threads, "
+                         f"{dt:.2f} s"}
     pr.close()
-    # the same graph in the plain in-link CSR layout (CSR-stream pull, no source blocking)
+    log("pageRank 10M: CSR layout and push decomposition comparators")
     U.pr_set_layout(False)
     prc = U.PageRank(rp, col, per)
     U.pr_set_layout(True)
-    prc.run(iters, U.DEVICE_GPU)
-    U.device_sync()
-    U.set_kernel_timing(True, every=KT_EVERY)
-    U.kernel_time(reset=True)
-    e, l = time_tasks(lambda: prc.run(iters, U.DEVICE_GPU), 1)
-    U.device_sync()
-    cms, cfl = U.kernel_time(reset=True)
-    U.set_kernel_timing(False)
-    comps["csr_layout"] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms / max(1, cfl), 4)}
+    e, l, cms, cfl, _, _ = timed_batched(U, lambda: prc.run(iters, U.DEVICE_GPU), 1, 1, U.FAMILY_PAGERANK,
+                                         every=1, warm=False)
+    comps["csr_layout"] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms, 4)}
     prc.close()
     # the reference's decomposition: source-page subtasks push into private full-length vectors
     # that REDUCE_ADD folds in HBM (records padded to the largest out-degree, 101 uint32 per page)
@@ -273,26 +336,16 @@ def bench_pagerank(args, U):
                                 "rel_l1_vs_pull": float(np.abs(yp.astype(np.float64) - y).sum() / np.abs(y).sum())}
     it_bytes = 12 * pages + 4 * edges
     achieved = it_bytes / (kernel_ms * 1e-3) / 1e9
-    cpu, err = None, None
-    if not args.no_cpu_baseline:
-        from oracle import oracle_lib as O
-        t0 = time.perf_counter()
-        yo = O.pagerank(rp, col, iters, per)
-        dt = time.perf_counter() - t0
-        err = float(np.abs(y.astype(np.float64) - yo).sum() / np.abs(yo).sum())
-        cpu = {"value": round(1 / dt, 4), "unit": "tasks/s", "cores": 1, "kind": "port",
-               "sample": f"one full task (20 iterations, 10 push subtasks + REDUCE_ADD each), one thread, {dt:.1f} s"}
     return {
         "metric": "tasks/s (served+batched, pageRank 10M pages x 20 iterations)",
-        "value": round(args.steps / elapsed, 3), "unit": "tasks/s",
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "dtype": "f32",
+        "value": round(steps / elapsed, 3), "unit": "tasks/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 3), "dtype": "f32",
         "config": {"workload": f"pageRank {pages} pages, {edges} links, 20 iterations, 1M-page subtasks",
-                   "hip_mode": "batched+graph"},
-        "p50_ms": round(1e3 * percentile(lat, 50), 3), "p99_ms": round(1e3 * percentile(lat, 99), 3),
-        "per_subtask_launch": comps, "rel_l1_vs_oracle": err,
+                   "hip_mode": "batched"},
+        **lat_stats(lat), "parity": parity, "per_subtask_launch": comps,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("pagerank_10m"),
-                     "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent",
+                     "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent (one iteration)",
                      "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
                      "algorithmic_bytes_per_launch": it_bytes},
         "prewarm": {"seconds": PREWARM_S, "tasks": warm},
@@ -300,52 +353,68 @@ def bench_pagerank(args, U):
     }
 
 
-def bench_lu(args, U):
+# ---------------------------------------------------------------- luDecomposition
+
+def _lu_system(rng, n, nrhs):
+    M = rng.random((n, n), dtype=np.float32)
+    M[np.diag_indices(n)] += np.float32(n)
+    x_true = rng.uniform(-1, 1, (n, nrhs))
+    b = np.asfortranarray(M.T.astype(np.float64) @ x_true)
+    return M, x_true, b
+
+
+def bench_lu(args, U, steps, warmup, cores):
     """luDecomposition N=16384 float32, A = U[0,1) + N I (column-major), 512 x 512 blocks, the
     trailing update in split-BF16 MFMA, then A x = b for 8 seeded right-hand sides with
     FP64-residual iterative refinement. One step = factorisation + refined solve, with the
-    matrix resident in HBM. Errors are against the FP64 truth (b = A x_true in FP64); the plain
-    FP32 path (VALU trailing update) is compared at N=4096."""
+    matrix resident in HBM. Errors are against the FP64 truth (b = A x_true in FP64). Roofline:
+    the stage-3 trailing-update GEMM (k_gemm_persistent), its flop issued over its flushes'
+    device time; the plain FP32 path (VALU update) is compared at N=4096."""
     n, bs, nrhs = 16384, 512, 8
-    rng = np.random.default_rng(20240601)
-    M = rng.random((n, n), dtype=np.float32)
-    M[np.diag_indices(n)] += np.float32(n)
+    rng = np.random.default_rng(SEED)
+    log("luDecomposition 16384: matrix")
+    M, x_true, b = _lu_system(rng, n, nrhs)
     A = M.T                                   # column-major view of a U[0,1) + N I matrix
-    x_true = rng.uniform(-1, 1, (n, nrhs))
-    b = np.asfortranarray(M.T.astype(np.float64) @ x_true)
     lu = U.LU(n, bs, 0)
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
     fact, solve, infos = [], [], []
-    for step in range(args.warmup + args.steps):
+    gemm = [0.0, 0, 0.0]
+    trsm = [0.0, 0]
+    for step in range(warmup + steps):
         lu.set_matrix(A)
         lu.prefetch()
         U.device_sync()
+        timed = step >= warmup
+        if timed:
+            U.set_kernel_timing(True, every=1)
+            U.kernel_time(reset=True)
         t0 = time.perf_counter()
         lu.factor(U.DEVICE_GPU)
         t1 = time.perf_counter()
+        if timed:
+            ms, fl, w = U.kernel_time_family(U.FAMILY_GEMM, reset=True)
+            gemm[0] += ms; gemm[1] += fl; gemm[2] += w
+            ms, fl, _ = U.kernel_time_family(U.FAMILY_TRSM, reset=True)
+            trsm[0] += ms; trsm[1] += fl
+            U.set_kernel_timing(False)
         x, info = lu.solve(b, 5, U.DEVICE_GPU)
         t2 = time.perf_counter()
-        if step >= args.warmup:
+        if timed:
             fact.append(t1 - t0)
             solve.append(t2 - t1)
             infos.append(info)
-        print(f"lu: step {step + 1}/{args.warmup + args.steps} factor {t1 - t0:.4f} s solve {t2 - t1:.4f} s",
-              file=sys.stderr, flush=True)
+        log(f"lu: step {step + 1}/{warmup + steps} factor {t1 - t0:.4f} s solve {t2 - t1:.4f} s")
     fwd = float(np.abs(x - x_true).max() / np.abs(x_true).max())
     lu.close()
     del M, A
     factor_s, solve_s = float(np.median(fact)), float(np.median(solve))
     flop = 2.0 / 3.0 * n ** 3
     trailing_issued = 3 * 2.0 * sum((n - (k + 1) * bs) ** 2 * bs for k in range(n // bs - 1))
-    # plain FP32 comparison at N=4096 (VALU trailing update) against split-BF16 at the same size
     n2 = 4096
-    M2 = rng.random((n2, n2), dtype=np.float32)
-    M2[np.diag_indices(n2)] += np.float32(n2)
-    x2 = rng.uniform(-1, 1, (n2, nrhs))
-    b2 = np.asfortranarray(M2.T.astype(np.float64) @ x2)
+    M2, x2, b2 = _lu_system(rng, n2, nrhs)
     small = {}
     for name, prec in (("split_bf16", 0), ("plain_fp32", 2)):
-        print(f"lu: N={n2} comparison, {name}", file=sys.stderr, flush=True)
+        log(f"lu: N={n2} comparison, {name}")
         l2 = U.LU(n2, bs, prec)
         l2.set_matrix(M2.T)
         l2.prefetch()
@@ -365,42 +434,59 @@ def bench_lu(args, U):
                        "forward_err": float(np.abs(y - x2).max() / np.abs(x2).max())}
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import oracle_lib as O
-        n3 = 2048
-        print(f"lu: CPU baseline at N={n3}", file=sys.stderr, flush=True)
-        M3 = np.asfortranarray(rng.random((n3, n3), dtype=np.float32) + n3 * np.eye(n3, dtype=np.float32))
+        log(f"lu: CPU stubs at N={n2}")
+        l3 = U.LU(n2, bs, 0)
+        l3.set_matrix(M2.T)
         t0 = time.perf_counter()
-        O.lu(M3)
-        dt = time.perf_counter() - t0
-        est = dt * (n / n3) ** 3
-        cpu = {"value": round(1.0 / est, 6), "unit": "tasks/s", "cores": 1, "kind": "port",
-               "sample": f"unblocked LU (sscal + sger order) at N={n3} on one thread, {dt:.2f} s, "
-                         f"scaled by (N/{n3})^3 to N={n} (factorisation only)"}
+        l3.factor(U.DEVICE_CPU)
+        t1 = time.perf_counter()
+        y3, inf3 = l3.solve(b2, 5, U.DEVICE_CPU)
+        t2 = time.perf_counter()
+        l3.close()
+        cpu = {"value": round(1.0 / (t2 - t0), 4), "unit": "tasks/s", "cores": cores, "kind": "port",
+               "sample": f"one task at N={n2} (not 16384): factorisation {t1 - t0:.2f} s + refined solve "
+                         f"{t2 - t1:.2f} s on the pm:: CPU stubs, {cores} threads; solve backward error "
+                         f"{inf3['backward_error']:.1e}"}
+    bwd = infos[-1]["backward_error"]
+    parity = {"checked": True, "solve_backward_err": bwd, "solve_backward_tolerance": 2.0 ** -24,
+              "forward_err": fwd, "pass": bool(bwd <= 2.0 ** -24),
+              "against": "FP64 truth (b = A x_true in FP64); N=4096 factor errors in n4096_comparison"}
+    gemm_ms, gemm_fl, gemm_w = gemm[0] / steps, gemm[1] / steps, gemm[2] / steps
+    achieved = gemm_w / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     return {
         "metric": "tasks/s (luDecomposition N=16384 factorisation + 8-RHS refined solve)",
-        "value": round(1.0 / (factor_s + solve_s), 4), "unit": "tasks/s",
-        "ms_per_step": round(1e3 * (factor_s + solve_s), 1), "dtype": "f32 (split-bf16 x3 MFMA update) + f64 residual",
+        "value": round(1.0 / (factor_s + solve_s), 4), "unit": "tasks/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * (factor_s + solve_s), 2),
+        "dtype": "f32 (split-bf16 x3 MFMA update) + f64 residual",
         "config": {"workload": "luDecomposition N=16384 fp32 U[0,1)+N*I, 512 blocks, 8 RHS refinement",
-                   "hip_mode": "batched+graph"},
+                   "hip_mode": "batched"},
         "factor_s": round(factor_s, 4), "solve_s": round(solve_s, 4),
         "factor_fp32_equivalent_tflops": round(flop / factor_s / 1e12, 2),
-        "refine_steps": infos[-1]["steps"], "solve_backward_err": infos[-1]["backward_error"],
-        "forward_err": fwd, "n4096_comparison": small,
-        "roofline": {"bound": "mfma", "achieved": round(trailing_issued / factor_s / 1e12, 1), "peak": 2500.0,
-                     "unit": "TFLOP/s", "frac": round(trailing_issued / factor_s / 1e12 / 2500.0, 4),
-                     "traffic": None, "kernel": "whole factorisation (trailing-update MFMA flop / factor time)",
-                     "algorithmic_flop_per_launch": trailing_issued},
+        "factor_mfma_issued_tflops": round(trailing_issued / factor_s / 1e12, 1),
+        "refine_steps": infos[-1]["steps"], "parity": parity, "n4096_comparison": small,
+        "stage2_trsm": {"kernel": "k_trsm_persistent", "ms_per_factorisation": round(trsm[0] / steps, 3),
+                        "flushes": trsm[1] // max(1, steps)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles("lu_16384_stage3"),
+                     "kernel": "k_gemm_persistent (stage-3 trailing update)",
+                     "kernel_ms": round(gemm_ms / max(1, gemm_fl), 4), "timed_flushes": gemm_fl,
+                     "kernel_ms_per_factorisation": round(gemm_ms, 3),
+                     "algorithmic_flop_per_launch": round(gemm_w / max(1, gemm_fl)),
+                     "algorithmic_flop_per_factorisation": gemm_w},
         "cpu_baseline": cpu,
     }
 
 
+# ---------------------------------------------------------------- serving mix
+
 def bench_serving(args, U):
-    """Serving mix on one GPU: a seeded Poisson stream of matrixMultiply (N log-uniform in
-    256..4096, float32, split-BF16 MFMA), imageFiltering (RGB u8 side uniform in 512..4096, 5x5,
-    >>4 + clamp) and pageRank (100k..2M pages, 20 iterations) requests over resident datasets,
-    each request's output returned to the host. The same stream is served by the BATCHED stub
-    (continuous batching into persistent kernels, hipGraphs) and by the PLAIN per-subtask-launch
-    stub; both report tasks/s and p50/p99 latency (arrival -> result on the host)."""
+    """Serving mix on one GPU: seeded matrixMultiply (N log-uniform in 256..4096, float32,
+    split-BF16 MFMA), imageFiltering (RGB u8 side uniform in 512..4096, 5x5, >>4 + clamp) and
+    pageRank (100k..2M pages, 20 iterations) requests over resident datasets, each request's
+    output returned to the host and hashed. Saturation: all requests queued at t = 0 and served by
+    `slots` clients, on the BATCHED stub (continuous batching into persistent kernels) and on the
+    PLAIN per-subtask-launch stub; equal accuracy = every request's output hash is the same on
+    both. Latency: Poisson arrivals at 25 % and 50 % of the BATCHED capacity."""
     S = U.Server()
     t0 = time.perf_counter()
     mm = [256, 512, 768, 1024, 1536, 2048, 3072, 4096]
@@ -409,29 +495,43 @@ def bench_serving(args, U):
            "pr": [(p_, S.add_pr(p_, 1_000_000, 20, seed=300 + i))
                   for i, p_ in enumerate([100_000, 500_000, 1_000_000, 2_000_000])]}
     setup_s = time.perf_counter() - t0
-    n, rate, slots = args.serving_requests, args.serving_rate, args.serving_slots
-    ds, arr = U.serving_stream(20240601, n, rate, cat)
-    kind_of = {}
-    for k, entries in cat.items():
-        for _, d in entries:
-            kind_of[d] = k
+    n, slots = args.serving_requests, args.serving_slots
+    ds, _ = U.serving_stream(SEED, n, 0.0, cat)
+    kind_of = {d: k for k, entries in cat.items() for _, d in entries}
     kinds = np.array([kind_of[int(d)] for d in ds])
-    out = {"requests": n, "rate_per_s": rate, "slots": slots, "seed": 20240601,
+    out = {"requests": n, "slots": slots, "seed": SEED,
            "mix": {k: int((kinds == k).sum()) for k in ("mm", "if", "pr")},
            "results_to_host": True, "dataset_setup_s": round(setup_s, 2)}
     warm = np.array(sorted({int(d) for d in ds}), dtype=np.int32)
-    for name, mode in (("batched", U.PM_HIP_MODE_BATCHED), ("plain", U.PM_HIP_MODE_PLAIN)):
+
+    def serve(mode, arrivals, label):
         U.set_hip_mode(mode, graphs=True)
         S.run(warm, np.zeros(len(warm)), 4, U.DEVICE_GPU)          # uploads, clones, graphs
-        lat, _, wall = S.run(ds, arr, slots, U.DEVICE_GPU)
+        log(f"serving: {label}")
+        lat, cs, wall = S.run(ds, arrivals, slots, U.DEVICE_GPU, checksums=True)
         res = {"tasks_per_s": round(n / wall, 2), "wall_s": round(wall, 3),
-               "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3)}
-        res["per_class"] = {k: {"p50_ms": round(percentile(lat[kinds == k], 50), 3),
-                                "p99_ms": round(percentile(lat[kinds == k], 99), 3)}
-                            for k in ("mm", "if", "pr") if (kinds == k).any()}
-        out[name] = res
+               "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+               "per_class": {k: {"p50_ms": round(percentile(lat[kinds == k], 50), 3),
+                                 "p99_ms": round(percentile(lat[kinds == k], 99), 3)}
+                             for k in ("mm", "if", "pr") if (kinds == k).any()}}
+        return res, cs
+
+    zero = np.zeros(n)
+    out["saturation_batched"], cs_b = serve(U.PM_HIP_MODE_BATCHED, zero, "saturation, batched")
+    out["saturation_plain"], cs_p = serve(U.PM_HIP_MODE_PLAIN, zero, "saturation, plain")
+    out["equal_outputs_batched_vs_plain"] = bool(np.array_equal(cs_b, cs_p))
+    cap = out["saturation_batched"]["tasks_per_s"]
+    out["speedup_vs_plain"] = round(cap / out["saturation_plain"]["tasks_per_s"], 2)
+    loads = []
+    for frac in (0.25, 0.5):
+        rate = frac * cap
+        _, arr = U.serving_stream(SEED, n, rate, cat)
+        res, cs = serve(U.PM_HIP_MODE_BATCHED, arr, f"offered {rate:.0f}/s, batched")
+        res["offered_per_s"] = round(rate, 1)
+        res["equal_outputs"] = bool(np.array_equal(cs, cs_b))
+        loads.append(res)
+    out["offered_load_batched"] = loads
     U.set_hip_mode(U.PM_HIP_MODE_BATCHED, graphs=True)
-    out["speedup_vs_plain"] = round(out["batched"]["tasks_per_s"] / out["plain"]["tasks_per_s"], 2)
     S.close()
     return out
 
@@ -449,19 +549,7 @@ def copy_calibration(nbytes, reps=3):
     dst.free()
 
 
-def cpu_baseline_imagefilter(img, weights, r, shift, tile):
-    """The oracle (C restatement of the reference CPU callback) on the host cores: one full task
-    (16 tile subtasks), one worker thread per core pulling tiles, as the reference's CPU stubs."""
-    from oracle import oracle_lib as O
-    cores = os.cpu_count() or 1
-    H, W, _ = img.shape
-    ntiles = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
-    out = np.zeros_like(img)
-    t0 = time.perf_counter()
-    O.imagefilter_tiled(img, weights, r, 1, shift, tile, 0, ntiles, cores, out=out)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "tasks/s", "cores": cores, "kind": "port",
-            "sample": f"one full task ({ntiles} tiles of {tile}x{tile}) on {cores} threads, {dt:.2f} s"}
+BENCHES = {"imagefilter": bench_imagefilter, "matmul": bench_matmul, "pagerank": bench_pagerank, "lu": bench_lu}
 
 
 def main():
@@ -469,41 +557,49 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="imagefilter", choices=["imagefilter", "matmul", "pagerank", "lu"])
+    ap.add_argument("--workload", default="all", choices=["all"] + list(BENCHES))
     ap.add_argument("--comparator-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=300, help="0 skips the serving mix")
-    ap.add_argument("--serving-rate", type=float, default=1000.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run D2D copies of one image size (reference for HBM counters)")
     args = ap.parse_args()
-    steps, warmup = {"imagefilter": (200, 3), "matmul": (200, 3), "pagerank": (10, 1), "lu": (5, 1)}[args.workload]
-    if args.steps is None:
-        args.steps = steps
-    if args.warmup is None:
-        args.warmup = warmup
-    if args.steps < 1 or args.warmup < 0:
+    primary = "imagefilter" if args.workload == "all" else args.workload
+    steps, warmup = DEFAULTS[primary]
+    if args.steps is not None:
+        steps = args.steps
+    if args.warmup is not None:
+        warmup = args.warmup
+    if steps < 1 or warmup < 0:
         ap.error("--steps must be >= 1 and --warmup >= 0")
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}), flush=True)
+        print(json.dumps({"error": "this build measures one GPU only (no collectives or multi-GPU path); "
+                                   "run with --gpus 1"}), flush=True)
         return 2
     import unicorn_amd as U
-    U.init(0)
+    cores = host_cores()
+    U.init(cores)
     if not U.has_gpu():
         print(json.dumps({"error": "no HIP device visible"}), flush=True)
         return 1
     if args.copy_calibration:
         copy_calibration(8192 * 8192 * 3)
-    res = {"imagefilter": bench_imagefilter, "matmul": bench_matmul, "pagerank": bench_pagerank,
-           "lu": bench_lu}[args.workload](args, U)
+    res = BENCHES[primary](args, U, steps, warmup, cores)
     line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
+            "steps": res.pop("steps"), "warmup": res.pop("warmup"), "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
             "data": "synthetic (seeded)", "config": res.pop("config")}
     line.update(res)
-    if args.serving_requests > 0 and args.workload == "imagefilter":
-        line["serving"] = bench_serving(args, U)
+    line["host_cores"] = cores
+    if args.workload == "all":
+        rows = {}
+        for name in ("matmul", "pagerank", "lu"):
+            s, w = DEFAULTS[name]
+            rows[name] = BENCHES[name](args, U, s, w, cores)
+        line["rows"] = rows
+        if args.serving_requests > 0:
+            line["serving"] = bench_serving(args, U)
     print(json.dumps(line), flush=True)
     return 0
 

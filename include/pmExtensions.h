@@ -97,6 +97,11 @@ pmStatus pmReadMemoryRange(pmMemHandle pMemHandle, size_t pOffset, size_t pLengt
 pmStatus pmSetHipKernelTiming(bool pEnable);
 pmStatus pmSetHipKernelTimingEvery(unsigned int pEvery);
 pmStatus pmGetHipKernelTime(double* pMilliseconds, uint64_t* pFlushes, bool pReset);
+/* The timed flushes that launched one kernel family only (family numbers as UH_FAMILY_* of
+ * unicorn_hip.h): milliseconds, count, and the family's algorithmic work over those flushes
+ * (GEMM: flop issued; filter: bytes moved; others: 0). */
+pmStatus pmGetHipKernelTimeByFamily(int pFamily, double* pMilliseconds, uint64_t* pFlushes, double* pWork,
+                                    bool pReset);
 /* Whether a HIP device is available to the runtime. */
 bool pmHasHipDevice();
 

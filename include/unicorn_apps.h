@@ -43,6 +43,8 @@ int ua_get_stats(ua_stats* s);
 int ua_reset_stats(void);
 int ua_set_kernel_timing(int every);           /* events around every every-th BATCHED flush (0: off) */
 int ua_kernel_time(double* ms, uint64_t* flushes, int reset);
+/* flushes that launched one kernel family only (UH_FAMILY_*): time, count, algorithmic work */
+int ua_kernel_time_family(int family, double* ms, uint64_t* flushes, double* work, int reset);
 int ua_device_sync(void);
 const char* ua_last_error(void);
 
@@ -62,6 +64,8 @@ int ua_if_run(void* h, int device, double* seconds);
 int ua_if_get_output(void* h, uint8_t* out);
 void* ua_if_clone(void* h);                  /* same input image, own output */
 int ua_if_fetch(void* h, uint64_t* checksum); /* output to the host; checksum may be NULL */
+int ua_if_get_input(void* h, uint8_t* image); /* the input image (rows bottom-up) */
+int ua_if_get_config(void* h, ua_if_config* cfg);
 int ua_if_destroy(void* h);
 
 /* ---- matrixMultiply (reference testSuite/matrixMultiplyBlas decomposition, plain CPU method) ----
@@ -72,6 +76,7 @@ int ua_mm_run(void* h, int device, double* seconds);
 int ua_mm_get_output(void* h, float* C);
 void* ua_mm_clone(void* h);                  /* same A and B, own C */
 int ua_mm_fetch(void* h, uint64_t* checksum);
+int ua_mm_get_inputs(void* h, float* A, float* B);
 int ua_mm_destroy(void* h);
 
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
@@ -98,14 +103,23 @@ int ua_pr_destroy(void* h);
  * 1, or 0 when a page has more than 255 in-links from one source block or pages >= 2^30 (the
  * blocked kernels address the rank vector with 32-bit byte offsets). ua_pr_build_graph writes
  * [in_row_ptr | outdeg | in_col (| bseg | bcnt)] into ua_pr_graph_bytes(pages, edges, bcnt != 0)
- * bytes; in_col is source-blocked when bcnt is given. ua_pr_set_layout(0) makes later
- * ua_pr_create calls keep the plain in-link CSR; ua_pr_blocked reports an instance's layout. */
+ * bytes; in_col is source-blocked when bcnt is given. ua_pr_set_layout chooses the layout of later
+ * ua_pr_create calls: 2 = binned (uh_pagerank_binned_args; the default, used when damping <= 1
+ * and the link slots fit 32 bits), 1 = source-blocked, 0 = plain in-link CSR; a layout that cannot
+ * hold a graph falls back to the next (binned -> blocked -> CSR). ua_pr_layout reports an instance's
+ * layout, ua_pr_blocked whether it is the source-blocked one. ua_prb_layout plans (graph == NULL:
+ * returns the graph bytes and info = {units, runs, slots, subtasks, offsets of the deg, unit_start,
+ * bin_start, run_ptr, run_off, run_pos, dst16 and src16 regions}) or builds the binned graph
+ * address space for `cus` CUs; it returns 0 when the layout cannot hold the graph. */
 int ua_pr_block_counts(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col, uint8_t* bcnt);
 uint64_t ua_pr_graph_bytes(uint32_t pages, uint64_t edges, int blocked);
 int ua_pr_build_graph(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
                       const uint8_t* bcnt, uint32_t* graph);
-void ua_pr_set_layout(int blocked);
+void ua_pr_set_layout(int layout);
 int ua_pr_blocked(void* h);
+int ua_pr_layout(void* h);
+uint64_t ua_prb_layout(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                       uint32_t pages_per_subtask, uint32_t cus, void* graph, uint64_t* info);
 /* The reference's on-disk web dump (pageRank.cpp:53-197, scripts/webGenerator.pl:121-274):
  * <dir>/web_dump_metadata = {pages, max out-links, pages per file} (uint32); <dir>/web/page_<n>
  * = fixed records {count, count 1-based targets, zero padding to max out-links}, file n holding
@@ -156,6 +170,12 @@ int ua_serve_add_pr(void* s, uint32_t pages, uint32_t pages_per_subtask, int ite
 int ua_serve_run(void* s, int n, const int* dataset, const double* arrival_s, int slots, int device,
                  int fetch, double* latency_ms, uint64_t* checksums, double* wall_s);
 int ua_serve_destroy(void* s);
+/* The instance that owns dataset `ds`'s inputs (requests run on its clones, never on it), or NULL:
+ * ua_mm_* / ua_if_* / ua_pr_* handle by the dataset's kind, for checking requests' outputs. */
+void* ua_serve_dataset(void* s, int ds);
+/* The output hash that ua_*_fetch and ua_serve_run report (FNV-1a-64 over 8-byte words, then
+ * the tail bytes). */
+uint64_t ua_hash(const void* p, uint64_t n);
 
 #ifdef __cplusplus
 }

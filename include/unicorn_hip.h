@@ -151,6 +151,36 @@ int uh_pagerank_pull_noalloc(const uh_pagerank_args* args, void* stream);
 /* w[i] = (damping * x0) / deg[i] (or 0 for dangling pages) for the first iteration. */
 int uh_pagerank_init_w(float* w, const uint32_t* deg, uint32_t n, float x0, float damping,
                        void* stream);
+
+/* ---- pageRank over the binned (propagation-blocked) layout (replaces pageRankCUDA.cu:46-164) ----
+ * Pages form units of at most UH_PRB_UNIT_MAX consecutive pages (unit_start, K+1 entries); a unit
+ * is a destination bin and a source chunk. Link values (the contribution w[src] of each link)
+ * are stored bin-major: bin b's links occupy [bin_start[b], bin_start[b+1]) of values and dst16
+ * (bin_start entries are multiples of 4; padding slots hold value 0, destination 0), grouped by
+ * source chunk. A run is the links of one chunk into one bin: the runs of chunk c are
+ * [run_ptr[c], run_ptr[c+1]); run r takes its sources from src16[run_off[r] .. run_off[r+1]) and
+ * its values go to values[run_pos[r] ..]. dst16 = destination - unit_start[b], src16 = source -
+ * unit_start[c]. For units [unit0, unit0 + units):
+ *   kind 0 (first iteration): y = x0;  kind 1: y[t] = sum of the values of bin t's links, added
+ *   as floor(v * 2^scale_exp) in 64-bit integers and converted back (order-independent);
+ *   w[t] = deg[t] ? (damping * y[t]) / deg[t] : 0 for the unit's pages, and, if values_out is not
+ *   NULL, values_out[run_pos[r] + k] = w[unit_start[c] + src16[run_off[r] + k]] for the unit's runs.
+ * scale_exp must keep every row sum below 2^62 (damping <= 1: scale_exp <= 62 - ceil(log2(N x0))).
+ * Pointers are indexed globally (page id, link slot, run id); values_in must be 16-byte and dst16
+ * 8-byte aligned at index 0. values_in / dst16 (run_pos / src16) are read only for the units'
+ * slots (runs) and may be NULL when there are none. */
+#define UH_PRB_UNIT_MAX 16384
+typedef struct uh_pagerank_binned_args {
+    const uint32_t* unit_start; const uint32_t* bin_start;
+    const uint32_t* run_ptr; const uint32_t* run_off; const uint32_t* run_pos;
+    const uint16_t* dst16; const uint16_t* src16;
+    const uint32_t* deg;
+    const float* values_in; float* values_out; float* y;
+    uint32_t unit0, units;
+    float damping, x0;
+    int32_t kind, scale_exp;
+} uh_pagerank_binned_args;
+int uh_pagerank_binned(const uh_pagerank_binned_args* args, void* stream);
 /* Push formulation, as the reference callback: y[t-1] += (damping*x[src])/deg over a padded
  * record slice (count, t1..t_count, 0-pad to max_out) for pages [0, pages); float atomics. */
 int uh_pagerank_push_ell(const uint32_t* records, uint32_t pages, uint32_t max_out,
@@ -217,6 +247,14 @@ int uh_batch_flush(void* batch_handle, int use_graph);
  * the pending ones and returns the accumulated milliseconds and the number of flushes timed. */
 int uh_batch_set_timing(void* batch_handle, int every);
 int uh_batch_timing(void* batch_handle, double* total_ms, uint64_t* flushes, int reset);
+/* The timed flushes that launched one kernel family only: their milliseconds, count and
+ * algorithmic work (UH_FAMILY_GEMM: flop issued, 2 M N K per product, 3x for split-BF16;
+ * UH_FAMILY_FILTER: bytes, 2 * rows * cols * 3 per item; other families 0). reset clears that
+ * family's totals; uh_batch_timing's reset clears every family's. */
+enum { UH_FAMILY_FILTER = 0, UH_FAMILY_GEMM = 1, UH_FAMILY_PAGERANK = 2, UH_FAMILY_TRSM = 3,
+       UH_FAMILY_PAGERANK_BLOCKED = 4, UH_FAMILY_PAGERANK_BINNED = 5 };
+int uh_batch_timing_family(void* batch_handle, int family, double* ms, uint64_t* flushes, double* work,
+                           int reset);
 /* Launch counters (kernels launched through this library since load). */
 int uh_launch_counters(uint64_t* plain_launches, uint64_t* persistent_launches, uint64_t* items);
 

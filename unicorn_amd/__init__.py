@@ -86,6 +86,7 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_set_kernel_timing": (i32, [i32]),
         "ua_kernel_time": (i32, [C.POINTER(dbl), C.POINTER(C.c_uint64), i32]),
         "ua_device_sync": (i32, []),
+        "ua_kernel_time_family": (i32, [i32, C.POINTER(dbl), C.POINTER(C.c_uint64), C.POINTER(dbl), i32]),
         "ua_if_create": (vp, [C.POINTER(ua_if_config)]), "ua_if_set_input": (i32, [vp, vp]),
         "ua_if_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_if_get_output": (i32, [vp, vp]),
         "ua_if_destroy": (i32, [vp]),
@@ -102,7 +103,8 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_pr_block_counts": (i32, [C.c_uint32, vp, vp, vp]),
         "ua_pr_graph_bytes": (C.c_uint64, [C.c_uint32, C.c_uint64, i32]),
         "ua_pr_build_graph": (i32, [C.c_uint32, vp, vp, vp, vp]),
-        "ua_pr_set_layout": (None, [i32]), "ua_pr_blocked": (i32, [vp]),
+        "ua_pr_set_layout": (None, [i32]), "ua_pr_blocked": (i32, [vp]), "ua_pr_layout": (i32, [vp]),
+        "ua_prb_layout": (C.c_uint64, [C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, vp, vp]),
         "ua_webdump_meta": (i32, [C.c_char_p, vp, vp, vp]),
         "ua_webdump_read": (i32, [C.c_char_p, vp, vp, C.c_uint64, vp]),
         "ua_webdump_write": (i32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]),
@@ -123,6 +125,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_serve_add_pr": (i32, [vp, C.c_uint32, C.c_uint32, i32, C.c_uint64]),
         "ua_serve_run": (i32, [vp, i32, vp, vp, i32, i32, i32, vp, vp, C.POINTER(dbl)]),
         "ua_serve_destroy": (i32, [vp]),
+        "ua_serve_dataset": (vp, [vp, i32]), "ua_hash": (C.c_uint64, [vp, C.c_uint64]),
+        "ua_mm_get_inputs": (i32, [vp, vp, vp]), "ua_if_get_input": (i32, [vp, vp]),
+        "ua_if_get_config": (i32, [vp, C.POINTER(ua_if_config)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -187,6 +192,19 @@ def kernel_time(reset: bool = False):
     ms, n = C.c_double(0), C.c_uint64(0)
     _check(apps_lib().ua_kernel_time(C.byref(ms), C.byref(n), 1 if reset else 0), "pmGetHipKernelTime")
     return ms.value, int(n.value)
+
+
+# kernel families of the batched flushes (UH_FAMILY_* of unicorn_hip.h)
+FAMILY_FILTER, FAMILY_GEMM, FAMILY_PAGERANK, FAMILY_TRSM, FAMILY_PAGERANK_BLOCKED = 0, 1, 2, 3, 4
+
+
+def kernel_time_family(family: int, reset: bool = False):
+    """(milliseconds, flushes, algorithmic work) of the timed BATCHED flushes that launched only
+    `family`'s kernel: work is flop issued (GEMM), bytes moved (filter) or 0."""
+    ms, n, w = C.c_double(0), C.c_uint64(0), C.c_double(0)
+    _check(apps_lib().ua_kernel_time_family(family, C.byref(ms), C.byref(n), C.byref(w), 1 if reset else 0),
+           "pmGetHipKernelTimeByFamily")
+    return ms.value, int(n.value), w.value
 
 
 def device_sync() -> None:
@@ -287,6 +305,11 @@ class PageRank:
         """True if the graph is held in the source-blocked layout."""
         return bool(apps_lib().ua_pr_blocked(self._h))
 
+    @property
+    def layout(self) -> str:
+        """'csr', 'blocked' or 'binned': the graph layout this instance holds."""
+        return PR_LAYOUTS[apps_lib().ua_pr_layout(self._h)]
+
     def run(self, iterations: int, device: int = DEVICE_GPU) -> float:
         t = C.c_double(0.0)
         _check(apps_lib().ua_pr_run(self._h, iterations, device, C.byref(t)), "pageRank tasks")
@@ -384,10 +407,43 @@ def webdump_write(path: str, row_ptr: np.ndarray, col: np.ndarray, max_out: int,
                                        _ptr(row_ptr), _ptr(col)), "ua_webdump_write")
 
 
-def pr_set_layout(blocked: bool) -> None:
-    """Layout of the pageRank graphs created from now on (source-blocked where possible, or the
-    plain in-link CSR)."""
-    apps_lib().ua_pr_set_layout(1 if blocked else 0)
+PR_LAYOUTS = ("csr", "blocked", "binned")
+
+
+def pr_set_layout(layout) -> None:
+    """Layout of the pageRank graphs created from now on: 'binned' (the default), 'blocked' or
+    'csr' (True / False mean 'blocked' / 'csr'); a layout that cannot hold a graph falls back to
+    the next one (binned -> blocked -> csr)."""
+    if layout is True:
+        layout = "blocked"
+    elif layout is False:
+        layout = "csr"
+    apps_lib().ua_pr_set_layout(PR_LAYOUTS.index(layout))
+
+
+def pr_binned_layout(row_ptr: np.ndarray, col: np.ndarray, pages_per_subtask: int, cus: int = 256):
+    """The binned graph address space built from an out-link CSR, as numpy views: a dict with
+    units, runs, slots, subtasks and the arrays deg, unit_start, bin_start, run_ptr, run_off,
+    run_pos, dst16, src16; None when the layout cannot hold the graph."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col if col.size else np.zeros(1, np.uint32), dtype=np.uint32)
+    n = row_ptr.size - 1
+    L = apps_lib()
+    info = np.zeros(12, dtype=np.uint64)
+    nbytes = int(L.ua_prb_layout(n, _ptr(row_ptr), _ptr(col), pages_per_subtask, cus, None, _ptr(info)))
+    if nbytes == 0:
+        return None
+    g = np.zeros(nbytes, dtype=np.uint8)
+    if int(L.ua_prb_layout(n, _ptr(row_ptr), _ptr(col), pages_per_subtask, cus, _ptr(g), _ptr(info))) != nbytes:
+        raise RuntimeError("ua_prb_layout: build failed")
+    K, R, S, subtasks = (int(v) for v in info[:4])
+    off = [int(v) for v in info[4:]]
+    u32 = lambda o, n_: g[o:o + 4 * n_].view(np.uint32)
+    u16 = lambda o, n_: g[o:o + 2 * n_].view(np.uint16)
+    return {"units": K, "runs": R, "slots": S, "subtasks": subtasks,
+            "deg": u32(off[0], n), "unit_start": u32(off[1], K + 1), "bin_start": u32(off[2], K + 1),
+            "run_ptr": u32(off[3], K + 1), "run_off": u32(off[4], R + 1), "run_pos": u32(off[5], R),
+            "dst16": u16(off[6], S), "src16": u16(off[7], int(row_ptr[-1]))}
 
 
 # Row group size and source block count of the source-blocked pageRank layout
@@ -552,6 +608,52 @@ class Server:
                "serving run")
         return lat, cs, wall.value
 
+    def _base(self, ds: int):
+        h = apps_lib().ua_serve_dataset(self._h, ds)
+        if not h:
+            _check(-1, "ua_serve_dataset")
+        return h
+
+    def mm_inputs(self, ds: int):
+        """(A, B) of a matrixMultiply dataset."""
+        n = self.kinds[ds][1]
+        A = np.empty((n, n), dtype=np.float32)
+        B = np.empty((n, n), dtype=np.float32)
+        _check(apps_lib().ua_mm_get_inputs(self._base(ds), _ptr(A), _ptr(B)), "ua_mm_get_inputs")
+        return A, B
+
+    def if_input(self, ds: int):
+        """(image rows bottom-up, weights, radius, mode, shift) of an imageFiltering dataset."""
+        cfg = ua_if_config()
+        h = self._base(ds)
+        _check(apps_lib().ua_if_get_config(h, C.byref(cfg)), "ua_if_get_config")
+        img = np.empty((cfg.height, cfg.width, 3), dtype=np.uint8)
+        _check(apps_lib().ua_if_get_input(h, _ptr(img)), "ua_if_get_input")
+        d = 2 * cfg.radius + 1
+        w = np.array([cfg.weights[i] for i in range(d * d)], dtype=np.int8)
+        return img, w, cfg.radius, cfg.mode, cfg.shift
+
+    def run_alone(self, ds: int, device: int = DEVICE_GPU, iterations: int = 20) -> np.ndarray:
+        """Runs one task of dataset `ds` on the instance that owns its inputs (which no request
+        uses) and returns its output: C, the filtered image, or the ranks."""
+        kind, size = self.kinds[ds]
+        h = self._base(ds)
+        L = apps_lib()
+        t = C.c_double(0.0)
+        if kind == "mm":
+            _check(L.ua_mm_run(h, device, C.byref(t)), "matrixMultiply task")
+            out = np.empty((size, size), dtype=np.float32)
+            _check(L.ua_mm_get_output(h, _ptr(out)), "ua_mm_get_output")
+        elif kind == "if":
+            _check(L.ua_if_run(h, device, C.byref(t)), "imageFilter task")
+            out = np.empty((size, size, 3), dtype=np.uint8)
+            _check(L.ua_if_get_output(h, _ptr(out)), "ua_if_get_output")
+        else:
+            _check(L.ua_pr_run(h, iterations, device, C.byref(t)), "pageRank tasks")
+            out = np.empty(size, dtype=np.float32)
+            _check(L.ua_pr_get_ranks(h, _ptr(out)), "ua_pr_get_ranks")
+        return out
+
     def close(self) -> None:
         if self._h:
             apps_lib().ua_serve_destroy(self._h)
@@ -562,6 +664,12 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def output_hash(a: np.ndarray) -> int:
+    """The hash that the serving driver reports for a task output (ua_hash)."""
+    a = np.ascontiguousarray(a)
+    return int(apps_lib().ua_hash(a.ctypes.data, a.nbytes))
 
 
 def serving_stream(seed: int, n: int, rate_per_s: float, catalogue: dict, mix=(1, 1, 1)):

@@ -97,13 +97,34 @@ def build_all(verbose: bool = True) -> None:
     _link(LIB / "libpmlib.so", CXX, rt_o, rpath + ["-lunicorn_hip", "-pthread"])
     if app_o:
         _link(LIB / "libunicorn_apps.so", CXX, app_o, rpath + ["-lpmlib", "-lunicorn_hip", "-pthread"])
-    # drop stale objects
+    # drop stale objects (test-support objects are named after their own sources)
     keep = {p.name for p in hip_o + rt_o + app_o}
+    test_stems = {p.stem for p in TEST_NATIVE.glob("*.hip")} if TEST_NATIVE.exists() else set()
     for p in OBJ.glob("*.o"):
-        if p.name not in keep:
+        if p.name not in keep and p.name.rsplit("-", 1)[0] not in test_stems:
             p.unlink()
     if verbose:
         print(f"built {LIB}/libunicorn_hip.so libpmlib.so libunicorn_apps.so", flush=True)
+
+
+TEST_NATIVE = ROOT / "tests" / "native"
+
+
+def build_test_support(verbose: bool = True) -> None:
+    """Test-support libraries of tests/native (HIP sources written against the pm:: API, e.g. the
+    kernel-kind contract probe), one shared library per source: tests/native/lib<stem>.so."""
+    srcs = sorted(TEST_NATIVE.glob("*.hip"))
+    if not srcs:
+        return
+    OBJ.mkdir(parents=True, exist_ok=True)
+    deps = _headers(INC)
+    rpath = ["-Wl,-rpath,$ORIGIN/../../unicorn_amd/lib", f"-L{LIB}", "-Wl,--no-undefined"]
+    for src in srcs:
+        obj = _compile(src, HIPCC, HIP_FLAGS, deps)
+        _link(TEST_NATIVE / f"lib{src.stem}.so", HIPCC, [obj],
+              [f"--offload-arch={ARCH}", "-fPIC"] + rpath + ["-lpmlib", "-lunicorn_hip"])
+    if verbose:
+        print(f"built test support in {TEST_NATIVE}", flush=True)
 
 
 def clean() -> None:
@@ -115,3 +136,4 @@ if __name__ == "__main__":
         clean()
     else:
         build_all()
+        build_test_support()

@@ -72,6 +72,10 @@ int ua_kernel_time(double* ms, uint64_t* flushes, int reset) {
     return (int)pmGetHipKernelTime(ms, flushes, reset != 0);
 }
 
+int ua_kernel_time_family(int family, double* ms, uint64_t* flushes, double* work, int reset) {
+    return (int)pmGetHipKernelTimeByFamily(family, ms, flushes, work, reset != 0);
+}
+
 int ua_device_sync(void) { return pmHasHipDevice() ? uh_device_sync() : 0; }
 
 const char* ua_last_error(void) {

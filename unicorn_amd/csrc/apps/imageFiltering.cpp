@@ -283,6 +283,33 @@ extern "C" int ua_if_fetch(void* h, uint64_t* checksum) {
     return apps::fetch_output(in->out, in->bytes, checksum);
 }
 
+extern "C" int ua_if_get_input(void* h, uint8_t* image) {
+    Instance* in = (Instance*)h;
+    if (!in || !image) return -1;
+    pmStatus s = pmFetchMemory(in->in);
+    pmRawMemPtr p;
+    if (s == pmSuccess) s = pmGetRawMemPtr(in->in, &p);
+    if (s != pmSuccess) return (int)s;
+    memcpy(image, p, in->bytes);
+    return 0;
+}
+
+extern "C" int ua_if_get_config(void* h, ua_if_config* cfg) {
+    const Instance* in = (const Instance*)h;
+    if (!in || !cfg) return -1;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->width = (int32_t)in->conf.imageWidth;
+    cfg->height = (int32_t)in->conf.imageHeight;
+    cfg->radius = (int32_t)in->conf.filterRadius;
+    cfg->mode = in->conf.arithmeticMode;
+    cfg->shift = in->conf.rightShift;
+    cfg->tile = (int32_t)in->conf.tileDim;
+    const int d = 2 * cfg->radius + 1;
+    for (int y = 0; y < d; ++y)
+        for (int x = 0; x < d; ++x) cfg->weights[y * d + x] = (int8_t)in->conf.filter[y][x];
+    return 0;
+}
+
 extern "C" int ua_if_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;

@@ -207,6 +207,22 @@ extern "C" int ua_mm_fetch(void* h, uint64_t* checksum) {
     return apps::fetch_output(in->c, in->conf.matrixDim * in->conf.matrixDim * sizeof(float), checksum);
 }
 
+extern "C" int ua_mm_get_inputs(void* h, float* A, float* B) {
+    Instance* in = (Instance*)h;
+    if (!in || !A || !B) return -1;
+    const size_t bytes = in->conf.matrixDim * in->conf.matrixDim * sizeof(float);
+    pmMemHandle m[2] = {in->a, in->b};
+    float* dst[2] = {A, B};
+    for (int i = 0; i < 2; ++i) {
+        pmStatus s = pmFetchMemory(m[i]);
+        pmRawMemPtr p;
+        if (s == pmSuccess) s = pmGetRawMemPtr(m[i], &p);
+        if (s != pmSuccess) return (int)s;
+        memcpy(dst[i], p, bytes);
+    }
+    return 0;
+}
+
 extern "C" int ua_mm_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;

@@ -25,6 +25,7 @@
 // [s0, s1)), to all of w, and writes y[s0:s1) and w'[s0:s1). An "init" task writes the
 // iteration-0 state (x = 1, w = 0.85 / outdeg).
 #include "app_common.h"
+#include "prbinned.h"
 #include "unicorn_hip.h"
 #include "unicorn_apps.h"
 
@@ -37,6 +38,8 @@ using namespace pm;
 
 enum memIndex { GRAPH_MEM_INDEX = 0, STATE_IN_MEM_INDEX = 1, STATE_OUT_MEM_INDEX = 2 };
 
+enum Layout { LAYOUT_CSR = 0, LAYOUT_BLOCKED = 1, LAYOUT_BINNED = 2 };
+
 struct pageRankTaskConf {
     uint32_t totalWebPages;
     uint32_t webPagesPerSubtask;
@@ -47,6 +50,10 @@ struct pageRankTaskConf {
     const uint32_t* hostRowPtr;      // in-link row pointers on the host (subscription sizing)
     int blocked;                     // graph in the source-blocked layout
     const uint32_t* hostBseg;        // its segment starts on the host (subscription sizing)
+    int layout;                      // Layout
+    int produceNext;                 // binned: write the next iteration's link values
+    int scaleExp;                    // binned: fixed-point exponent of the row sums
+    const prb::Layout* hostBinned;   // binned: host tables (subscription sizing)
 };
 
 static constexpr uint32_t GR = UH_PR_GROUP_ROWS;
@@ -81,8 +88,124 @@ static void range_of(const pageRankTaskConf* c, ulong id, uint32_t* s0, uint32_t
     *s1 = (uint32_t)(e < c->totalWebPages ? e : c->totalWebPages);
 }
 
+// ---- binned layout (prbinned.h): the records a subtask subscribes, and pointers into them ----
+
+static inline size_t down16(size_t v) { return v & ~(size_t)15; }
+static inline size_t up16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+struct Rec { size_t off, len; };
+
+// Read records of the graph for subtask s (each rounded out to 16 bytes, so that a COMPACT view
+// keeps every region's 16-byte alignment), in address order.
+static std::vector<Rec> binned_graph_records(const pageRankTaskConf* c, uint32_t s) {
+    const prb::Layout& L = *c->hostBinned;
+    uint32_t s0, s1;
+    range_of(c, s, &s0, &s1);
+    const uint32_t u0 = L.sub_unit[s], u1 = L.sub_unit[s + 1];
+    std::vector<Rec> r;
+    auto add = [&](size_t off, size_t len) {
+        if (len) r.push_back({down16(off), up16(off + len) - down16(off)});
+    };
+    add(L.off_deg + (size_t)s0 * 4, (size_t)(s1 - s0) * 4);
+    add(L.off_unit + (size_t)u0 * 4, ((size_t)u1 - u0 + 1) * 4);
+    if (c->kind == 1) add(L.off_bin + (size_t)u0 * 4, ((size_t)u1 - u0 + 1) * 4);
+    if (c->produceNext) {
+        const uint32_t r0 = L.run_ptr[u0], r1 = L.run_ptr[u1];
+        add(L.off_rptr + (size_t)u0 * 4, ((size_t)u1 - u0 + 1) * 4);
+        add(L.off_roff + (size_t)r0 * 4, ((size_t)r1 - r0 + 1) * 4);
+        add(L.off_rpos + (size_t)r0 * 4, ((size_t)r1 - r0) * 4);
+    }
+    if (c->kind == 1) add(L.off_dst + (size_t)L.bin_start[u0] * 2, ((size_t)L.bin_start[u1] - L.bin_start[u0]) * 2);
+    if (c->produceNext) {
+        const uint32_t o0 = L.run_off[L.run_ptr[u0]], o1 = L.run_off[L.run_ptr[u1]];
+        add(L.off_src + (size_t)o0 * 2, ((size_t)o1 - o0) * 2);
+    }
+    return r;
+}
+
+// Pointer to address-space offset `off` inside the view of records `recs` (address order).
+static uint8_t* rec_ptr(const pmMemInfo& mi, const std::vector<Rec>& recs, size_t off) {
+    uint8_t* base = (uint8_t*)mi.ptr;
+    if (mi.visibilityType == SUBSCRIPTION_NATURAL) return base + (off - recs.front().off);
+    size_t packed = 0;
+    for (const Rec& r : recs) {
+        if (off >= r.off && off < r.off + r.len) return base + packed + (off - r.off);
+        packed += r.len;
+    }
+    return nullptr;
+}
+
+static void binned_subscribe(pmTaskInfo ti, pmDeviceInfo di, pmSubtaskInfo si, const pageRankTaskConf* c) {
+    const prb::Layout& L = *c->hostBinned;
+    const uint32_t s = (uint32_t)si.subtaskId, S = L.subtasks;
+    uint32_t s0, s1;
+    range_of(c, s, &s0, &s1);
+    for (const Rec& r : binned_graph_records(c, s))
+        pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, GRAPH_MEM_INDEX,
+                            READ_SUBSCRIPTION, pmSubscriptionInfo(r.off, r.len));
+    const uint32_t u0 = L.sub_unit[s], u1 = L.sub_unit[s + 1];
+    if (c->kind == 1 && L.bin_start[u1] > L.bin_start[u0])
+        pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_IN_MEM_INDEX,
+                            READ_SUBSCRIPTION,
+                            pmSubscriptionInfo(L.state_values_off + (size_t)L.bin_start[u0] * 4,
+                                               ((size_t)L.bin_start[u1] - L.bin_start[u0]) * 4));
+    pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_OUT_MEM_INDEX,
+                        WRITE_SUBSCRIPTION, pmSubscriptionInfo((size_t)s0 * 4, (size_t)(s1 - s0) * 4));
+    if (c->produceNext) {
+        // the values of this subtask's chunks: one contiguous range in every bin
+        for (uint32_t b = 0; b < L.units; ++b) {
+            const size_t v0 = L.sub_start[(size_t)b * (S + 1) + s], v1 = L.sub_start[(size_t)b * (S + 1) + s + 1];
+            if (v1 > v0)
+                pmSubscribeToMemory(ti.taskHandle, di.deviceHandle, si.subtaskId, si.splitInfo, STATE_OUT_MEM_INDEX,
+                                    WRITE_SUBSCRIPTION, pmSubscriptionInfo(L.state_values_off + v0 * 4, (v1 - v0) * 4));
+        }
+    }
+}
+
+// Global-indexed pointers of a binned subtask (device or host memory alike).
+static prb::Views binned_views(const pageRankTaskConf* c, const pmSubtaskInfo& si) {
+    const prb::Layout& L = *c->hostBinned;
+    const uint32_t s = (uint32_t)si.subtaskId;
+    const uint32_t u0 = L.sub_unit[s];
+    uint32_t s0, s1;
+    range_of(c, s, &s0, &s1);
+    const std::vector<Rec> recs = binned_graph_records(c, s);
+    const pmMemInfo& g = si.memInfo[GRAPH_MEM_INDEX];
+    prb::Views v;
+    memset(&v, 0, sizeof(v));
+    v.deg = (const uint32_t*)rec_ptr(g, recs, L.off_deg + (size_t)s0 * 4) - s0;
+    v.unit_start = (const uint32_t*)rec_ptr(g, recs, L.off_unit + (size_t)u0 * 4) - u0;
+    if (c->kind == 1) {
+        v.bin_start = (const uint32_t*)rec_ptr(g, recs, L.off_bin + (size_t)u0 * 4) - u0;
+        const uint32_t k0 = L.bin_start[u0];
+        if (L.bin_start[L.sub_unit[s + 1]] > k0) {
+            v.dst16 = (const uint16_t*)rec_ptr(g, recs, L.off_dst + (size_t)k0 * 2) - k0;
+            v.values_in = (const float*)si.memInfo[STATE_IN_MEM_INDEX].ptr - k0;   // one record
+        }
+    }
+    if (c->produceNext) {
+        const uint32_t r0 = L.run_ptr[u0];
+        v.run_ptr = (const uint32_t*)rec_ptr(g, recs, L.off_rptr + (size_t)u0 * 4) - u0;
+        v.run_off = (const uint32_t*)rec_ptr(g, recs, L.off_roff + (size_t)r0 * 4) - r0;
+        if (L.run_ptr[L.sub_unit[s + 1]] > r0) {
+            v.run_pos = (const uint32_t*)rec_ptr(g, recs, L.off_rpos + (size_t)r0 * 4) - r0;
+            const uint32_t o0 = L.run_off[r0];
+            v.src16 = (const uint16_t*)rec_ptr(g, recs, L.off_src + (size_t)o0 * 2) - o0;
+        }
+    }
+    // the output state is subscribed with a NATURAL view: ptr is the offset of y[s0]
+    uint8_t* ob = (uint8_t*)si.memInfo[STATE_OUT_MEM_INDEX].ptr - (size_t)s0 * 4;
+    v.y = (float*)ob;
+    v.values_out = c->produceNext ? (float*)(ob + L.state_values_off) : nullptr;
+    return v;
+}
+
 static pmStatus prDataDistribution(pmTaskInfo ti, pmDeviceInfo di, pmSubtaskInfo si) {
     const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    if (c->layout == LAYOUT_BINNED) {
+        binned_subscribe(ti, di, si, c);
+        return pmSuccess;
+    }
     uint32_t s0, s1;
     range_of(c, si.subtaskId, &s0, &s1);
     const size_t N = c->totalWebPages;
@@ -184,6 +307,16 @@ static Views views_of(const pageRankTaskConf* c, const pmSubtaskInfo& si, uint32
 
 static pmStatus pr_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
     const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    if (c->layout == LAYOUT_BINNED) {
+        // the device units' arithmetic, unit by unit (the same bits as the GPU)
+        const prb::Layout& L = *c->hostBinned;
+        const prb::Views v = binned_views(c, si);
+        std::vector<uint64_t> acc;
+        std::vector<float> w;
+        for (uint32_t u = L.sub_unit[si.subtaskId]; u < L.sub_unit[si.subtaskId + 1]; ++u)
+            prb::unit_cpu(v, u, c->kind, c->dampingFactor, c->initialPageRank, c->scaleExp, acc, w);
+        return pmSuccess;
+    }
     uint32_t s0, s1;
     range_of(c, si.subtaskId, &s0, &s1);
     Views v = views_of(c, si, s0, s1);
@@ -227,6 +360,33 @@ static pmStatus pr_cpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si) {
 
 static pmStatus pr_gpu(pmTaskInfo ti, pmDeviceInfo, pmSubtaskInfo si, void* stream) {
     const pageRankTaskConf* c = (const pageRankTaskConf*)ti.taskConf;
+    if (c->layout == LAYOUT_BINNED) {
+        const prb::Layout& L = *c->hostBinned;
+        const prb::Views v = binned_views(c, si);
+        uh_pagerank_binned_args a;
+        a.unit_start = v.unit_start;
+        a.bin_start = v.bin_start;
+        a.run_ptr = v.run_ptr;
+        a.run_off = v.run_off;
+        a.run_pos = v.run_pos;
+        a.dst16 = v.dst16;
+        a.src16 = v.src16;
+        a.deg = v.deg;
+        a.values_in = v.values_in;
+        a.values_out = v.values_out;
+        a.y = v.y;
+        a.unit0 = L.sub_unit[si.subtaskId];
+        a.units = L.sub_unit[si.subtaskId + 1] - a.unit0;
+        a.damping = c->dampingFactor;
+        a.x0 = c->initialPageRank;
+        a.kind = c->kind;
+        a.scale_exp = c->scaleExp;
+        if (uh_pagerank_binned(&a, stream)) {
+            apps::set_error(std::string("pageRank GPU callback: ") + uh_last_error());
+            return pmGraphicsCardError;
+        }
+        return pmSuccess;
+    }
     uint32_t s0, s1;
     range_of(c, si.subtaskId, &s0, &s1);
     Views v = views_of(c, si, s0, s1);
@@ -262,8 +422,11 @@ struct Instance {
     uint32_t pages = 0, per_subtask = 0;
     uint64_t edges = 0;
     float damping = 0.85f;
-    std::shared_ptr<std::vector<uint32_t>> in_row_ptr;   // host copy for subscription sizing
+    int layout = LAYOUT_CSR;
+    std::shared_ptr<std::vector<uint32_t>> in_row_ptr;   // host copy for subscription sizing (CSR, blocked)
     std::shared_ptr<std::vector<uint32_t>> bseg;         // host copy (blocked layout), or null
+    std::shared_ptr<prb::Layout> binned;                 // host tables (binned layout), or null
+    size_t state_bytes = 0;
     std::shared_ptr<apps::SharedMem> inputs;             // the graph, shared with clones
     pmMemHandle graph = nullptr, state[2] = {nullptr, nullptr};
     int current = 0;                    // state holding the latest ranks
@@ -271,24 +434,33 @@ struct Instance {
 
 static apps::CallbackVariants g_cb;
 static std::mutex g_mtx;
-static int g_use_blocked = 1;       // layout of graphs created from now on
+static int g_layout = LAYOUT_BLOCKED;   // layout of graphs created from now on (falling back when it cannot hold one)
 
-static int run_kind(Instance* in, int kind, int device, double* secs) {
+static const float kInitialPageRank = 1.0f;
+
+static int run_kind(Instance* in, int kind, int device, double* secs, bool produce_next) {
     pageRankTaskConf conf;
     conf.totalWebPages = in->pages;
     conf.webPagesPerSubtask = in->per_subtask;
     conf.totalLinks = in->edges;
     conf.dampingFactor = in->damping;
-    conf.initialPageRank = 1.0f;
+    conf.initialPageRank = kInitialPageRank;
     conf.kind = kind;
-    conf.hostRowPtr = in->in_row_ptr->data();
-    conf.blocked = in->bseg ? 1 : 0;
+    conf.hostRowPtr = in->in_row_ptr ? in->in_row_ptr->data() : nullptr;
+    conf.blocked = in->layout == LAYOUT_BLOCKED ? 1 : 0;
     conf.hostBseg = in->bseg ? in->bseg->data() : nullptr;
+    conf.layout = in->layout;
+    conf.produceNext = produce_next ? 1 : 0;
+    conf.scaleExp = prb::scale_exp(in->pages, kInitialPageRank);
+    conf.hostBinned = in->binned.get();
     const int src = in->current, dst = kind == 0 ? in->current : 1 - in->current;
     pmTaskMem mem[3];
     mem[GRAPH_MEM_INDEX] = pmTaskMem(in->graph, READ_ONLY, SUBSCRIPTION_OPTIMAL);
     mem[STATE_IN_MEM_INDEX] = pmTaskMem(in->state[kind == 0 ? 1 - dst : src], READ_ONLY, SUBSCRIPTION_OPTIMAL);
-    mem[STATE_OUT_MEM_INDEX] = pmTaskMem(in->state[dst], WRITE_ONLY, SUBSCRIPTION_OPTIMAL);
+    // binned: a subtask writes its y and one range of link values in every bin; kernels index them
+    // by page and by link slot, which only a NATURAL view keeps linear
+    mem[STATE_OUT_MEM_INDEX] = pmTaskMem(in->state[dst], WRITE_ONLY,
+                                         in->layout == LAYOUT_BINNED ? SUBSCRIPTION_NATURAL : SUBSCRIPTION_OPTIMAL);
     const ulong subtasks = (in->pages + in->per_subtask - 1) / in->per_subtask;
     pmTaskDetails d(&conf, sizeof(conf), mem, 3, g_cb.h[device], subtasks);
     int rc = apps::run_task(d, secs);
@@ -377,8 +549,53 @@ extern "C" int ua_pr_build_graph(uint32_t pages, const uint32_t* out_row_ptr, co
 
 // Layout of the graphs that ua_pr_create builds from now on: 1 = source-blocked where possible
 // (the default), 0 = plain in-link CSR.
-extern "C" void ua_pr_set_layout(int blocked) { g_use_blocked = blocked ? 1 : 0; }
-extern "C" int ua_pr_blocked(void* h) { return ((Instance*)h)->bseg ? 1 : 0; }
+extern "C" void ua_pr_set_layout(int layout) {
+    g_layout = layout == LAYOUT_CSR ? LAYOUT_CSR : layout == LAYOUT_BLOCKED ? LAYOUT_BLOCKED : LAYOUT_BINNED;
+}
+extern "C" int ua_pr_blocked(void* h) { return ((Instance*)h)->layout == LAYOUT_BLOCKED ? 1 : 0; }
+extern "C" int ua_pr_layout(void* h) { return ((Instance*)h)->layout; }
+
+// The binned layout outside a task (for inspection): with graph == NULL, plans and returns the
+// graph address space's bytes and fills info = {units, runs, slots, subtasks, region offsets
+// (deg, unit_start, bin_start, run_ptr, run_off, run_pos, dst16, src16)}; with a graph buffer of
+// that size, also fills it. Returns 0 when the layout cannot hold the graph.
+extern "C" uint64_t ua_prb_layout(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
+                                  uint32_t pages_per_subtask, uint32_t cus, void* graph, uint64_t* info) {
+    prb::Layout L;
+    if (!pages || !out_row_ptr || !pages_per_subtask ||
+        !prb::plan(pages, out_row_ptr, out_col, pages_per_subtask, cus, L))
+        return 0;
+    if (info) {
+        const uint64_t v[12] = {L.units, L.runs, L.slots, L.subtasks, L.off_deg, L.off_unit, L.off_bin,
+                                L.off_rptr, L.off_roff, L.off_rpos, L.off_dst, L.off_src};
+        memcpy(info, v, sizeof(v));
+    }
+    if (graph && prb::build(out_row_ptr, out_col, L, (uint8_t*)graph) != 0) return 0;
+    return L.graph_bytes;
+}
+
+// Creates the two rank-state address spaces of an instance, zeroed (the binned layout's padding
+// slots of the link values are read as zeros and never written).
+static bool create_states(Instance* in) {
+    for (int i = 0; i < 2; ++i) {
+        if (pmCreateMemory(in->state_bytes, &in->state[i]) != pmSuccess) {
+            in->state[i] = nullptr;
+            return false;
+        }
+        pmRawMemPtr p;
+        if (pmGetRawMemPtr(in->state[i], &p) != pmSuccess) return false;
+        memset(p, 0, in->state_bytes);
+    }
+    return true;
+}
+
+// CUs of the device the units are planned for (whole rounds of units per flush).
+static uint32_t planning_cus() {
+    int cus = 0;
+    if (pmHasHipDevice() && uh_device_props(0, nullptr, 0, nullptr, &cus, nullptr, nullptr, nullptr) == 0 && cus > 0)
+        return (uint32_t)cus;
+    return 256;
+}
 
 // out_row_ptr / out_col: out-link CSR (0-based targets), e.g. from ua_webgen.
 extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const uint32_t* out_col,
@@ -404,21 +621,44 @@ extern "C" void* ua_pr_create(uint32_t pages, const uint32_t* out_row_ptr, const
     in->damping = damping;
     in->edges = out_row_ptr[pages];
     const size_t N = pages;
+    in->inputs = std::make_shared<apps::SharedMem>();
+    // binned where it can hold the graph (its fixed-point sums assume damping <= 1)
+    if (g_layout == LAYOUT_BINNED && damping >= 0.f && damping <= 1.f && in->edges < (1ull << 32)) {
+        auto L = std::make_shared<prb::Layout>();
+        if (prb::plan(pages, out_row_ptr, out_col, pages_per_subtask, planning_cus(), *L)) {
+            in->layout = LAYOUT_BINNED;
+            in->binned = L;
+            in->state_bytes = L->state_bytes;
+            pmRawMemPtr p;
+            if (pmCreateMemory(L->graph_bytes, &in->graph) != pmSuccess) in->graph = nullptr;
+            in->inputs->handles.push_back(in->graph);
+            if (!in->graph || !create_states(in) || pmGetRawMemPtr(in->graph, &p) != pmSuccess) {
+                apps::set_error(std::string("create memory: ") + pmGetLastError());
+                ua_pr_destroy(in);
+                return nullptr;
+            }
+            if (prb::build(out_row_ptr, out_col, *L, (uint8_t*)p) != 0) {
+                apps::set_error("ua_pr_create: binned layout does not add up");
+                ua_pr_destroy(in);
+                return nullptr;
+            }
+            return in;
+        }
+    }
     std::vector<uint8_t> bcnt;
     bool blocked = false;
-    if (g_use_blocked && in->edges < (1ull << 32)) {
+    if (g_layout != LAYOUT_CSR && in->edges < (1ull << 32)) {
         bcnt.resize((size_t)groups_of(pages) * NB * GR);
         blocked = ua_pr_block_counts(pages, out_row_ptr, out_col, bcnt.data()) == 1;
     }
+    in->layout = blocked ? LAYOUT_BLOCKED : LAYOUT_CSR;
+    in->state_bytes = 2 * N * 4;
     const size_t graph_bytes = ua_pr_graph_bytes(pages, in->edges, blocked);
-    in->inputs = std::make_shared<apps::SharedMem>();
     if (pmCreateMemory(graph_bytes, &in->graph) != pmSuccess) in->graph = nullptr;
     in->inputs->handles.push_back(in->graph);
-    if (!in->graph || pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
-        pmCreateMemory(2 * N * 4, &in->state[1]) != pmSuccess) {
+    if (!in->graph || !create_states(in)) {
         apps::set_error(std::string("create memory: ") + pmGetLastError());
-        if (in->state[0]) pmReleaseMemory(in->state[0]);
-        delete in;
+        ua_pr_destroy(in);
         return nullptr;
     }
     pmRawMemPtr p;
@@ -446,16 +686,16 @@ extern "C" void* ua_pr_clone(void* h) {
     in->per_subtask = src->per_subtask;
     in->edges = src->edges;
     in->damping = src->damping;
+    in->layout = src->layout;
     in->in_row_ptr = src->in_row_ptr;
     in->bseg = src->bseg;
+    in->binned = src->binned;
+    in->state_bytes = src->state_bytes;
     in->inputs = src->inputs;
     in->graph = src->graph;
-    const size_t N = src->pages;
-    if (pmCreateMemory(2 * N * 4, &in->state[0]) != pmSuccess ||
-        pmCreateMemory(2 * N * 4, &in->state[1]) != pmSuccess) {
+    if (!create_states(in)) {
         apps::set_error(std::string("create memory: ") + pmGetLastError());
-        if (in->state[0]) pmReleaseMemory(in->state[0]);
-        delete in;
+        ua_pr_destroy(in);
         return nullptr;
     }
     return in;
@@ -466,10 +706,10 @@ extern "C" int ua_pr_run(void* h, int iterations, int device, double* seconds) {
     Instance* in = (Instance*)h;
     if (device < 0 || device > 2 || iterations < 0) return -1;
     double t = 0, s = 0;
-    int rc = run_kind(in, 0, device, &s);
+    int rc = run_kind(in, 0, device, &s, iterations > 0);
     t += s;
     for (int i = 0; i < iterations && rc == 0; ++i) {
-        rc = run_kind(in, 1, device, &s);
+        rc = run_kind(in, 1, device, &s, i + 1 < iterations);
         t += s;
     }
     if (seconds) *seconds = t;
@@ -499,8 +739,8 @@ extern "C" uint64_t ua_pr_edges(void* h) { return ((Instance*)h)->edges; }
 extern "C" int ua_pr_destroy(void* h) {
     Instance* in = (Instance*)h;
     if (!in) return 0;
-    pmReleaseMemory(in->state[0]);
-    pmReleaseMemory(in->state[1]);
+    if (in->state[0]) pmReleaseMemory(in->state[0]);
+    if (in->state[1]) pmReleaseMemory(in->state[1]);
     delete in;
     return 0;
 }

@@ -223,6 +223,14 @@ int ua_serve_run(void* h, int n, const int* dataset, const double* arrival_s, in
     return failed.load();
 }
 
+void* ua_serve_dataset(void* h, int ds) {
+    Server* s = (Server*)h;
+    if (!s || ds < 0 || ds >= (int)s->ds.size()) return nullptr;
+    return s->ds[ds]->base;
+}
+
+uint64_t ua_hash(const void* p, uint64_t n) { return apps::hash_bytes(p, (size_t)n); }
+
 int ua_serve_destroy(void* h) {
     Server* s = (Server*)h;
     if (!s) return 0;

@@ -339,6 +339,7 @@ int uh_batch_create(void** handle, void* stream) {
     (void)cu_count();
     (void)filter_wg_per_cu();
     (void)gemm_wg_per_cu();
+    if (int rc = pagerank_binned_prepare()) return rc;
     Batch* b = new Batch();
     b->stream = (hipStream_t)stream;
     b->slot_bytes = 0;
@@ -375,7 +376,7 @@ int uh_batch_pending(void* handle, int* n) {
 
 static const size_t kItemBytes[FAM_COUNT] = {
     sizeof(FilterItem), sizeof(uh_gemm_args), sizeof(uh_pagerank_args), sizeof(TrsmItem),
-    sizeof(uh_pagerank_args)};
+    sizeof(uh_pagerank_args), sizeof(uh_pagerank_binned_args)};
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -397,6 +398,23 @@ __global__ void __launch_bounds__(256) k_upload_queue(uint32_t* dst, const uint3
     uint32_t* d = dst + r.off[f];
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
         d[i] = __hip_atomic_load(s + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Algorithmic work of a flush that launches one family: GEMM items issue 2 M N K flop per
+// product (times 3 for the split-BF16 products), filter items move 2 * rows * cols * 3 bytes
+// (input and output once); other families report 0 (their callers know their work).
+static double flush_work(const Batch* b, int mask) {
+    if (__builtin_popcount(mask) != 1) return 0.0;
+    double w = 0.0;
+    if (mask == 1 << FAM_GEMM) {
+        const uh_gemm_args* g = (const uh_gemm_args*)b->items[FAM_GEMM].data();
+        for (size_t i = 0; i < b->tiles[FAM_GEMM].size(); ++i)
+            w += 2.0 * g[i].M * g[i].N * g[i].K * ((g[i].precision & 0xff) == 0 ? 3.0 : 1.0);   // bits 8+: tile
+    } else if (mask == 1 << FAM_FILTER) {
+        const FilterItem* f = (const FilterItem*)b->items[FAM_FILTER].data();
+        for (size_t i = 0; i < b->tiles[FAM_FILTER].size(); ++i) w += 6.0 * f[i].a.rows * f[i].a.cols;
+    }
+    return w;
 }
 
 // Layout of one upload: per family a QueueHeader block at a fixed offset of the slot
@@ -492,6 +510,7 @@ int uh_batch_flush(void* handle, int use_graph) {
         if ((mask >> FAM_PAGERANK) & 1) rc |= pagerank_persistent(dev + FAM_PAGERANK * family_stride, s);
         if ((mask >> FAM_TRSM) & 1) rc |= trsm_persistent(dev + FAM_TRSM * family_stride, s);
         if ((mask >> FAM_PRBLK) & 1) rc |= pagerank_blocked_persistent(dev + FAM_PRBLK * family_stride, s);
+        if ((mask >> FAM_PRBIN) & 1) rc |= pagerank_binned_persistent(dev + FAM_PRBIN * family_stride, s);
         return rc;
     };
     int rc;
@@ -533,11 +552,31 @@ int uh_batch_flush(void* handle, int use_graph) {
     }
     if (ev.first) {
         UH_HIP(hipEventRecord(ev.second, b->stream));
-        b->timed.push_back(ev);
+        b->timed.push_back({ev.first, ev.second, (uint32_t)mask, flush_work(b, mask)});
     }
     UH_HIP(hipEventRecord(b->slot_free[pos], b->stream));
     for (int f = 0; f < FAM_COUNT; ++f) { b->items[f].clear(); b->tiles[f].clear(); b->kinds[f] = 0; }
     return rc;
+}
+
+// Moves the finished timed flushes into the totals (caller holds tmtx).
+static int drain_timed(Batch* b) {
+    for (auto& t : b->timed) {
+        UH_HIP(hipEventSynchronize(t.stop));
+        float ms = 0.f;
+        UH_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+        b->timed_ms += ms;
+        b->timed_flushes++;
+        if (__builtin_popcount(t.mask) == 1) {
+            const int f = __builtin_ctz(t.mask);
+            b->fam_ms[f] += ms;
+            b->fam_work[f] += t.work;
+            b->fam_flushes[f]++;
+        }
+        b->spare.push_back({t.start, t.stop});
+    }
+    b->timed.clear();
+    return UH_OK;
 }
 
 int uh_batch_set_timing(void* handle, int on) {
@@ -553,18 +592,27 @@ int uh_batch_timing(void* handle, double* total_ms, uint64_t* flushes, int reset
     Batch* b = as_batch(handle);
     if (!b) return fail(UH_ERR_ARG, "uh_batch_timing: not a batch");
     std::lock_guard<std::mutex> tlock(b->tmtx);
-    for (auto& ev : b->timed) {
-        UH_HIP(hipEventSynchronize(ev.second));
-        float ms = 0.f;
-        UH_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
-        b->timed_ms += ms;
-        b->timed_flushes++;
-        b->spare.push_back(ev);
-    }
-    b->timed.clear();
+    if (int rc = drain_timed(b)) return rc;
     if (total_ms) *total_ms = b->timed_ms;
     if (flushes) *flushes = b->timed_flushes;
-    if (reset) { b->timed_ms = 0; b->timed_flushes = 0; }
+    if (reset) {
+        b->timed_ms = 0;
+        b->timed_flushes = 0;
+        for (int f = 0; f < FAM_COUNT; ++f) { b->fam_ms[f] = 0; b->fam_work[f] = 0; b->fam_flushes[f] = 0; }
+    }
+    return UH_OK;
+}
+
+int uh_batch_timing_family(void* handle, int family, double* ms, uint64_t* flushes, double* work, int reset) {
+    Batch* b = as_batch(handle);
+    if (!b) return fail(UH_ERR_ARG, "uh_batch_timing_family: not a batch");
+    if (family < 0 || family >= FAM_COUNT) return fail(UH_ERR_ARG, "uh_batch_timing_family: no family %d", family);
+    std::lock_guard<std::mutex> tlock(b->tmtx);
+    if (int rc = drain_timed(b)) return rc;
+    if (ms) *ms = b->fam_ms[family];
+    if (flushes) *flushes = b->fam_flushes[family];
+    if (work) *work = b->fam_work[family];
+    if (reset) { b->fam_ms[family] = 0; b->fam_work[family] = 0; b->fam_flushes[family] = 0; }
     return UH_OK;
 }
 

@@ -45,7 +45,8 @@ int cu_count();
 
 // Kernel families that a batch can hold. Each family records fixed-size descriptors.
 // FAM_PRBLK holds pageRank items over the source-blocked layout; their tiles are row groups.
-enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_PRBLK = 4, FAM_COUNT = 5 };
+enum Family { FAM_FILTER = 0, FAM_GEMM = 1, FAM_PAGERANK = 2, FAM_TRSM = 3, FAM_PRBLK = 4, FAM_PRBIN = 5,
+              FAM_COUNT = 6 };
 
 // Work item of the filter family: the launcher's arguments (pad = view width in pixels) plus the
 // dot4 weight vectors of radius <= 2, packed once on the host (wv[dy][q][d], see imagefilter.hip).
@@ -103,9 +104,18 @@ struct Batch {
     uint32_t timing = 0;          // events around every timing-th flush (0: none)
     uint64_t timing_seq = 0;      // flushes since timing was set
     std::mutex tmtx;                 // guards timed / spare / the totals
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed, spare;
+    struct Timed {
+        hipEvent_t start, stop;
+        uint32_t mask;                   // families the flush launched
+        double work;                     // algorithmic work of a one-family flush (see uh_batch_timing_family)
+    };
+    std::vector<Timed> timed;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
     double timed_ms = 0;
     uint64_t timed_flushes = 0;
+    // flushes that launched one family only, per family
+    double fam_ms[FAM_COUNT] = {}, fam_work[FAM_COUNT] = {};
+    uint64_t fam_flushes[FAM_COUNT] = {};
     // partial sums of the FAM_PRBLK items of one flush (flushes on the batch stream run in
     // order, so one buffer serves all of them); grown when a flush needs more
     void* pr_scratch = nullptr;
@@ -133,6 +143,9 @@ int gemm_wg_per_cu();
 int pagerank_persistent(const uint8_t* queue, hipStream_t s);
 int pagerank_blocked_persistent(const uint8_t* queue, hipStream_t s);
 int trsm_persistent(const uint8_t* queue, hipStream_t s);
+int pagerank_binned_persistent(const uint8_t* queue, hipStream_t s);
+// Raises the binned pageRank kernels' LDS limit (once; outside any graph capture).
+int pagerank_binned_prepare();
 // Workgroups of the persistent filter kernel resident per CU (an occupancy query, cached).
 int filter_wg_per_cu();
 

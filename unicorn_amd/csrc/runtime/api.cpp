@@ -241,7 +241,7 @@ pmStatus pmGetRawMemPtr(pmMemHandle h, pmRawMemPtr* p) {
 
 static Subtask& ctx_subtask(pmTaskHandle th, ulong sid) {
     DistributionContext* c = current_context();
-    if (!c || !c->subtask) raise(pmInvalidCommand, "only callable from a dataDistribution callback");
+    if (!c || !c->subtask || !c->distribution) raise(pmInvalidCommand, "only callable from a dataDistribution callback");
     if ((Task*)th != c->task || sid != c->subtask->id) raise(pmInvalidIndex, "subscription for another subtask");
     return *c->subtask;
 }
@@ -576,6 +576,13 @@ pmStatus pmGetHipKernelTime(double* ms, uint64_t* flushes, bool reset) {
     Runtime* r = Runtime::get();
     if (!r || !r->gpu_count) return pmNoCompatibleDevice;
     return uh_batch_timing(r->batch, ms, flushes, reset ? 1 : 0) ? pmGraphicsCardError : pmSuccess;
+}
+
+pmStatus pmGetHipKernelTimeByFamily(int family, double* ms, uint64_t* flushes, double* work, bool reset) {
+    Runtime* r = Runtime::get();
+    if (!r || !r->gpu_count) return pmNoCompatibleDevice;
+    int rc = uh_batch_timing_family(r->batch, family, ms, flushes, work, reset ? 1 : 0);
+    return rc == UH_ERR_ARG ? pmInvalidIndex : rc ? pmGraphicsCardError : pmSuccess;
 }
 
 }  // namespace pm

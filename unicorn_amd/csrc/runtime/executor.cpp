@@ -133,6 +133,13 @@ void Runtime::init() {
         uh_stream_create(&r->batch_stream);
         uh_stream_create(&r->copy_stream);
         uh_batch_create(&r->batch, r->batch_stream);
+        // device copies handed to kernel-kind callbacks: the stub's pmDeviceInfo and one status
+        // word per subtask slot (16) and for the batch
+        if (uh_malloc(&r->dev_info, sizeof(pmDeviceInfo)) ||
+            uh_memcpy_h2d(r->dev_info, d, sizeof(pmDeviceInfo), r->copy_stream) ||
+            uh_malloc(&r->dev_status, (kBatchStatusSlot + 1) * sizeof(pmStatus)) ||
+            uh_stream_sync(r->copy_stream))
+            fprintf(stderr, "pmlib: device copies for kernel-kind callbacks: %s\n", uh_last_error());
     }
     r->hip_mode = env_int("PMLIB_HIP_MODE", r->hip_mode.load());
     r->use_graphs = env_int("PMLIB_HIP_GRAPHS", 1);
@@ -172,6 +179,8 @@ void Runtime::finalize() {
         }
         uh_stream_destroy(r->batch_stream);
         uh_stream_destroy(r->copy_stream);
+        uh_free(r->dev_info);
+        uh_free(r->dev_status);
     }
     for (auto* d : r->cpu_devices) delete d;
     delete r->gpu_device;
@@ -361,6 +370,10 @@ void Runtime::complete(Task* t) {
         if (pinned) as->gpu_pins--;
     }
     if (pinned) gpu_admitted--;
+    if (t->dev_conf) {          // every kernel that read it has finished: its subtasks retired
+        Memory::get().device_free(t->dev_conf, t->conf.size());
+        t->dev_conf = nullptr;
+    }
     if (t->callbacks->cb.taskCompletionCallback) t->callbacks->cb.taskCompletionCallback(t->info());
     std::lock_guard<std::mutex> g(t->mtx);
     t->end_time = now_secs();
@@ -405,7 +418,11 @@ void Runtime::cpu_loop(int idx) {
                     run_distribution(t, *st, dev);
                     std::vector<std::vector<uint8_t>> compact;
                     pmSubtaskInfo si = host_subtask_info(t, *st, compact);
-                    pmStatus rs = t->callbacks->cb.subtask_cpu(t->info(), *dev, si);
+                    pmStatus rs;
+                    {
+                        SubtaskScope scope(t, st.get(), dev->deviceHandle);
+                        rs = t->callbacks->cb.subtask_cpu(t->info(), *dev, si);
+                    }
                     if (rs != pmSuccess) raise(rs, "subtask_cpu callback failed");
                     // COMPACT views: scatter written records back
                     for (uint i = 0; i < t->spaces.size(); ++i) {
@@ -542,17 +559,24 @@ void stat_add(uint64_t pmHipStubStats::*f, uint64_t v) {
     r->stats.*f += v;
 }
 
-void upload_scratch(Task* t, Subtask& st, GpuSlot& s, pmSubtaskInfo& si, void* stream) {
+// Copies a small host table to the device on the copy stream and waits for that copy only (not
+// for the work queued on the subtask's or the batch's stream). The source stays host memory the
+// caller owns; the slot's pinned buffer is not used, since it may still feed an H2D copy in flight.
+void upload_small(void* dst, const void* src, size_t bytes, const char* what) {
+    Runtime* r = Runtime::get();
+    hipchk(uh_memcpy_h2d(dst, src, bytes, r->copy_stream), what);
+    hipchk(uh_stream_sync(r->copy_stream), what);
+}
+
+// The subtask's scratch buffer, filled on the host by its data-distribution callback, becomes the
+// device buffer gpuContext.scratchBuffer (what pmGetScratchBuffer returns in device code).
+void upload_scratch(Subtask& st, GpuSlot& s, pmSubtaskInfo& si) {
     auto it = st.scratch.find(PRE_SUBTASK_TO_SUBTASK);
     if (it == st.scratch.end()) it = st.scratch.find(PRE_SUBTASK_TO_POST_SUBTASK);
     if (it == st.scratch.end() || it->second.empty()) return;
     void* d = slot_device_alloc(s, it->second.size());
-    uint8_t* pin = slot_pinned(s, it->second.size());
-    memcpy(pin, it->second.data(), it->second.size());
-    hipchk(uh_memcpy_h2d(d, pin, it->second.size(), stream), "scratch upload");
-    hipchk(uh_stream_sync(stream), "scratch upload");
+    upload_small(d, it->second.data(), it->second.size(), "scratch upload");
     si.gpuContext.scratchBuffer = d;
-    (void)t;
 }
 
 // PLAIN mode: stage every address space of the subtask as the reference stub does.
@@ -734,16 +758,13 @@ pmSubtaskInfo stage_resident(Task* t, GpuSlot& s, void* stream) {
                 std::vector<uint64_t> tab(3 * n);
                 for (size_t r = 0; r < n; ++r) { tab[r] = cl.src_off[r]; tab[n + r] = cl.dst_off[r]; tab[2 * n + r] = cl.len[r]; }
                 uint64_t* dtab = (uint64_t*)slot_device_alloc(s, tab.size() * 8);
-                uint8_t* pin = slot_pinned(s, tab.size() * 8);
-                memcpy(pin, tab.data(), tab.size() * 8);
-                hipchk(uh_memcpy_h2d(dtab, pin, tab.size() * 8, stream), "record table upload");
+                upload_small(dtab, tab.data(), tab.size() * 8, "record table upload");
                 uint64_t maxlen = 0;
                 for (size_t r = 0; r < n; ++r) maxlen = std::max<uint64_t>(maxlen, cl.len[r]);
                 if (t->read_only(i) || !t->write_only(i)) {
                     hipchk(uh_copy_records(buf, as->dev, dtab, dtab + n, dtab + 2 * n, (uint32_t)n, maxlen, stream), "device gather");
                     stat_add(&pmHipStubStats::d2d_bytes, cl.total);
                 }
-                hipchk(uh_stream_sync(stream), "record table upload");   // pinned buffer reuse
                 mi.ptr = buf;
                 mi.length = cl.total;
                 if (cl.has_read) mi.readPtr = buf + cl.first_read;
@@ -779,26 +800,32 @@ void scatter_resident(Task* t, GpuSlot& s, void* stream) {
 
 }  // namespace
 
-// Launch a kernel-kind callback: copies of taskConf and pmDeviceInfo are placed in device
-// memory owned by the slot; arguments are passed by value as the kernel signature requires.
-static pmStatus launch_kernel_kind(Task* t, Subtask& st, pmSubtaskInfo& si, void* stream,
-                                   pmDeviceInfo* dev, GpuSlot& s) {
+// Launch a kernel-kind callback (reference pmCudaInterface.cu:532-543). taskInfo.taskConf points
+// at a device copy of the task's configuration, uploaded once per task at its first kernel-kind
+// subtask (reference CreateTaskConf, pmExecutionStub.cpp:3321-3337) and freed when the task
+// completes; pDeviceInfo is the stub's device copy made at initialisation (reference
+// CreateDeviceInfoCudaPtr, 3344-3368). The kernel writes its pmStatus into a word of the slot,
+// which is not read back (as the reference, whose read-back is disabled at 3487-3497). Nothing
+// here waits for the device: the launch is ordered after the subtask's staging on its stream.
+// A subtask that set no launch configuration runs the default one (one block of one thread), as
+// the reference's default pmCudaLaunchConf.
+static pmStatus launch_kernel_kind(Task* t, Subtask& st, pmSubtaskInfo& si, void* stream, int status_slot) {
+    Runtime* r = Runtime::get();
     pmTaskInfo ti = t->info();
     if (!t->conf.empty()) {
-        void* dconf = slot_device_alloc(s, t->conf.size());
-        uint8_t* pin = slot_pinned(s, t->conf.size() + sizeof(pmDeviceInfo));
-        memcpy(pin, t->conf.data(), t->conf.size());
-        hipchk(uh_memcpy_h2d(dconf, pin, t->conf.size(), stream), "taskConf upload");
-        ti.taskConf = dconf;
+        if (!t->dev_conf) {
+            void* d = Memory::get().device_alloc(t->conf.size());
+            if (uh_memcpy_h2d(d, t->conf.data(), t->conf.size(), r->copy_stream) || uh_stream_sync(r->copy_stream)) {
+                Memory::get().device_free(d, t->conf.size());
+                raise(pmGraphicsCardError, std::string("taskConf upload: ") + uh_last_error());
+            }
+            t->dev_conf = d;
+        }
+        ti.taskConf = t->dev_conf;
     }
-    void* ddev = slot_device_alloc(s, sizeof(pmDeviceInfo));
-    uint8_t* pin = slot_pinned(s, t->conf.size() + sizeof(pmDeviceInfo));
-    memcpy(pin + t->conf.size(), dev, sizeof(pmDeviceInfo));
-    hipchk(uh_memcpy_h2d(ddev, pin + t->conf.size(), sizeof(pmDeviceInfo), stream), "device info upload");
-    pmStatus* dstatus = (pmStatus*)slot_device_alloc(s, sizeof(pmStatus));
-    hipchk(uh_stream_sync(stream), "kernel-kind argument upload");
-    pmDeviceInfo* ddevp = (pmDeviceInfo*)ddev;
-    void* args[] = {&ti, &ddevp, &si, &dstatus};
+    pmDeviceInfo* ddev = (pmDeviceInfo*)r->dev_info;
+    pmStatus* dstatus = (pmStatus*)r->dev_status + status_slot;
+    void* args[] = {&ti, &ddev, &si, &dstatus};
     const pmGpuLaunchConf& lc = st.launch;
     hipchk(uh_launch_kernel((const void*)t->callbacks->cb.subtask_gpu_hip, lc.blocksX, lc.blocksY, lc.blocksZ,
                             lc.threadsX, lc.threadsY, lc.threadsZ, (size_t)lc.sharedMem, stream, args),
@@ -807,11 +834,12 @@ static pmStatus launch_kernel_kind(Task* t, Subtask& st, pmSubtaskInfo& si, void
 }
 
 static pmStatus run_gpu_callback(Task* t, Subtask& st, pmSubtaskInfo& si, void* stream,
-                                 pmDeviceInfo* dev, GpuSlot& s) {
-    if (t->callbacks->cb.subtask_gpu_custom)
+                                 pmDeviceInfo* dev, int status_slot) {
+    if (t->callbacks->cb.subtask_gpu_custom) {
+        SubtaskScope scope(t, &st, dev->deviceHandle);
         return t->callbacks->cb.subtask_gpu_custom(t->info(), *dev, si, stream);
-    if (!st.has_launch) raise(pmInvalidCallbacks, "subtask_gpu_hip needs pmSetGpuLaunchConf");
-    return launch_kernel_kind(t, st, si, stream, dev, s);
+    }
+    return launch_kernel_kind(t, st, si, stream, status_slot);
 }
 
 static void release_slot(GpuSlot& s) {
@@ -1075,8 +1103,8 @@ void Runtime::gpu_loop() {
                             pmSubtaskInfo si = stage_resident(tt, *sp, batch_stream);
                             if (sp->st->reserved_gpu)
                                 si.gpuContext.reservedGlobalMem = slot_device_alloc(*sp, sp->st->reserved_gpu);
-                            upload_scratch(tt, *sp->st, *sp, si, batch_stream);
-                            pmStatus rs = run_gpu_callback(tt, *sp->st, si, batch, gpu_device, *sp);
+                            upload_scratch(*sp->st, *sp, si);
+                            pmStatus rs = run_gpu_callback(tt, *sp->st, si, batch, gpu_device, kBatchStatusSlot);
                             if (rs != pmSuccess) raise(rs, "GPU callback failed");
                         } catch (Error& e) { fail_task(tt, e); }
                     }
@@ -1150,8 +1178,8 @@ void Runtime::gpu_loop() {
                     run_distribution(t, *s.st, gpu_device);
                     pmSubtaskInfo info = plain ? stage_plain(t, s) : stage_resident(t, s, s.stream);
                     if (s.st->reserved_gpu) info.gpuContext.reservedGlobalMem = slot_device_alloc(s, s.st->reserved_gpu);
-                    upload_scratch(t, *s.st, s, info, s.stream);
-                    pmStatus rs = run_gpu_callback(t, *s.st, info, s.stream, gpu_device, s);
+                    upload_scratch(*s.st, s, info);
+                    pmStatus rs = run_gpu_callback(t, *s.st, info, s.stream, gpu_device, si_idx);
                     if (rs != pmSuccess) raise(rs, "GPU callback failed");
                     if (plain) {
                         // D2H of written records into the pinned buffer (pmCudaInterface.cu:510-527)

@@ -181,6 +181,7 @@ struct Task {
     bool gpu_resident = false;                   // GPU subtasks use HBM mirrors (GPU-only task)
     bool gpu_prepared = false;                   // mirrors made current by the HIP stub
     int hip_mode = PM_HIP_MODE_BATCHED;          // HIP stub mode at submission
+    void* dev_conf = nullptr;                    // device copy of conf for kernel-kind callbacks
     double submit_time = 0, start_time = 0, end_time = 0;
     double deadline = 0;
     std::mutex mtx;
@@ -206,8 +207,32 @@ struct DistributionContext {
     Task* task = nullptr;
     Subtask* subtask = nullptr;
     pmDeviceHandle device = nullptr;
+    bool distribution = true;    // false: a subtask callback (scratch buffers only, no subscribing)
 };
 DistributionContext*& current_context();
+
+// Makes the subtask's scratch buffers reachable (pmGetScratchBuffer) while one of its subtask
+// callbacks runs on this thread, as the reference allows (matrixTranspose.cpp:325 takes its
+// PRE_SUBTASK_TO_SUBTASK buffer inside subtask_cpu).
+struct SubtaskScope {
+    DistributionContext ctx;
+    DistributionContext* saved;
+    SubtaskScope(Task* t, Subtask* st, pmDeviceHandle dev) {
+        ctx.task = t;
+        ctx.subtask = st;
+        ctx.device = dev;
+        ctx.distribution = false;
+        saved = current_context();
+        current_context() = &ctx;
+    }
+    ~SubtaskScope() { current_context() = saved; }
+    SubtaskScope(const SubtaskScope&) = delete;
+    SubtaskScope& operator=(const SubtaskScope&) = delete;
+};
+
+// Status word of kernel-kind callbacks launched from a batch (slots 0..15 are the per-subtask
+// slots of the PLAIN / RESIDENT modes).
+constexpr int kBatchStatusSlot = 16;
 
 class Runtime {
 public:
@@ -248,6 +273,8 @@ public:
     void* batch = nullptr;
     void* batch_stream = nullptr;
     void* copy_stream = nullptr;
+    void* dev_info = nullptr;            // device copy of gpu_device (kernel-kind callbacks)
+    void* dev_status = nullptr;          // kBatchStatusSlot + 1 pmStatus words, written by kernels
     std::atomic<int> gpu_admitted{0};    // GPU-resident tasks admitted and not yet complete
     // common helpers
     Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count);

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "uh_gemm_tile_count": (i32, [C.POINTER(i32)]),
             "uh_pagerank_pull": (i32, [C.POINTER(uh_pagerank_args), vp]),
             "uh_pagerank_pull_noalloc": (i32, [C.POINTER(uh_pagerank_args), vp]),
+            "uh_pagerank_init_w": (i32, [vp, vp, u32, f32, f32, vp]),
             "uh_lu_panel": (i32, [vp, u64, i32, vp]),
             "uh_trsm_lln": (i32, [vp, u64, vp, u64, i32, i32, vp]),
             "uh_trsm_run": (i32, [vp, u64, vp, u64, i32, i32, vp]),
