@@ -279,20 +279,21 @@ def bench_matmul(args, U, steps, warmup, cores):
 
 def bench_pagerank(args, U, steps, warmup, cores):
     """pageRank on a 10M-page synthetic power-law web graph (out-degree P(k) ~ (k + k0)^-2 on
-    [1, 100], mean 10, 5 % dangling, preferential targets), 20 iterations, 1M-page subtasks.
-    One step = one pageRank computation (init task + 20 iteration tasks). Roofline: per iteration
-    12 N + 4 E bytes (row pointers, degree, rank in / out, in-link ids) over one iteration's
-    flush time."""
+    [1, 100], mean 10, 5 % dangling, preferential targets), 20 iterations, 1M-page subtasks,
+    over the binned graph layout. One step = one pageRank computation (init task + 20 iteration
+    tasks). Roofline: per iteration 12 N + 4 E bytes (row pointers, degree, rank in / out, in-link
+    ids: the algorithmic bytes of a pull) over one iteration's flush time."""
     pages, per, iters = 10_000_000, 1_000_000, 20
     log("pageRank 10M: graph")
     rp, col = U.webgen(pages, SEED)
     edges = int(col.size)
+    U.pr_set_layout("binned")
     pr = U.PageRank(rp, col, per)
-    if not pr.blocked:
-        raise RuntimeError("bench: the 10M-page graph did not take the source-blocked layout")
+    if pr.layout != "binned":
+        raise RuntimeError("bench: the 10M-page graph did not take the binned layout")
     log("pageRank 10M: batched")
     run = lambda: pr.run(iters, U.DEVICE_GPU)
-    elapsed, lat, kernel_ms, flushes, _, warm = timed_batched(U, run, steps, warmup, U.FAMILY_PAGERANK_BLOCKED)
+    elapsed, lat, kernel_ms, flushes, _, warm = timed_batched(U, run, steps, warmup, U.FAMILY_PAGERANK_BINNED)
     y = pr.ranks()
     comps = {}
     U.set_hip_mode(U.PM_HIP_MODE_PLAIN)
@@ -316,14 +317,18 @@ def bench_pagerank(args, U, steps, warmup, cores):
                "sample": f"one full task (20 iterations x 10 subtasks) on the pm:: CPU stubs, {cores} threads, "
                          f"{dt:.2f} s"}
     pr.close()
-    log("pageRank 10M: CSR layout and push decomposition comparators")
-    U.pr_set_layout(False)
-    prc = U.PageRank(rp, col, per)
-    U.pr_set_layout(True)
-    e, l, cms, cfl, _, _ = timed_batched(U, lambda: prc.run(iters, U.DEVICE_GPU), 1, 1, U.FAMILY_PAGERANK,
-                                         every=1, warm=False)
-    comps["csr_layout"] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms, 4)}
-    prc.close()
+    log("pageRank 10M: source-blocked and CSR layouts, push decomposition comparators")
+    for name, layout, fam in (("blocked_layout", "blocked", U.FAMILY_PAGERANK_BLOCKED),
+                              ("csr_layout", "csr", U.FAMILY_PAGERANK)):
+        U.pr_set_layout(layout)
+        prc = U.PageRank(rp, col, per)
+        U.pr_set_layout("binned")
+        e, l, cms, cfl, _, _ = timed_batched(U, lambda: prc.run(iters, U.DEVICE_GPU), 1, 1, fam,
+                                             every=1, warm=False)
+        yl = prc.ranks()
+        comps[name] = {"tasks_per_s": round(1 / e, 4), "kernel_ms_per_iteration": round(cms, 4),
+                       "rel_l1_vs_binned": float(np.abs(yl.astype(np.float64) - y).sum() / np.abs(y).sum())}
+        prc.close()
     # the reference's decomposition: source-page subtasks push into private full-length vectors
     # that REDUCE_ADD folds in HBM (records padded to the largest out-degree, 101 uint32 per page)
     prp = U.PageRankPush(rp, col, per)
@@ -345,7 +350,7 @@ def bench_pagerank(args, U, steps, warmup, cores):
         **lat_stats(lat), "parity": parity, "per_subtask_launch": comps,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("pagerank_10m"),
-                     "kernel": "k_pagerank_blocked_persistent + k_pagerank_combine_persistent (one iteration)",
+                     "kernel": "k_pagerank_binned_persistent (one iteration)",
                      "kernel_ms": round(kernel_ms, 4), "timed_flushes": flushes,
                      "algorithmic_bytes_per_launch": it_bytes},
         "prewarm": {"seconds": PREWARM_S, "tasks": warm},

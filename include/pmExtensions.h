@@ -57,6 +57,20 @@ typedef struct pmHipStubStats
 } pmHipStubStats;
 
 #ifdef __cplusplus
+/* Options of one submission (pmSubmitTaskWithOptions).
+ *   hipMode    PM_HIP_MODE_* for this task only; -1 = the mode set with pmSetHipStubMode.
+ *   unordered  start without waiting for earlier tasks on the same address spaces. By default a
+ *              task waits for every earlier, unfinished task that it conflicts with on some
+ *              address space (one of the two writes it), as the reference's address-space locks
+ *              order tasks; an unordered task is for callers that know the two touch disjoint
+ *              bytes (e.g. blocks of one matrix). Later tasks still wait for it. */
+typedef struct pmTaskSubmitOptions
+{
+    int hipMode;
+    bool unordered;
+} pmTaskSubmitOptions;
+pmStatus pmSubmitTaskWithOptions(pmTaskDetails pTaskDetails, const pmTaskSubmitOptions* pOptions,
+                                 pmTaskHandle* pTaskHandle);
 /* Select the HIP stub mode for tasks submitted afterwards. */
 pmStatus pmSetHipStubMode(int pMode);
 int pmGetHipStubMode();

@@ -77,3 +77,37 @@ def test_matmul_app(gpu_runtime, hip_mode):
     finally:
         mm.close()
     assert _rel(got, O.matmul_f64(A, B)) <= 1e-5
+
+
+@pytest.mark.parametrize("tile", range(8))
+def test_gemm_every_tile_configuration(gpu_runtime, tile):
+    """Each tile configuration (forced for the shape's class), plain launch and batched flush, on
+    ragged shapes whose classes no other test uses."""
+    import ctypes as C
+    from unicorn_amd import hip as H
+    L = H.lib()
+    n_tiles = C.c_int(0)
+    H.check(L.uh_gemm_tile_count(C.byref(n_tiles)), "uh_gemm_tile_count")
+    assert n_tiles.value == 8
+    for (M, N, K) in [(200, 700, 1100), (96, 80, 48)]:
+        rng = np.random.default_rng(tile * 100 + M)
+        A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+        B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        ref = O.matmul_f64(A, B)
+        H.check(L.uh_gemm_set_tile(M, N, K, 0, tile), "uh_gemm_set_tile")
+        dA, dB, dC = H.DeviceBuffer.from_array(A), H.DeviceBuffer.from_array(B), H.DeviceBuffer(M * N * 4)
+        print(f"gemm tile {tile}: {M}x{N}x{K} plain", flush=True)
+        H.gemm(dA, dB, dC, M, N, K, K, N, N, precision=0)
+        assert _rel(dC.to_array((M, N), np.float32), ref) <= 1e-5
+        s = H.Stream()
+        bh = H.vp()
+        H.check(L.uh_batch_create(C.byref(bh), s.h), "uh_batch_create")
+        try:
+            dC2 = H.DeviceBuffer(M * N * 4)
+            print(f"gemm tile {tile}: {M}x{N}x{K} batched", flush=True)
+            H.gemm(dA, dB, dC2, M, N, K, K, N, N, precision=0, stream=bh.value)
+            H.check(L.uh_batch_flush(bh.value, 0), "uh_batch_flush")
+            s.sync()
+            assert np.array_equal(dC2.to_array((M, N), np.float32), dC.to_array((M, N), np.float32))
+        finally:
+            H.check(L.uh_batch_destroy(bh.value), "uh_batch_destroy")

@@ -4,9 +4,10 @@ Tolerance: relative L1 <= 1e-5 against the oracle's restatement of the reference
 subtask + REDUCE_ADD in subtask order); the error against the FP64 oracle is asserted too.
 Closed forms pin the recurrence: a ring gives 0.85 everywhere, a star gives the hub
 0.85 * leaves after the first iteration, and a graph of dangling pages gives zero.
-Both graph layouts run: the source-blocked one (the default) and the plain in-link CSR. In the
-blocked layout the GPU kernels and the CPU callback add in the same order, so their ranks are
-bitwise equal.
+All three graph layouts run: binned (the default: propagation-blocked link values with
+fixed-point row sums), source-blocked, and the plain in-link CSR. In the blocked layout the GPU
+kernels and the CPU callback add in the same order, and in the binned layout both add fixed-point
+integers, so in both their ranks are bitwise equal.
 """
 import numpy as np
 import pytest
@@ -20,14 +21,17 @@ def _rel_l1(a, b):
     return float(np.abs(a.astype(np.float64) - b).sum() / max(np.abs(b).sum(), 1e-300))
 
 
+LAYOUTS = ["binned", "blocked", "csr"]
+
+
 @pytest.fixture
 def layout(gpu_runtime, request):
     gpu_runtime.pr_set_layout(request.param)
     yield request.param
-    gpu_runtime.pr_set_layout(True)
+    gpu_runtime.pr_set_layout("binned")
 
 
-@pytest.mark.parametrize("layout", [True, False], indirect=True, ids=["blocked", "csr"])
+@pytest.mark.parametrize("layout", LAYOUTS, indirect=True)
 @pytest.mark.parametrize("hip_mode", [0, 1, 2])
 def test_pagerank_matches_oracle(gpu_runtime, hip_mode, layout):
     U = gpu_runtime
@@ -35,7 +39,8 @@ def test_pagerank_matches_oracle(gpu_runtime, hip_mode, layout):
     rp, col = U.webgen(200_000, 11)
     pr = U.PageRank(rp, col, 50_000)
     try:
-        assert pr.blocked == layout
+        assert pr.layout == layout
+        print(f"pagerank: layout {layout}, mode {hip_mode}", flush=True)
         pr.run(20, U.DEVICE_GPU)
         y = pr.ranks()
     finally:
@@ -44,14 +49,14 @@ def test_pagerank_matches_oracle(gpu_runtime, hip_mode, layout):
     assert _rel_l1(y, O.pagerank_f64(rp, col, 20)) <= 1e-5
 
 
-@pytest.mark.parametrize("layout", [True, False], indirect=True, ids=["blocked", "csr"])
+@pytest.mark.parametrize("layout", LAYOUTS, indirect=True)
 def test_pagerank_compact_views(gpu_runtime, layout):
     U = gpu_runtime
     U.set_hip_mode(1, compact=True)
     rp, col = U.webgen(30_000, 5)
     pr = U.PageRank(rp, col, 7_000)      # ragged last subtask; subtasks share row groups
     try:
-        assert pr.blocked == layout
+        assert pr.layout == layout
         pr.run(4, U.DEVICE_GPU)
         y = pr.ranks()
     finally:
@@ -73,7 +78,8 @@ def _csr(n, edges):
     return rp, col[:len(edges)]
 
 
-def test_pagerank_closed_forms(gpu_runtime):
+@pytest.mark.parametrize("layout", LAYOUTS, indirect=True)
+def test_pagerank_closed_forms(gpu_runtime, layout):
     U = gpu_runtime
     U.set_hip_mode(2)
     n = 1000
@@ -109,8 +115,11 @@ def test_pagerank_cpu_gpu_mix(gpu_runtime):
     assert _rel_l1(y, O.pagerank(rp, col, 5, 5_000)) <= 1e-5
 
 
+@pytest.mark.parametrize("layout", ["binned", "blocked"], indirect=True)
 @pytest.mark.parametrize("hip_mode", [0, 2])
-def test_pagerank_blocked_cpu_gpu_bitwise(gpu_runtime, hip_mode):
+def test_pagerank_cpu_gpu_bitwise(gpu_runtime, hip_mode, layout):
+    """The layouts whose CPU callback adds in the kernels' order (blocked) or in an order-free
+    fixed point (binned): CPU and GPU ranks are the same bits."""
     U = gpu_runtime
     U.set_hip_mode(hip_mode)
     rp, col = U.webgen(100_000, 4)
@@ -118,7 +127,7 @@ def test_pagerank_blocked_cpu_gpu_bitwise(gpu_runtime, hip_mode):
     for device in (U.DEVICE_CPU, U.DEVICE_GPU):
         pr = U.PageRank(rp, col, 30_000)
         try:
-            assert pr.blocked
+            assert pr.layout == layout
             pr.run(3, device)
             ys.append(pr.ranks())
         finally:
@@ -127,7 +136,8 @@ def test_pagerank_blocked_cpu_gpu_bitwise(gpu_runtime, hip_mode):
     assert _rel_l1(ys[1], O.pagerank(rp, col, 3, 30_000)) <= 1e-5
 
 
-def test_pagerank_csr_fallback_for_heavy_rows(gpu_runtime):
+@pytest.mark.parametrize("layout", ["blocked"], indirect=True)
+def test_pagerank_csr_fallback_for_heavy_rows(gpu_runtime, layout):
     U = gpu_runtime
     U.set_hip_mode(2)
     n = 20_000
@@ -156,7 +166,7 @@ def test_pagerank_from_webdump(gpu_runtime, hip_mode, tmp_path):
     U.webdump_write(tmp_path, rp, col, 100, 40_000)
     pr = U.PageRank.from_webdump(tmp_path, 50_000)
     try:
-        assert pr.blocked
+        assert pr.layout == "binned"
         pr.run(10, U.DEVICE_GPU)
         y = pr.ranks()
     finally:

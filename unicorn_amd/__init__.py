@@ -195,7 +195,7 @@ def kernel_time(reset: bool = False):
 
 
 # kernel families of the batched flushes (UH_FAMILY_* of unicorn_hip.h)
-FAMILY_FILTER, FAMILY_GEMM, FAMILY_PAGERANK, FAMILY_TRSM, FAMILY_PAGERANK_BLOCKED = 0, 1, 2, 3, 4
+FAMILY_FILTER, FAMILY_GEMM, FAMILY_PAGERANK, FAMILY_TRSM, FAMILY_PAGERANK_BLOCKED, FAMILY_PAGERANK_BINNED = 0, 1, 2, 3, 4, 5
 
 
 def kernel_time_family(family: int, reset: bool = False):

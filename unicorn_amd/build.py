@@ -99,7 +99,8 @@ def build_all(verbose: bool = True) -> None:
         _link(LIB / "libunicorn_apps.so", CXX, app_o, rpath + ["-lpmlib", "-lunicorn_hip", "-pthread"])
     # drop stale objects (test-support objects are named after their own sources)
     keep = {p.name for p in hip_o + rt_o + app_o}
-    test_stems = {p.stem for p in TEST_NATIVE.glob("*.hip")} if TEST_NATIVE.exists() else set()
+    test_stems = ({p.stem for p in TEST_NATIVE.glob("*.hip")} | {p.stem for p in TEST_NATIVE.glob("*.cpp")}
+                  if TEST_NATIVE.exists() else set())
     for p in OBJ.glob("*.o"):
         if p.name not in keep and p.name.rsplit("-", 1)[0] not in test_stems:
             p.unlink()
@@ -111,18 +112,23 @@ TEST_NATIVE = ROOT / "tests" / "native"
 
 
 def build_test_support(verbose: bool = True) -> None:
-    """Test-support libraries of tests/native (HIP sources written against the pm:: API, e.g. the
-    kernel-kind contract probe), one shared library per source: tests/native/lib<stem>.so."""
-    srcs = sorted(TEST_NATIVE.glob("*.hip"))
+    """Test-support libraries of tests/native (sources written against the pm:: API: HIP ones such
+    as the kernel-kind contract probe, C++ ones such as the task-ordering probe), one shared
+    library per source: tests/native/lib<stem>.so."""
+    srcs = sorted(TEST_NATIVE.glob("*.hip")) + sorted(TEST_NATIVE.glob("*.cpp"))
     if not srcs:
         return
     OBJ.mkdir(parents=True, exist_ok=True)
     deps = _headers(INC)
     rpath = ["-Wl,-rpath,$ORIGIN/../../unicorn_amd/lib", f"-L{LIB}", "-Wl,--no-undefined"]
     for src in srcs:
-        obj = _compile(src, HIPCC, HIP_FLAGS, deps)
-        _link(TEST_NATIVE / f"lib{src.stem}.so", HIPCC, [obj],
-              [f"--offload-arch={ARCH}", "-fPIC"] + rpath + ["-lpmlib", "-lunicorn_hip"])
+        if src.suffix == ".hip":
+            obj = _compile(src, HIPCC, HIP_FLAGS, deps)
+            _link(TEST_NATIVE / f"lib{src.stem}.so", HIPCC, [obj],
+                  [f"--offload-arch={ARCH}", "-fPIC"] + rpath + ["-lpmlib", "-lunicorn_hip"])
+        else:
+            obj = _compile(src, CXX, CXX_FLAGS, deps)
+            _link(TEST_NATIVE / f"lib{src.stem}.so", CXX, [obj], rpath + ["-lpmlib", "-pthread"])
     if verbose:
         print(f"built test support in {TEST_NATIVE}", flush=True)
 

@@ -375,7 +375,7 @@ static int register_all() {
 
 // One stage task, submitted (not waited for): conf must outlive it.
 static int submit_stage(Instance* in, int stage, size_t k, int part, int device, luTaskConf* conf,
-                        pmTaskHandle* th, ushort priority = 0) {
+                        pmTaskHandle* th, const pmTaskSubmitOptions* opt = nullptr) {
     conf->matrixDim = in->n;
     conf->blockDim = in->block;
     conf->taskStep = k;
@@ -391,8 +391,7 @@ static int submit_stage(Instance* in, int stage, size_t k, int part, int device,
     mem[0] = pmTaskMem(in->mat, READ_WRITE, SUBSCRIPTION_OPTIMAL, true);
     pmTaskDetails d(conf, sizeof(*conf), mem, 1, g_cb[stage].h[device], subtasks);
     d.taskId = k;
-    d.priority = priority;
-    return (int)pmSubmitTask(d, th);
+    return (int)pmSubmitTaskWithOptions(d, opt, th);
 }
 
 static int wait_stage(pmTaskHandle th) {
@@ -478,9 +477,10 @@ extern "C" int ua_lu_prefetch(void* h) {
 // chain of small launches, submitted as a RESIDENT-mode task so that its launches go to a stream
 // of their own) and the rest of step k's trailing update (batch stream) run together: the stub
 // stages the update while the chain runs. The two tasks write disjoint blocks and read none of
-// each other's. The hip stub mode is switched around that one submission, so other threads
-// should not submit tasks meanwhile. (Measured: 47.6 ms against 49.4 ms at N = 16384; the GPU
-// overlaps little, because the persistent update holds every CU while it runs.)
+// each other's, so the update is submitted unordered (it does not wait for the factorisation,
+// which shares its address space); both carry their stub mode in their submit options.
+// (Measured: 47.6 ms against 49.4 ms at N = 16384; the GPU overlaps little, because the
+// persistent update holds every CU while it runs.)
 extern "C" int ua_lu_factor(void* h, int device, double* seconds) {
     Instance* in = (Instance*)h;
     if (device < 0 || device > 2) return -1;
@@ -496,10 +496,10 @@ extern "C" int ua_lu_factor(void* h, int device, double* seconds) {
             const size_t m = nb - k - 1;
             luTaskConf c1, c3;
             pmTaskHandle h1 = nullptr, h3 = nullptr;
-            pmSetHipStubMode(PM_HIP_MODE_RESIDENT);
-            rc = submit_stage(in, LU_DECOMPOSE, k + 1, 0, device, &c1, &h1);
-            pmSetHipStubMode(PM_HIP_MODE_BATCHED);
-            if (!rc && m >= 2) rc = submit_stage(in, DIAGONAL_COMP, k, 2, device, &c3, &h3);
+            const pmTaskSubmitOptions chain{PM_HIP_MODE_RESIDENT, false};
+            const pmTaskSubmitOptions update{PM_HIP_MODE_BATCHED, true};
+            rc = submit_stage(in, LU_DECOMPOSE, k + 1, 0, device, &c1, &h1, &chain);
+            if (!rc && m >= 2) rc = submit_stage(in, DIAGONAL_COMP, k, 2, device, &c3, &h3, &update);
             const int r1 = h1 ? wait_stage(h1) : 0;
             const int r3 = h3 ? wait_stage(h3) : 0;
             if (!rc) rc = r1 ? r1 : r3;

@@ -144,7 +144,10 @@ extern "C" void* ua_mm_create(int n, int block, int precision) {
         delete in;
         return nullptr;
     }
-    apps::autotune_gemm(block, block, n, precision);         // one subtask's product shape
+    // The (n / block)^2 products of a task run in one batched flush, which fills the device the
+    // way one n x n x n product does: tune on that, record for the subtask's product shape.
+    if (n / block >= 2) apps::autotune_gemm_as(n, n, n, block, block, n, precision);
+    else apps::autotune_gemm(block, block, n, precision);
     return in;
 }
 

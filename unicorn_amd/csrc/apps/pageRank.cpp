@@ -434,7 +434,7 @@ struct Instance {
 
 static apps::CallbackVariants g_cb;
 static std::mutex g_mtx;
-static int g_layout = LAYOUT_BLOCKED;   // layout of graphs created from now on (falling back when it cannot hold one)
+static int g_layout = LAYOUT_BINNED;    // layout of graphs created from now on (falling back when it cannot hold one)
 
 static const float kInitialPageRank = 1.0f;
 

@@ -44,7 +44,10 @@ static bool plan_units(uint32_t N, uint64_t E, uint32_t P, uint32_t cus, Layout&
     for (uint32_t s = 0; s < S; ++s) {
         const uint32_t s0 = s * P;
         const uint32_t len = (uint32_t)std::min<uint64_t>(P, (uint64_t)N - s0);
-        uint64_t nu = std::max<uint64_t>(unit_count(len), (uint64_t)std::llround((double)target * len / N));
+        // cumulative rounding: the shares of the subtasks add up to the target
+        const uint64_t share = (uint64_t)std::llround((double)target * ((double)s0 + len) / N) -
+                               (uint64_t)std::llround((double)target * (double)s0 / N);
+        uint64_t nu = std::max<uint64_t>(unit_count(len), share);
         nu = std::max<uint64_t>(1, std::min<uint64_t>(nu, len));
         L.sub_unit[s] = (uint32_t)L.unit_start.size();
         for (uint64_t j = 0; j < nu; ++j) L.unit_start.push_back(s0 + (uint32_t)((uint64_t)len * j / nu));

@@ -53,6 +53,8 @@ static constexpr TileCfg kTiles[] = {
     {1, 2, 64},   // 3:  64 x 128, BK 64 (108 KB)
     {2, 2, 32},   // 4: 128 x 128, BK 32 (80 KB, 2 per CU)
     {1, 1, 32},   // 5:  64 x  64, BK 32 (40 KB)
+    {2, 1, 32},   // 6: 128 x  64, BK 32 (60 KB, 2 per CU)
+    {1, 2, 32},   // 7:  64 x 128, BK 32 (60 KB, 2 per CU)
 };
 static constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 static constexpr int G_VALU = kNumTiles;          // variant of the precision-2 reference path
@@ -309,6 +311,8 @@ __device__ __forceinline__ void gemm_run(const uh_gemm_args& g, int cfg, uint32_
             case 2: gemm_tile_split3<2, 1, 64>(g, tile, lds); break;
             case 3: gemm_tile_split3<1, 2, 64>(g, tile, lds); break;
             case 4: gemm_tile_split3<2, 2, 32>(g, tile, lds); break;
+            case 6: gemm_tile_split3<2, 1, 32>(g, tile, lds); break;
+            case 7: gemm_tile_split3<1, 2, 32>(g, tile, lds); break;
             default: gemm_tile_split3<1, 1, 32>(g, tile, lds); break;
         }
     } else {
@@ -377,11 +381,12 @@ __global__ void __launch_bounds__(G_THREADS) k_gemm_persistent(const uint8_t* qu
 typedef void (*GemmPlainFn)(uh_gemm_args, int);
 typedef void (*GemmPersistentFn)(const uint8_t*);
 static const GemmPlainFn kGemmPlain[kNumTiles + 1] = {
-    k_gemm<0>, k_gemm<1>, k_gemm<2>, k_gemm<3>, k_gemm<4>, k_gemm<5>, k_gemm<G_VALU>};
+    k_gemm<0>, k_gemm<1>, k_gemm<2>, k_gemm<3>, k_gemm<4>, k_gemm<5>, k_gemm<6>, k_gemm<7>, k_gemm<G_VALU>};
 static const GemmPersistentFn kGemmPersistent[kNumTiles + 2] = {
     k_gemm_persistent<0>, k_gemm_persistent<1>, k_gemm_persistent<2>, k_gemm_persistent<3>,
-    k_gemm_persistent<4>, k_gemm_persistent<5>, k_gemm_persistent<G_VALU>, k_gemm_persistent<G_GENERIC>};
-static_assert(kNumTiles == 6, "kernel tables list every tile configuration");
+    k_gemm_persistent<4>, k_gemm_persistent<5>, k_gemm_persistent<6>, k_gemm_persistent<7>,
+    k_gemm_persistent<G_VALU>, k_gemm_persistent<G_GENERIC>};
+static_assert(kNumTiles == 8, "kernel tables list every tile configuration");
 
 // Workgroups per CU of each persistent variant (occupancy queries, cached).
 static int g_gemm_wg[kNumTiles + 2] = {};

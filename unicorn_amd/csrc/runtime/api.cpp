@@ -299,9 +299,13 @@ pmStatus pmReserveGpuGlobalMem(pmTaskHandle th, pmDeviceHandle, ulong sid, pmSpl
     PM_API_END
 }
 
-pmStatus pmSubmitTask(pmTaskDetails d, pmTaskHandle* th) {
+pmStatus pmSubmitTask(pmTaskDetails d, pmTaskHandle* th) { return pmSubmitTaskWithOptions(d, nullptr, th); }
+
+pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt, pmTaskHandle* th) {
     PM_API_BEGIN
     Runtime* r = need_runtime();
+    if (opt && opt->hipMode != -1 && (opt->hipMode < PM_HIP_MODE_PLAIN || opt->hipMode > PM_HIP_MODE_BATCHED))
+        raise(pmInvalidCommand, "invalid HIP stub mode in the submit options");
     if (!th) raise(pmFatalError, "null task handle pointer");
     if (d.taskMemCount > MAX_MEM_SECTIONS_PER_TASK) raise(pmBeyondComputationalLimits, "too many memory sections");
     if (d.taskMemCount && !d.taskMem) raise(pmFatalError, "taskMem is null");
@@ -328,21 +332,17 @@ pmStatus pmSubmitTask(pmTaskDetails d, pmTaskHandle* th) {
     t->cpu_allowed = cs->cb.subtask_cpu != nullptr;
     t->gpu_allowed = r->gpu_count > 0 && (cs->cb.subtask_gpu_hip || cs->cb.subtask_gpu_custom);
     if (!t->cpu_allowed && !t->gpu_allowed) raise(pmNoCompatibleDevice, "no device can run this task's callbacks");
-    t->hip_mode = r->hip_mode.load();
+    t->hip_mode = opt && opt->hipMode != -1 ? opt->hipMode : r->hip_mode.load();
+    t->unordered = opt && opt->unordered;
     t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
         AddressSpace* as = t->spaces[i];
-        {
-            std::lock_guard<std::mutex> g(as->mtx);
-            as->users++;
-        }
-        // Any subtask reading or writing host memory needs the host copy current.
-        if (t->cpu_allowed || !t->gpu_resident) {
-            Memory::get().sync_host(as);
-            if (t->writable(i)) Memory::get().mark_host_written(as);
-        }
+        std::lock_guard<std::mutex> g(as->mtx);
+        as->users++;
     }
+    // host copies are made current when the task becomes ready (Runtime::make_ready), after the
+    // earlier tasks on its address spaces have completed
     Task* raw = t.release();
     *th = raw;
     r->submit(raw);

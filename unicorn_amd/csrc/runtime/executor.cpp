@@ -188,6 +188,22 @@ void Runtime::finalize() {
     delete r;
 }
 
+// Set while this thread holds q_mtx inside pick / cancel_locked: a task completed there releases
+// its dependents later (drain_ready), since making them ready takes q_mtx.
+static thread_local bool tl_queue_locked = false;
+
+struct QueueLockedScope {
+    bool saved;
+    QueueLockedScope() : saved(tl_queue_locked) { tl_queue_locked = true; }
+    ~QueueLockedScope() { tl_queue_locked = saved; }
+};
+
+// Ordering. Like the reference, which lets one task at a time lock an address space and queues
+// the others (pmAddressSpace::EnqueueForLock / ScanLockQueue, pmAddressSpace.cpp:500-530;
+// pmTask::LockAddressSpaces, pmTask.cpp:176-193), a task does not start before the earlier
+// tasks it conflicts with have completed. Conflict is per address space: either task writes it
+// (tasks that only read a space, such as served requests on one dataset, run together). A task
+// submitted as unordered (pmSubmitTaskWithOptions) skips the wait; later tasks still wait for it.
 void Runtime::submit(Task* t) {
     t->submit_time = now_secs();
     {
@@ -199,14 +215,66 @@ void Runtime::submit(Task* t) {
             t->start_time = t->end_time = t->submit_time;
             return;
         }
+    }
+    bool ready_now;
+    {
+        std::lock_guard<std::mutex> g(dep_mtx);
+        std::vector<std::pair<AddressSpace*, bool>> uses;     // distinct spaces, written or not
+        for (uint i = 0; i < t->spaces.size(); ++i) {
+            auto it = std::find_if(uses.begin(), uses.end(), [&](auto& u) { return u.first == t->spaces[i]; });
+            if (it == uses.end()) uses.push_back({t->spaces[i], t->writable(i)});
+            else it->second = it->second || t->writable(i);
+        }
+        std::vector<Task*> deps;
+        for (auto& u : uses) {
+            if (!t->unordered)
+                for (auto& h : u.first->holders)
+                    if ((h.second || u.second) && std::find(deps.begin(), deps.end(), h.first) == deps.end())
+                        deps.push_back(h.first);
+            u.first->holders.push_back({t, u.second});
+        }
+        for (Task* d : deps) d->dependents.push_back(t);
+        t->pending = (int)deps.size();
+        ready_now = t->pending == 0;
+    }
+    if (ready_now) make_ready(t);
+}
+
+void Runtime::make_ready(Task* t) {
+    // Subtasks that read or write host memory need the host copy current. This runs once the
+    // earlier conflicting tasks have completed, so it sees their results.
+    if (t->cpu_allowed || !t->gpu_resident) {
+        for (uint i = 0; i < t->spaces.size(); ++i) {
+            try {
+                Memory::get().sync_host(t->spaces[i]);
+                if (t->writable(i)) Memory::get().mark_host_written(t->spaces[i]);
+            } catch (Error& e) {
+                int expected = pmSuccess;
+                t->status.compare_exchange_strong(expected, (int)e.status);
+                set_last_error(e.what);
+            }
+        }
+    }
+    {
+        std::lock_guard<std::mutex> g(q_mtx);
         // keep `active` ordered by (priority, seq): lower priority value first
         auto it = std::find_if(active.begin(), active.end(), [&](Task* o) {
-            return o->details.priority > t->details.priority;
+            return o->details.priority > t->details.priority ||
+                   (o->details.priority == t->details.priority && o->seq > t->seq);
         });
         active.insert(it, t);
     }
     submitted.fetch_add(1, std::memory_order_release);
     q_cv.notify_all();
+}
+
+void Runtime::drain_ready() {
+    std::vector<Task*> ready;
+    {
+        std::lock_guard<std::mutex> g(dep_mtx);
+        ready.swap(deferred_ready);
+    }
+    for (Task* t : ready) make_ready(t);
 }
 
 void Runtime::wait(Task* t) {
@@ -222,6 +290,7 @@ void Runtime::wait(Task* t) {
 
 Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
     // caller holds q_mtx
+    QueueLockedScope locked;
     double now = 0;
     for (auto it = active.begin(); it != active.end(); ++it) {
         Task* t = *it;
@@ -260,6 +329,7 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
 }
 
 void Runtime::cancel_locked(Task* t, pmStatus status) {
+    QueueLockedScope locked;
     ulong f = t->next.exchange(t->subtask_count);
     if (f >= t->subtask_count) return;
     int expected = pmSuccess;
@@ -370,6 +440,21 @@ void Runtime::complete(Task* t) {
         if (pinned) as->gpu_pins--;
     }
     if (pinned) gpu_admitted--;
+    std::vector<Task*> ready;
+    {
+        std::lock_guard<std::mutex> g(dep_mtx);
+        for (AddressSpace* as : t->spaces) {
+            auto& h = as->holders;
+            h.erase(std::remove_if(h.begin(), h.end(), [&](auto& x) { return x.first == t; }), h.end());
+        }
+        for (Task* d : t->dependents)
+            if (--d->pending == 0) {
+                if (tl_queue_locked) deferred_ready.push_back(d);
+                else ready.push_back(d);
+            }
+        t->dependents.clear();
+    }
+    for (Task* d : ready) make_ready(d);
     if (t->dev_conf) {          // every kernel that read it has finished: its subtasks retired
         Memory::get().device_free(t->dev_conf, t->conf.size());
         t->dev_conf = nullptr;
@@ -405,8 +490,9 @@ void Runtime::cpu_loop(int idx) {
             });
             if (stopping) return;
             t = pick(false, first, count, 1);
-            if (!t) continue;
         }
+        drain_ready();      // tasks released by a time-out inside pick
+        if (!t) continue;
         for (ulong s = first; s < first + count; ++s) {
             std::unique_ptr<Subtask> st(new Subtask());
             st->id = s;
@@ -986,9 +1072,11 @@ void Runtime::gpu_loop() {
                 GpuSlot& s = *sp;
                 Task* t = s.task;
                 try {
+                    // the batch has finished (its event fired), so the scatter goes to the copy
+                    // stream: waiting on the batch stream would also wait for the younger batch
                     if (!s.dev_scatters.empty()) {
-                        scatter_resident(t, s, batch_stream);
-                        hipchk(uh_stream_sync(batch_stream), "device scatter");
+                        scatter_resident(t, s, copy_stream);
+                        hipchk(uh_stream_sync(copy_stream), "device scatter");
                     }
                 } catch (Error& e) { fail_task(t, e); }
                 retire_slot(this, s, false);
@@ -1015,6 +1103,7 @@ void Runtime::gpu_loop() {
         retire_batches(false);
         for (uint64_t uid : Memory::get().take_released()) cache_purge_uid(uid);
         admit_queued();
+        drain_ready();      // tasks released by a cancellation or time-out under the queue lock
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
         Task* t = nullptr;
         ulong first = 0, count = 0;

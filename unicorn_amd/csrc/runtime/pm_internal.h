@@ -97,6 +97,9 @@ struct AddressSpace {
     std::atomic<uint64_t> version{0};   // bumped on every write; keys the PLAIN-mode cache
     std::mutex mtx;
     bool released = false;
+    // Unfinished tasks using this space, in submission order, and whether each writes it
+    // (guarded by Runtime::dep_mtx).
+    std::vector<std::pair<struct Task*, bool>> holders;
 };
 
 class Memory {
@@ -182,6 +185,11 @@ struct Task {
     bool gpu_prepared = false;                   // mirrors made current by the HIP stub
     int hip_mode = PM_HIP_MODE_BATCHED;          // HIP stub mode at submission
     void* dev_conf = nullptr;                    // device copy of conf for kernel-kind callbacks
+    // Ordering against earlier tasks on the same address spaces (guarded by Runtime::dep_mtx):
+    // the task starts when `pending` earlier conflicting tasks have completed.
+    bool unordered = false;                      // submitted without waiting for earlier tasks
+    int pending = 0;
+    std::vector<Task*> dependents;               // later tasks waiting for this one
     double submit_time = 0, start_time = 0, end_time = 0;
     double deadline = 0;
     std::mutex mtx;
@@ -276,6 +284,11 @@ public:
     void* dev_info = nullptr;            // device copy of gpu_device (kernel-kind callbacks)
     void* dev_status = nullptr;          // kBatchStatusSlot + 1 pmStatus words, written by kernels
     std::atomic<int> gpu_admitted{0};    // GPU-resident tasks admitted and not yet complete
+    // ordering of tasks that share address spaces
+    std::mutex dep_mtx;
+    std::vector<Task*> deferred_ready;   // released while q_mtx was held; made ready by drain_ready
+    void make_ready(Task* t);            // host copies made current, then queued for the stubs
+    void drain_ready();                  // caller does not hold q_mtx
     // common helpers
     Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count);
     void finish_subtasks(Task* t, ulong n);
