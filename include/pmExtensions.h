@@ -71,6 +71,13 @@ typedef struct pmTaskSubmitOptions
 } pmTaskSubmitOptions;
 pmStatus pmSubmitTaskWithOptions(pmTaskDetails pTaskDetails, const pmTaskSubmitOptions* pOptions,
                                  pmTaskHandle* pTaskHandle);
+/* Pieces per split subtask for tasks submitted afterwards with canSplitCpuSubtasks or
+ * canSplitGpuSubtasks; 0 (the default) takes one piece per stub of the splitting device type, as
+ * the reference's split groups do (pmSubtaskSplitter.cpp:49-140), so that with one GPU, GPU
+ * splitting is off unless a count is set here. A split subtask's data distribution and subtask
+ * callbacks run once per piece with pSubtaskInfo.splitInfo = {splitId, splitCount}; the subtask
+ * completes with its last piece. */
+pmStatus pmSetSplitFactors(int pCpuPieces, int pGpuPieces);
 /* Select the HIP stub mode for tasks submitted afterwards. */
 pmStatus pmSetHipStubMode(int pMode);
 int pmGetHipStubMode();

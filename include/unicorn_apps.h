@@ -39,6 +39,10 @@ int ua_set_hip_graphs(int on);
 int ua_set_hip_depth(int depth);
 int ua_set_batch_limit(int subtasks);
 int ua_set_hbm_budget(uint64_t bytes);
+/* canSplitCpuSubtasks / canSplitGpuSubtasks of the imageFiltering and matrixMultiply tasks
+ * submitted afterwards (default 1 / 0, as the reference apps) and the pieces per split subtask
+ * (pmSetSplitFactors; 0 = one per stub of the splitting device type). */
+int ua_set_split(int cpu, int gpu, int cpu_pieces, int gpu_pieces);
 int ua_get_stats(ua_stats* s);
 int ua_reset_stats(void);
 int ua_set_kernel_timing(int every);           /* events around every every-th BATCHED flush (0: off) */

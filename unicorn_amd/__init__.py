@@ -81,6 +81,7 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_set_hip_mode": (i32, [i32]), "ua_set_hip_compact": (i32, [i32]),
         "ua_set_hip_graphs": (i32, [i32]), "ua_set_hip_depth": (i32, [i32]),
         "ua_set_batch_limit": (i32, [i32]), "ua_set_hbm_budget": (i32, [C.c_uint64]),
+        "ua_set_split": (i32, [i32, i32, i32, i32]),
         "ua_get_stats": (i32, [C.POINTER(ua_stats)]), "ua_reset_stats": (i32, []),
         "ua_last_error": (C.c_char_p, []),
         "ua_set_kernel_timing": (i32, [i32]),
@@ -170,6 +171,14 @@ def set_hip_mode(mode: int, compact: bool = False, graphs: bool = True) -> None:
 def set_hbm_budget(nbytes: int) -> None:
     """HBM budget of the runtime in bytes (0 = the default)."""
     _check(apps_lib().ua_set_hbm_budget(nbytes), "ua_set_hbm_budget")
+
+
+def set_split(cpu: bool = True, gpu: bool = False, cpu_pieces: int = 0, gpu_pieces: int = 0) -> None:
+    """Subtask splitting of imageFiltering / matrixMultiply tasks submitted afterwards: the
+    canSplitCpuSubtasks / canSplitGpuSubtasks flags and the pieces per split subtask (0 = one per
+    stub of the splitting device type). The defaults are the reference apps' settings."""
+    init()
+    _check(apps_lib().ua_set_split(int(cpu), int(gpu), cpu_pieces, gpu_pieces), "ua_set_split")
 
 
 def stats() -> dict:

@@ -49,6 +49,15 @@ inline int register_variants(const char* key, pmCallbacks cb, CallbackVariants& 
     return 0;
 }
 
+// Subtask splitting requested by the apps that support it (imageFiltering, matrixMultiply), as
+// the reference apps set pmTaskDetails::canSplitCpuSubtasks (imageFiltering.cpp:264,
+// matrixMultiplyBlas.cpp:219). CPU splitting is on by default, GPU splitting off.
+struct SplitRequest {
+    bool cpu = true;
+    bool gpu = false;
+};
+SplitRequest& split_request();
+
 // Runs one task to completion; returns its pmStatus (0 = success) and the wall time.
 inline int run_task(pmTaskDetails& d, double* secs) {
     pmTaskHandle th = nullptr;

@@ -14,6 +14,11 @@ void set_error(const std::string& s) {
     g_err = s;
 }
 
+SplitRequest& split_request() {
+    static SplitRequest r;
+    return r;
+}
+
 }  // namespace apps
 
 using namespace pm;
@@ -40,6 +45,11 @@ int ua_set_hip_compact(int compact) {
 int ua_set_hip_graphs(int on) { return (int)pmSetHipGraphs(on != 0); }
 int ua_set_hip_depth(int depth) { return (int)pmSetHipPipelineDepth(depth); }
 int ua_set_batch_limit(int n) { return (int)pmSetHipBatchLimit(n); }
+int ua_set_split(int cpu, int gpu, int cpu_pieces, int gpu_pieces) {
+    apps::split_request().cpu = cpu != 0;
+    apps::split_request().gpu = gpu != 0;
+    return (int)pmSetSplitFactors(cpu_pieces, gpu_pieces);
+}
 int ua_set_hbm_budget(uint64_t bytes) { return (int)pmSetHbmBudget((size_t)bytes); }
 
 int ua_get_stats(ua_stats* s) {

@@ -263,6 +263,8 @@ extern "C" int ua_if_run(void* h, int device, double* seconds) {
     mem[INPUT_MEM_INDEX] = pmTaskMem(in->in, READ_ONLY, SUBSCRIPTION_OPTIMAL);
     mem[OUTPUT_MEM_INDEX] = pmTaskMem(in->out, WRITE_ONLY, SUBSCRIPTION_OPTIMAL);
     pmTaskDetails d(&in->conf, sizeof(in->conf), mem, MAX_MEM_INDICES, g_cb.h[device], in->subtasks);
+    d.canSplitCpuSubtasks = apps::split_request().cpu;
+    d.canSplitGpuSubtasks = apps::split_request().gpu;
     d.policy = SLOW_START;
     return apps::run_task(d, seconds);
 }

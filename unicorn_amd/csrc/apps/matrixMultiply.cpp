@@ -191,6 +191,8 @@ extern "C" int ua_mm_run(void* h, int device, double* seconds) {
     mem[INPUT_MATRIX2_MEM_INDEX] = pmTaskMem(in->b, READ_ONLY, SUBSCRIPTION_OPTIMAL);
     mem[OUTPUT_MATRIX_MEM_INDEX] = pmTaskMem(in->c, WRITE_ONLY, SUBSCRIPTION_OPTIMAL);
     pmTaskDetails d(&in->conf, sizeof(in->conf), mem, MAX_MEM_INDICES, g_cb.h[device], in->subtasks);
+    d.canSplitCpuSubtasks = apps::split_request().cpu;
+    d.canSplitGpuSubtasks = apps::split_request().gpu;
     return apps::run_task(d, seconds);
 }
 

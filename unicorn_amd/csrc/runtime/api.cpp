@@ -334,6 +334,21 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
     if (!t->cpu_allowed && !t->gpu_allowed) raise(pmNoCompatibleDevice, "no device can run this task's callbacks");
     t->hip_mode = opt && opt->hipMode != -1 ? opt->hipMode : r->hip_mode.load();
     t->unordered = opt && opt->unordered;
+    // Splitting, as the reference's pmSubtaskSplitter: CPU splitting takes precedence over GPU
+    // splitting (pmSubtaskSplitter.cpp:47-143). Subtasks of a task with a reduction are not
+    // split here (the pieces would need one shared reduction buffer).
+    if (!cs->cb.dataReduction) {
+        if (d.canSplitCpuSubtasks && t->cpu_allowed) {
+            const int f = r->split_factor_cpu.load();
+            t->split_device = 1;
+            t->split_count = (uint)(f > 0 ? f : (int)r->cpu_devices.size());
+        } else if (d.canSplitGpuSubtasks && t->gpu_allowed) {
+            t->split_device = 2;
+            const int f = r->split_factor_gpu.load();
+            t->split_count = (uint)(f > 0 ? f : r->gpu_count);
+        }
+        if (t->split_count < 2) t->split_device = 0;
+    }
     t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
@@ -576,6 +591,15 @@ pmStatus pmGetHipKernelTime(double* ms, uint64_t* flushes, bool reset) {
     Runtime* r = Runtime::get();
     if (!r || !r->gpu_count) return pmNoCompatibleDevice;
     return uh_batch_timing(r->batch, ms, flushes, reset ? 1 : 0) ? pmGraphicsCardError : pmSuccess;
+}
+
+pmStatus pmSetSplitFactors(int pCpuPieces, int pGpuPieces) {
+    Runtime* r = Runtime::get();
+    if (!r) return pmInitializationFailure;
+    if (pCpuPieces < 0 || pGpuPieces < 0 || pCpuPieces > 4096 || pGpuPieces > 4096) return pmInvalidCommand;
+    r->split_factor_cpu = pCpuPieces;
+    r->split_factor_gpu = pGpuPieces;
+    return pmSuccess;
 }
 
 pmStatus pmGetHipKernelTimeByFamily(int family, double* ms, uint64_t* flushes, double* work, bool reset) {

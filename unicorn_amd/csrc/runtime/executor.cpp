@@ -94,6 +94,14 @@ pmSubscriptionVisibilityType Task::visibility(uint i, bool gpu) const {
     return v;
 }
 
+bool Task::has_work(bool gpu) {
+    if (gpu ? !gpu_allowed : !cpu_allowed) return false;
+    if (next.load() < subtask_count) return true;
+    if (split_device != (gpu ? 2 : 1)) return false;
+    std::lock_guard<std::mutex> g(split_mtx);
+    return !open_splits.empty();
+}
+
 Runtime* Runtime::get() { return inst_; }
 
 static int env_int(const char* name, int dflt) {
@@ -288,9 +296,10 @@ void Runtime::wait(Task* t) {
     stat_add(&pmHipStubStats::ns_wake, (uint64_t)((now_secs() - t->end_time) * 1e9));
 }
 
-Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
+Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count, pmSplitInfo* split) {
     // caller holds q_mtx
     QueueLockedScope locked;
+    if (split) *split = pmSplitInfo();
     double now = 0;
     for (auto it = active.begin(); it != active.end(); ++it) {
         Task* t = *it;
@@ -302,22 +311,47 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
                 // time-out: remaining subtasks are cancelled (reference TASK_TIME_OUT,
                 // pmTimedEventManager.cpp:92-95)
                 ulong f = t->next.exchange(t->subtask_count);
-                if (f < t->subtask_count) {
+                if (f < t->subtask_count || t->split_device) {
                     int expected = pmSuccess;
                     t->status.compare_exchange_strong(expected, (int)pmTimerFailure);
                     active.erase(it);
-                    finish_subtasks(t, t->subtask_count - f);
-                    return pick(gpu, first, count, max_count);
+                    drop_open_splits(t);
+                    if (f < t->subtask_count) finish_subtasks(t, t->subtask_count - f);
+                    return pick(gpu, first, count, max_count, split);
                 }
             }
         }
-        ulong f = t->next.load();
-        if (f >= t->subtask_count) continue;
-        ulong n = std::min(max_count, t->subtask_count - f);
-        t->next = f + n;
-        if (t->next.load() >= t->subtask_count) active.erase(it);
-        first = f;
-        count = n;
+        if (t->split_device == (gpu ? 2 : 1) && split) {
+            // one piece: the next split of an open subtask, or split 0 of a new one
+            std::lock_guard<std::mutex> sg(t->split_mtx);
+            if (!t->open_splits.empty()) {
+                auto& rec = t->open_splits.front();
+                first = rec.first;
+                *split = pmSplitInfo(rec.second, t->split_count);
+                if (++rec.second == t->split_count) t->open_splits.pop_front();
+            } else {
+                const ulong f = t->next.load();
+                if (f >= t->subtask_count) continue;
+                t->next = f + 1;
+                first = f;
+                *split = pmSplitInfo(0, t->split_count);
+                t->open_splits.push_back({f, 1});
+                t->split_left[f] = t->split_count;
+            }
+            count = 1;
+            if (t->next.load() >= t->subtask_count && t->open_splits.empty()) active.erase(it);
+        } else {
+            ulong f = t->next.load();
+            if (f >= t->subtask_count) continue;
+            ulong n = std::min(max_count, t->subtask_count - f);
+            t->next = f + n;
+            if (t->next.load() >= t->subtask_count) {
+                std::lock_guard<std::mutex> sg(t->split_mtx);
+                if (t->open_splits.empty()) active.erase(it);
+            }
+            first = f;
+            count = n;
+        }
         if (t->state == TaskState::Queued) {
             t->state = TaskState::Running;
             t->start_time = now_secs();
@@ -331,12 +365,45 @@ Task* Runtime::pick(bool gpu, ulong& first, ulong& count, ulong max_count) {
 void Runtime::cancel_locked(Task* t, pmStatus status) {
     QueueLockedScope locked;
     ulong f = t->next.exchange(t->subtask_count);
-    if (f >= t->subtask_count) return;
+    bool open;
+    {
+        std::lock_guard<std::mutex> sg(t->split_mtx);
+        open = !t->open_splits.empty();
+    }
+    if (f >= t->subtask_count && !open) return;
     int expected = pmSuccess;
     t->status.compare_exchange_strong(expected, (int)status);
     auto it = std::find(active.begin(), active.end(), t);
     if (it != active.end()) active.erase(it);
-    finish_subtasks(t, t->subtask_count - f);
+    drop_open_splits(t);
+    if (f < t->subtask_count) finish_subtasks(t, t->subtask_count - f);
+}
+
+void Runtime::drop_open_splits(Task* t) {
+    ulong done = 0;
+    {
+        std::lock_guard<std::mutex> sg(t->split_mtx);
+        for (auto& rec : t->open_splits) {
+            uint& left = t->split_left[rec.first];
+            left -= t->split_count - rec.second;      // pieces never handed out
+            if (left == 0) {
+                t->split_left.erase(rec.first);
+                ++done;
+            }
+        }
+        t->open_splits.clear();
+    }
+    if (done) finish_subtasks(t, done);
+}
+
+void Runtime::finish_piece(Task* t, ulong subtask, const pmSplitInfo& split) {
+    if (split.splitCount) {
+        std::lock_guard<std::mutex> sg(t->split_mtx);
+        auto it = t->split_left.find(subtask);
+        if (it == t->split_left.end() || --it->second > 0) return;
+        t->split_left.erase(it);
+    }
+    finish_subtasks(t, 1);
 }
 
 void Runtime::finish_subtasks(Task* t, ulong n) {
@@ -480,22 +547,24 @@ void Runtime::cpu_loop(int idx) {
     for (;;) {
         Task* t;
         ulong first = 0, count = 0;
+        pmSplitInfo split;
         {
             std::unique_lock<std::mutex> g(q_mtx);
             q_cv.wait(g, [&] {
                 if (stopping) return true;
                 for (Task* x : active)
-                    if (x->cpu_allowed && x->next.load() < x->subtask_count) return true;
+                    if (x->has_work(false)) return true;
                 return false;
             });
             if (stopping) return;
-            t = pick(false, first, count, 1);
+            t = pick(false, first, count, 1, &split);
         }
         drain_ready();      // tasks released by a time-out inside pick
         if (!t) continue;
         for (ulong s = first; s < first + count; ++s) {
             std::unique_ptr<Subtask> st(new Subtask());
             st->id = s;
+            st->split = split;
             try {
                 if (t->status.load() == pmSuccess) {
                     if (t->callbacks->cb.deviceSelection &&
@@ -524,7 +593,7 @@ void Runtime::cpu_loop(int idx) {
             } catch (Error& e) {
                 fail_task(t, e);
             }
-            finish_subtasks(t, 1);
+            finish_piece(t, s, split);
         }
     }
 }
@@ -946,6 +1015,8 @@ static void release_slot(GpuSlot& s) {
 // Completion of one PLAIN/RESIDENT slot (its event has fired).
 static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
     Task* t = s.task;
+    const ulong sid = s.st->id;
+    const pmSplitInfo split = s.st->split;
     try {
         if (plain) {
             // scatter written records from pinned memory into the host address space
@@ -974,7 +1045,7 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
     }
     stat_add(&pmHipStubStats::subtasks, 1);
     release_slot(s);
-    r->finish_subtasks(t, 1);
+    r->finish_piece(t, sid, split);
 }
 
 void Runtime::gpu_loop() {
@@ -1025,7 +1096,7 @@ void Runtime::gpu_loop() {
         {
             std::lock_guard<std::mutex> g(q_mtx);
             for (Task* x : active)
-                if (x->gpu_allowed && x->gpu_resident && !x->gpu_prepared && x->next.load() < x->subtask_count)
+                if (x->gpu_allowed && x->gpu_resident && !x->gpu_prepared && x->has_work(true))
                     cand.push_back(x);
         }
         for (Task* x : cand) {
@@ -1107,18 +1178,19 @@ void Runtime::gpu_loop() {
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
         Task* t = nullptr;
         ulong first = 0, count = 0;
+        pmSplitInfo split;
         int mode = PM_HIP_MODE_PLAIN;
         {
             std::unique_lock<std::mutex> g(q_mtx);
             auto pending = [&]() -> bool {
                 for (Task* x : active)
-                    if (x->gpu_allowed && x->next.load() < x->subtask_count) return true;
+                    if (x->has_work(true)) return true;
                 return false;
             };
             // first task whose subtasks can be staged now (admitted, or staged per subtask)
             auto front = [&]() -> Task* {
                 for (Task* x : active)
-                    if (x->gpu_allowed && x->next.load() < x->subtask_count && (!x->gpu_resident || x->gpu_prepared))
+                    if (x->has_work(true) && (!x->gpu_resident || x->gpu_prepared))
                         return x;
                 return nullptr;
             };
@@ -1146,15 +1218,16 @@ void Runtime::gpu_loop() {
             // the mode is the one the task was submitted under
             mode = front()->hip_mode;
             ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? batch_cap() : 1;
-            t = pick(true, first, count, max_take);
+            t = pick(true, first, count, max_take, &split);
             if (!t) continue;
         }
 
         if (mode == PM_HIP_MODE_BATCHED && t->gpu_resident) {
             // Gather subtasks of this and other active tasks into one batch.
             BatchRun run;
-            std::vector<std::pair<Task*, std::pair<ulong, ulong>>> claims;
-            claims.push_back({t, {first, count}});
+            struct Claim { Task* task; ulong first, count; pmSplitInfo split; };
+            std::vector<Claim> claims;
+            claims.push_back({t, first, count, split});
             ulong taken = count;
             const ulong cap = batch_cap();
             {
@@ -1164,25 +1237,27 @@ void Runtime::gpu_loop() {
                     Task* t2 = nullptr;
                     // only tasks that can be batched (GPU-resident)
                     for (Task* x : active) {
-                        if (x->gpu_allowed && x->gpu_resident && x->gpu_prepared && x->hip_mode == PM_HIP_MODE_BATCHED &&
-                            x->next.load() < x->subtask_count) { t2 = x; break; }
+                        if (x->gpu_resident && x->gpu_prepared && x->hip_mode == PM_HIP_MODE_BATCHED &&
+                            x->has_work(true)) { t2 = x; break; }
                     }
                     if (!t2) break;
-                    Task* got = pick(true, f2, c2, cap - taken);
+                    pmSplitInfo sp2;
+                    Task* got = pick(true, f2, c2, cap - taken, &sp2);
                     if (!got) break;
-                    claims.push_back({got, {f2, c2}});
+                    claims.push_back({got, f2, c2, sp2});
                     taken += c2;
                 }
             }
             const double s0 = now_secs();
             for (auto& cl : claims) {
-                Task* tt = cl.first;
+                Task* tt = cl.task;
                 try { ensure_task_ready(tt); } catch (Error& e) { fail_task(tt, e); }
-                for (ulong sid = cl.second.first; sid < cl.second.first + cl.second.second; ++sid) {
+                for (ulong sid = cl.first; sid < cl.first + cl.count; ++sid) {
                     std::unique_ptr<GpuSlot> sp(new GpuSlot());
                     sp->task = tt;
                     sp->st.reset(new Subtask());
                     sp->st->id = sid;
+                    sp->st->split = cl.split;
                     sp->busy = true;
                     if (tt->status.load() == pmSuccess) {
                         try {
@@ -1205,7 +1280,7 @@ void Runtime::gpu_loop() {
             try {
                 hipchk(uh_batch_flush(batch, use_graphs.load()), "batch flush");
             } catch (Error& e) {
-                for (auto& cl : claims) fail_task(cl.first, e);
+                for (auto& cl : claims) fail_task(cl.task, e);
             }
             stat_add(&pmHipStubStats::ns_stage, (uint64_t)((f0 - s0) * 1e9));
             stat_add(&pmHipStubStats::ns_flush, (uint64_t)((now_secs() - f0) * 1e9));
@@ -1254,6 +1329,7 @@ void Runtime::gpu_loop() {
             s.task = t;
             s.st.reset(new Subtask());
             s.st->id = sid;
+            s.st->split = split;
             s.stream = gpu_streams[si_idx];
             s.event = gpu_events[si_idx];
             s.busy = true;

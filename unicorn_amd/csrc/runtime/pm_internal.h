@@ -190,6 +190,15 @@ struct Task {
     bool unordered = false;                      // submitted without waiting for earlier tasks
     int pending = 0;
     std::vector<Task*> dependents;               // later tasks waiting for this one
+    // Splitting (reference pmSubtaskSplitter.cpp): on the splitting device type a subtask runs as
+    // split_count pieces, each called with its pmSplitInfo, and completes with its last piece.
+    int split_device = 0;                        // 0: no splitting, 1: CPU stubs, 2: the HIP stub
+    uint split_count = 1;
+    std::mutex split_mtx;
+    std::deque<std::pair<ulong, uint>> open_splits;   // (subtask, next split id) not all handed out
+    std::map<ulong, uint> split_left;                 // subtask -> pieces not yet finished
+    // Whether a stub of this kind can take work from the task now (caller holds Runtime::q_mtx).
+    bool has_work(bool gpu);
     double submit_time = 0, start_time = 0, end_time = 0;
     double deadline = 0;
     std::mutex mtx;
@@ -274,6 +283,8 @@ public:
     std::atomic<int> pipeline_depth{4};
     std::atomic<int> batch_limit{4096};
     std::atomic<int> cpu_workers{0};
+    std::atomic<int> split_factor_cpu{0};    // pieces per split CPU subtask (0: one per CPU stub)
+    std::atomic<int> split_factor_gpu{0};    // pieces per split GPU subtask (0: one per GPU stub)
     pmHipStubStats stats{};
     std::mutex stats_mtx;
     void* gpu_streams[16] = {};
@@ -290,8 +301,14 @@ public:
     void make_ready(Task* t);            // host copies made current, then queued for the stubs
     void drain_ready();                  // caller does not hold q_mtx
     // common helpers
-    Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count);
+    // Takes work for a stub: subtasks [first, first + count), or one piece of a split subtask
+    // (count 1, *split set; split->splitCount stays 0 otherwise).
+    Task* pick(bool gpu, ulong& first, ulong& count, ulong max_count, pmSplitInfo* split = nullptr);
     void finish_subtasks(Task* t, ulong n);
+    // A subtask or one piece of it finished.
+    void finish_piece(Task* t, ulong subtask, const pmSplitInfo& split);
+    // Drops the pieces of open split records that were never handed out (cancellation).
+    void drop_open_splits(Task* t);
     void complete(Task* t);
     // Cancels the unpicked subtasks of a queued task with `status` (caller holds q_mtx).
     void cancel_locked(Task* t, pmStatus status);
