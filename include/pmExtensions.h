@@ -54,6 +54,8 @@ typedef struct pmHipStubStats
     uint64_t ns_wait;           /* BATCHED: stub thread blocked on the oldest batch */
     uint64_t ns_retire;         /* BATCHED: retiring completed batches (scatter, completion) */
     uint64_t ns_wake;           /* task completion -> blocked waiter resumed, per wait */
+    uint64_t ma_copies;         /* multi-assigned copies of CPU subtasks run by the HIP stub */
+    uint64_t ma_wins;           /* of those, copies that finished first and committed */
 } pmHipStubStats;
 
 #ifdef __cplusplus

@@ -28,6 +28,7 @@ typedef struct ua_stats {
     uint64_t subtasks, batches, h2d_bytes, d2h_bytes, d2d_bytes, cache_hits, evictions;
     uint64_t plain_launches, persistent_launches, queue_items;
     uint64_t ns_queue, ns_stage, ns_flush, ns_wait, ns_retire, ns_wake;   /* see pmHipStubStats */
+    uint64_t ma_copies, ma_wins;
 } ua_stats;
 
 int ua_init(int cpu_stubs);                   /* pmInitialize; cpu_stubs 0 = one per core */
@@ -43,6 +44,9 @@ int ua_set_hbm_budget(uint64_t bytes);
  * submitted afterwards (default 1 / 0, as the reference apps) and the pieces per split subtask
  * (pmSetSplitFactors; 0 = one per stub of the splitting device type). */
 int ua_set_split(int cpu, int gpu, int cpu_pieces, int gpu_pieces);
+/* multiAssignEnabled of the imageFiltering and matrixMultiply tasks submitted afterwards
+ * (default 1, as the reference). */
+int ua_set_multi_assign(int on);
 int ua_get_stats(ua_stats* s);
 int ua_reset_stats(void);
 int ua_set_kernel_timing(int every);           /* events around every every-th BATCHED flush (0: off) */

@@ -59,7 +59,7 @@ class ua_stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "subtasks", "batches", "h2d_bytes", "d2h_bytes", "d2d_bytes", "cache_hits", "evictions",
         "plain_launches", "persistent_launches", "queue_items",
-        "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake")]
+        "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake", "ma_copies", "ma_wins")]
 
 
 class ua_if_config(C.Structure):
@@ -81,7 +81,7 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_set_hip_mode": (i32, [i32]), "ua_set_hip_compact": (i32, [i32]),
         "ua_set_hip_graphs": (i32, [i32]), "ua_set_hip_depth": (i32, [i32]),
         "ua_set_batch_limit": (i32, [i32]), "ua_set_hbm_budget": (i32, [C.c_uint64]),
-        "ua_set_split": (i32, [i32, i32, i32, i32]),
+        "ua_set_split": (i32, [i32, i32, i32, i32]), "ua_set_multi_assign": (i32, [i32]),
         "ua_get_stats": (i32, [C.POINTER(ua_stats)]), "ua_reset_stats": (i32, []),
         "ua_last_error": (C.c_char_p, []),
         "ua_set_kernel_timing": (i32, [i32]),
@@ -179,6 +179,12 @@ def set_split(cpu: bool = True, gpu: bool = False, cpu_pieces: int = 0, gpu_piec
     stub of the splitting device type). The defaults are the reference apps' settings."""
     init()
     _check(apps_lib().ua_set_split(int(cpu), int(gpu), cpu_pieces, gpu_pieces), "ua_set_split")
+
+
+def set_multi_assign(on: bool = True) -> None:
+    """multiAssignEnabled of imageFiltering / matrixMultiply tasks submitted afterwards."""
+    init()
+    _check(apps_lib().ua_set_multi_assign(int(on)), "ua_set_multi_assign")
 
 
 def stats() -> dict:

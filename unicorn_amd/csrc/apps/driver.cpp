@@ -50,6 +50,10 @@ int ua_set_split(int cpu, int gpu, int cpu_pieces, int gpu_pieces) {
     apps::split_request().gpu = gpu != 0;
     return (int)pmSetSplitFactors(cpu_pieces, gpu_pieces);
 }
+int ua_set_multi_assign(int on) {
+    apps::split_request().multi_assign = on != 0;
+    return 0;
+}
 int ua_set_hbm_budget(uint64_t bytes) { return (int)pmSetHbmBudget((size_t)bytes); }
 
 int ua_get_stats(ua_stats* s) {
@@ -69,6 +73,8 @@ int ua_get_stats(ua_stats* s) {
     s->ns_wait = h.ns_wait;
     s->ns_retire = h.ns_retire;
     s->ns_wake = h.ns_wake;
+    s->ma_copies = h.ma_copies;
+    s->ma_wins = h.ma_wins;
     s->plain_launches = s->persistent_launches = s->queue_items = 0;
     if (pmHasHipDevice()) uh_launch_counters(&s->plain_launches, &s->persistent_launches, &s->queue_items);
     return 0;

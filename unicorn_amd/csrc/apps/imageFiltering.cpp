@@ -265,6 +265,7 @@ extern "C" int ua_if_run(void* h, int device, double* seconds) {
     pmTaskDetails d(&in->conf, sizeof(in->conf), mem, MAX_MEM_INDICES, g_cb.h[device], in->subtasks);
     d.canSplitCpuSubtasks = apps::split_request().cpu;
     d.canSplitGpuSubtasks = apps::split_request().gpu;
+    d.multiAssignEnabled = apps::split_request().multi_assign;
     d.policy = SLOW_START;
     return apps::run_task(d, seconds);
 }

@@ -349,6 +349,10 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
         }
         if (t->split_count < 2) t->split_device = 0;
     }
+    // multi-assign (the reference's default, pmTaskDetails::multiAssignEnabled) needs both stub
+    // kinds and whole subtasks; a reduced task's copies would need a second shadow per subtask
+    t->multi_assign = d.multiAssignEnabled && t->cpu_allowed && t->gpu_allowed && !t->split_device &&
+                      !cs->cb.dataReduction;
     t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
@@ -360,6 +364,10 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
     // earlier tasks on its address spaces have completed
     Task* raw = t.release();
     *th = raw;
+    if (raw->multi_assign) {
+        std::lock_guard<std::mutex> g(r->ma_list_mtx);
+        r->ma_tasks.push_back(raw);
+    }
     r->submit(raw);
     PM_API_END
 }
@@ -379,7 +387,16 @@ pmStatus pmWaitForTaskCompletion(pmTaskHandle th) {
 
 pmStatus pmReleaseTask(pmTaskHandle th) {
     pmStatus s = pmWaitForTaskCompletion(th);
-    if (th) delete (Task*)th;
+    if (th) {
+        Task* t = (Task*)th;
+        {
+            // multi-assigned copies that lost may still be running
+            std::unique_lock<std::mutex> g(t->mtx);
+            t->cv.wait(g, [&] { return t->runners == 0; });
+        }
+        if (t->dev_conf) rt::Memory::get().device_free(t->dev_conf, t->conf.size());
+        delete t;
+    }
     return s;
 }
 

@@ -102,6 +102,15 @@ bool Task::has_work(bool gpu) {
     return !open_splits.empty();
 }
 
+bool Task::claim_commit(ulong sid, bool gpu_copy) {
+    std::lock_guard<std::mutex> g(ma_mtx);
+    auto it = cpu_runs.find(sid);
+    if (it == cpu_runs.end()) return !gpu_copy;   // a HIP copy after the CPU run ended: it lost
+    if (it->second.committed) return false;
+    it->second.committed = true;
+    return true;
+}
+
 Runtime* Runtime::get() { return inst_; }
 
 static int env_int(const char* name, int dflt) {
@@ -406,6 +415,76 @@ void Runtime::finish_piece(Task* t, ulong subtask, const pmSplitInfo& split) {
     finish_subtasks(t, 1);
 }
 
+void Runtime::runner_enter(Task* t) {
+    {
+        std::lock_guard<std::mutex> g(t->mtx);
+        t->runners++;
+    }
+    for (AddressSpace* as : t->spaces) {
+        std::lock_guard<std::mutex> g(as->mtx);
+        as->runners++;
+    }
+}
+
+void Runtime::runner_exit(Task* t) {
+    for (AddressSpace* as : t->spaces) {
+        std::lock_guard<std::mutex> g(as->mtx);
+        if (--as->runners == 0) as->runners_cv.notify_all();
+    }
+    // last touch of the task: once runners is 0 its owner may delete it
+    std::lock_guard<std::mutex> g(t->mtx);
+    if (--t->runners == 0) t->cv.notify_all();
+}
+
+// Picks a subtask that a CPU stub is executing for the idle HIP stub to copy (reference
+// pmExecutionStub.cpp:1082-1143). Rates are this task's: the HIP stub's time per subtask Tg
+// (wall time from its first staging to its last retirement over the subtasks it finished) and
+// a CPU stub's mean subtask time Tc. A copy starts when 1.05 * Tg is below the CPU's expected
+// remaining time Tc - elapsed (SUBTASK_TRANSFER_OVERHEAD); before any CPU subtask has finished,
+// once the CPU run has taken 0.95 * Tg (MA_WAIT_FACTOR_LENIENT, a GPU asking a CPU on the same
+// host). A HIP stub that has finished no subtask of the task has no rate and copies nothing, as
+// in the reference. retry_in: seconds after which a waiting run may qualify (0: none).
+Task* Runtime::ma_candidate(ulong& sid, double& retry_in) {
+    retry_in = 0;
+    std::lock_guard<std::mutex> lg(ma_list_mtx);
+    const double now = now_secs();
+    for (Task* t : ma_tasks) {
+        if (t->status.load() != pmSuccess) continue;
+        std::lock_guard<std::mutex> g(t->ma_mtx);
+        if (!t->gpu_finished || t->cpu_runs.empty()) continue;
+        const double tg = (t->gpu_last - t->gpu_first) / (double)t->gpu_finished;
+        const double tc = t->cpu_finished ? t->cpu_secs / (double)t->cpu_finished : 0.0;
+        for (auto& kv : t->cpu_runs) {
+            Task::CpuRun& run = kv.second;
+            if (run.duplicated || run.committed) continue;
+            const double elapsed = now - run.start;
+            bool go;
+            if (tc == 0.0) {
+                go = elapsed >= 0.95 * tg;
+                if (!go) {
+                    const double w = 0.95 * tg - elapsed;
+                    retry_in = retry_in == 0 ? w : std::min(retry_in, w);
+                }
+            } else {
+                go = 1.05 * tg < tc - elapsed;
+            }
+            if (!go) continue;
+            run.duplicated = true;
+            sid = kv.first;
+            {
+                std::lock_guard<std::mutex> tm(t->mtx);
+                t->runners++;
+            }
+            for (AddressSpace* as : t->spaces) {
+                std::lock_guard<std::mutex> am(as->mtx);
+                as->runners++;
+            }
+            return t;
+        }
+    }
+    return nullptr;
+}
+
 void Runtime::finish_subtasks(Task* t, ulong n) {
     ulong d = t->done.fetch_add(n) + n;
     if (d == t->subtask_count) complete(t);
@@ -493,6 +572,10 @@ static void commit_reduction(Task* t) {
 }
 
 void Runtime::complete(Task* t) {
+    if (t->multi_assign) {
+        std::lock_guard<std::mutex> g(ma_list_mtx);
+        ma_tasks.erase(std::remove(ma_tasks.begin(), ma_tasks.end(), t), ma_tasks.end());
+    }
     try {
         commit_reduction(t);
     } catch (Error& e) {
@@ -522,7 +605,14 @@ void Runtime::complete(Task* t) {
         t->dependents.clear();
     }
     for (Task* d : ready) make_ready(d);
-    if (t->dev_conf) {          // every kernel that read it has finished: its subtasks retired
+    bool copies_running;
+    {
+        std::lock_guard<std::mutex> g(t->mtx);
+        copies_running = t->runners > 0;
+    }
+    // every kernel that read the device conf has finished once its subtasks retired; a
+    // multi-assigned copy still running keeps it (pmReleaseTask frees it after the copy ends)
+    if (t->dev_conf && !copies_running) {
         Memory::get().device_free(t->dev_conf, t->conf.size());
         t->dev_conf = nullptr;
     }
@@ -565,6 +655,15 @@ void Runtime::cpu_loop(int idx) {
             std::unique_ptr<Subtask> st(new Subtask());
             st->id = s;
             st->split = split;
+            const bool ma = t->multi_assign;
+            double t0 = 0;
+            if (ma) {
+                runner_enter(t);
+                t0 = now_secs();
+                std::lock_guard<std::mutex> g(t->ma_mtx);
+                t->cpu_runs[s].start = t0;
+            }
+            bool commit = true, claimed = false;
             try {
                 if (t->status.load() == pmSuccess) {
                     if (t->callbacks->cb.deviceSelection &&
@@ -572,28 +671,60 @@ void Runtime::cpu_loop(int idx) {
                         raise(pmNoCompatibleDevice, "device selection rejected the CPU stub");
                     run_distribution(t, *st, dev);
                     std::vector<std::vector<uint8_t>> compact;
-                    pmSubtaskInfo si = host_subtask_info(t, *st, compact);
+                    pmSubtaskInfo si = host_subtask_info(t, *st, compact, ma);
                     pmStatus rs;
                     {
                         SubtaskScope scope(t, st.get(), dev->deviceHandle);
                         rs = t->callbacks->cb.subtask_cpu(t->info(), *dev, si);
                     }
                     if (rs != pmSuccess) raise(rs, "subtask_cpu callback failed");
-                    // COMPACT views: scatter written records back
-                    for (uint i = 0; i < t->spaces.size(); ++i) {
-                        if (!t->writable(i) || si.memInfo[i].visibilityType != SUBSCRIPTION_COMPACT) continue;
-                        CompactLayout cl = make_compact(st->mem[i]);
-                        for (auto& kv : st->mem[i].write().map()) {
-                            size_t co = cl.compact_offset(kv.first);
-                            memcpy(t->spaces[i]->host + kv.first, compact[i].data() + co, kv.second);
+                    // a multi-assigned subtask writes the address space only if this copy is first
+                    if (ma) {
+                        commit = t->claim_commit(s, false);
+                        claimed = true;
+                    }
+                    if (commit) {
+                        for (uint i = 0; i < t->spaces.size(); ++i) {
+                            if (!t->writable(i)) continue;
+                            if (si.memInfo[i].visibilityType == SUBSCRIPTION_COMPACT) {
+                                // COMPACT views: scatter written records back
+                                CompactLayout cl = make_compact(st->mem[i]);
+                                for (auto& kv : st->mem[i].write().map()) {
+                                    size_t co = cl.compact_offset(kv.first);
+                                    memcpy(t->spaces[i]->host + kv.first, compact[i].data() + co, kv.second);
+                                }
+                            } else if (ma && st->shadow[i]) {
+                                // private NATURAL view: copy its written records
+                                const size_t base = st->mem[i].unified().offset;
+                                for (auto& kv : st->mem[i].write().map())
+                                    memcpy(t->spaces[i]->host + kv.first, (uint8_t*)st->shadow[i] + (kv.first - base),
+                                           kv.second);
+                            }
                         }
                     }
                     if (t->callbacks->cb.dataReduction) reduce_into(t, std::move(st));
                 }
             } catch (Error& e) {
-                fail_task(t, e);
+                if (ma && !claimed) {
+                    commit = t->claim_commit(s, false);
+                    claimed = true;
+                }
+                if (commit) fail_task(t, e);
             }
-            finish_piece(t, s, split);
+            if (ma && !claimed) commit = t->claim_commit(s, false);
+            if (ma) {
+                if (st)
+                    for (uint i = 0; i < MAX_MEM_SECTIONS_PER_TASK; ++i)
+                        if (st->shadow[i] && !st->shadow_on_device[i]) { free(st->shadow[i]); st->shadow[i] = nullptr; }
+                std::lock_guard<std::mutex> g(t->ma_mtx);
+                t->cpu_runs.erase(s);
+                if (commit) {
+                    t->cpu_secs += now_secs() - t0;
+                    t->cpu_finished++;
+                }
+            }
+            if (commit) finish_piece(t, s, split);
+            if (ma) runner_exit(t);
         }
     }
 }
@@ -626,6 +757,8 @@ struct GpuSlot {
     std::vector<std::pair<uint64_t, std::vector<size_t>>> cache_refs;
     bool plain = false;     // staged the PLAIN way (host gather / scatter)
     bool busy = false;
+    bool ma_copy = false;   // a multi-assigned copy of a CPU stub's subtask
+    bool copy_failed = false;
 };
 
 // Read-only device buffers of the PLAIN mode, keyed by (address space, exact records)
@@ -1010,6 +1143,7 @@ static void release_slot(GpuSlot& s) {
     s.busy = false;
     s.task = nullptr;
     s.st.reset();
+    s.ma_copy = s.copy_failed = false;
 }
 
 // Completion of one PLAIN/RESIDENT slot (its event has fired).
@@ -1017,8 +1151,11 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
     Task* t = s.task;
     const ulong sid = s.st->id;
     const pmSplitInfo split = s.st->split;
+    // a multi-assigned copy writes the address space only if it finished first
+    const bool copy = s.ma_copy;
+    const bool win = !copy || (!s.copy_failed && t->claim_commit(sid, true));
     try {
-        if (plain) {
+        if (plain && win) {
             // scatter written records from pinned memory into the host address space
             // (reference CopyDataFromPinnedBuffers, pmExecutionStub.cpp:3752-3847); records of
             // reduced spaces go to a host shadow of the subtask instead
@@ -1043,9 +1180,17 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
     } catch (Error& e) {
         fail_task(t, e);
     }
+    if (t->multi_assign && win) {
+        std::lock_guard<std::mutex> g(t->ma_mtx);
+        t->gpu_last = now_secs();
+        t->gpu_finished++;
+    }
     stat_add(&pmHipStubStats::subtasks, 1);
+    if (copy) stat_add(&pmHipStubStats::ma_copies, 1);
+    if (copy && win) stat_add(&pmHipStubStats::ma_wins, 1);
     release_slot(s);
-    r->finish_piece(t, sid, split);
+    if (win) r->finish_piece(t, sid, split);
+    if (copy) r->runner_exit(t);
 }
 
 void Runtime::gpu_loop() {
@@ -1180,6 +1325,7 @@ void Runtime::gpu_loop() {
         ulong first = 0, count = 0;
         pmSplitInfo split;
         int mode = PM_HIP_MODE_PLAIN;
+        bool ma_copy = false;
         {
             std::unique_lock<std::mutex> g(q_mtx);
             auto pending = [&]() -> bool {
@@ -1195,16 +1341,36 @@ void Runtime::gpu_loop() {
                 return nullptr;
             };
             if (inflight == 0 && batches.empty()) {
-                if (!stopping && !pending() && spin_secs() > 0) {
-                    const uint64_t seen = submitted.load(std::memory_order_acquire);
+                if (!stopping && !pending()) {
+                    // idle: run a copy of a subtask a CPU stub is executing (multi-assign)
                     g.unlock();
-                    const double until = now_secs() + spin_secs();
-                    while (submitted.load(std::memory_order_acquire) == seen && now_secs() < until) cpu_relax();
+                    double retry = 0;
+                    ulong sid = 0;
+                    Task* mt = ma_candidate(sid, retry);
                     g.lock();
+                    if (mt) {
+                        t = mt;
+                        first = sid;
+                        count = 1;
+                        ma_copy = true;
+                    } else if (retry > 0) {
+                        if (!stopping && !pending())
+                            q_cv.wait_for(g, std::chrono::microseconds((long)(retry * 1e6) + 1));
+                        continue;
+                    }
                 }
-                q_cv.wait(g, [&] { return stopping || pending(); });
-                if (stopping && !pending()) break;
-                if (!front()) continue;   // admission runs first
+                if (!ma_copy) {
+                    if (!stopping && !pending() && spin_secs() > 0) {
+                        const uint64_t seen = submitted.load(std::memory_order_acquire);
+                        g.unlock();
+                        const double until = now_secs() + spin_secs();
+                        while (submitted.load(std::memory_order_acquire) == seen && now_secs() < until) cpu_relax();
+                        g.lock();
+                    }
+                    q_cv.wait(g, [&] { return stopping || pending(); });
+                    if (stopping && !pending()) break;
+                    if (!front()) continue;   // admission runs first
+                }
             } else if (!front() || inflight >= depth || (int)batches.size() >= 2) {
                 g.unlock();
                 // wait for the oldest work to finish
@@ -1215,11 +1381,13 @@ void Runtime::gpu_loop() {
                 }
                 continue;
             }
-            // the mode is the one the task was submitted under
-            mode = front()->hip_mode;
-            ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? batch_cap() : 1;
-            t = pick(true, first, count, max_take, &split);
-            if (!t) continue;
+            if (!ma_copy) {
+                // the mode is the one the task was submitted under
+                mode = front()->hip_mode;
+                ulong max_take = (mode == PM_HIP_MODE_BATCHED && front()->gpu_resident) ? batch_cap() : 1;
+                t = pick(true, first, count, max_take, &split);
+                if (!t) continue;
+            }
         }
 
         if (mode == PM_HIP_MODE_BATCHED && t->gpu_resident) {
@@ -1335,6 +1503,11 @@ void Runtime::gpu_loop() {
             s.busy = true;
             const bool plain = !t->gpu_resident || mode == PM_HIP_MODE_PLAIN;
             s.plain = plain;
+            s.ma_copy = ma_copy;
+            if (t->multi_assign && !ma_copy) {
+                std::lock_guard<std::mutex> g(t->ma_mtx);
+                if (t->gpu_first == 0) t->gpu_first = now_secs();
+            }
             if (t->status.load() == pmSuccess) {
                 try {
                     if (t->callbacks->cb.deviceSelection && !t->callbacks->cb.deviceSelection(t->info(), *gpu_device))
@@ -1370,7 +1543,9 @@ void Runtime::gpu_loop() {
                             if (t->writable(i) && !t->is_reduced(i)) Memory::get().mark_device_written(t->spaces[i]);
                     }
                 } catch (Error& e) {
-                    fail_task(t, e);
+                    // a failed copy is dropped: the CPU stub's run of the subtask still counts
+                    if (s.ma_copy) s.copy_failed = true;
+                    else fail_task(t, e);
                 }
             }
             hipchk(uh_event_record(s.event, s.stream), "subtask event");

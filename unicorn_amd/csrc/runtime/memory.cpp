@@ -52,6 +52,11 @@ AddressSpace* Memory::create(size_t length, size_t rows, size_t cols) {
 
 void Memory::release(AddressSpace* as) {
     {
+        // a multi-assigned copy that lost may still be reading it
+        std::unique_lock<std::mutex> g(as->mtx);
+        as->runners_cv.wait(g, [&] { return as->runners == 0; });
+    }
+    {
         std::lock_guard<std::mutex> g(list_mtx);
         for (auto it = spaces.begin(); it != spaces.end(); ++it)
             if (*it == as) { spaces.erase(it); break; }

@@ -100,6 +100,9 @@ struct AddressSpace {
     // Unfinished tasks using this space, in submission order, and whether each writes it
     // (guarded by Runtime::dep_mtx).
     std::vector<std::pair<struct Task*, bool>> holders;
+    // multi-assigned copies still reading it (a copy that lost keeps running); release waits
+    int runners = 0;
+    std::condition_variable runners_cv;
 };
 
 class Memory {
@@ -199,6 +202,28 @@ struct Task {
     std::map<ulong, uint> split_left;                 // subtask -> pieces not yet finished
     // Whether a stub of this kind can take work from the task now (caller holds Runtime::q_mtx).
     bool has_work(bool gpu);
+    // Multi-assign (reference pmExecutionStub.cpp:1058-1170, at most one extra copy as
+    // MAX_SUBTASK_MULTI_ASSIGN_COUNT 2): the idle HIP stub may run a copy of a subtask that a CPU
+    // stub is executing; the copy that finishes first commits its output and completes the
+    // subtask, the other's output is dropped. CPU subtasks of such a task write into private
+    // shadows that are copied to the address space on commit.
+    bool multi_assign = false;
+    struct CpuRun {
+        double start = 0;
+        bool duplicated = false;                 // a HIP copy was started
+        bool committed = false;
+    };
+    std::mutex ma_mtx;
+    std::map<ulong, CpuRun> cpu_runs;            // subtasks executing on CPU stubs
+    double cpu_secs = 0;                         // summed CPU subtask durations
+    ulong cpu_finished = 0;
+    double gpu_first = 0, gpu_last = 0;          // first HIP staging, last HIP retirement
+    ulong gpu_finished = 0;
+    // copies still executing (CPU subtasks, losers included, and HIP copies); the task object
+    // and its address spaces stay alive until they end (guarded by mtx, signalled on cv)
+    int runners = 0;
+    // First finisher of subtask `sid` takes the commit; false when the other copy already has.
+    bool claim_commit(ulong sid, bool gpu_copy);
     double submit_time = 0, start_time = 0, end_time = 0;
     double deadline = 0;
     std::mutex mtx;
@@ -310,6 +335,12 @@ public:
     // Drops the pieces of open split records that were never handed out (cancellation).
     void drop_open_splits(Task* t);
     void complete(Task* t);
+    // multi-assign: tasks whose CPU subtasks the HIP stub may copy
+    std::mutex ma_list_mtx;
+    std::vector<Task*> ma_tasks;
+    Task* ma_candidate(ulong& sid, double& retry_in);
+    void runner_enter(Task* t);
+    void runner_exit(Task* t);
     // Cancels the unpicked subtasks of a queued task with `status` (caller holds q_mtx).
     void cancel_locked(Task* t, pmStatus status);
 private:
@@ -319,7 +350,9 @@ private:
 // Runs the dataDistribution callback of one subtask and fills its subscriptions.
 void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev);
 // Builds the pmSubtaskInfo of a CPU (host, NATURAL/COMPACT) subtask; may allocate shadows.
-pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs);
+// private_writes: writable NATURAL views point into shadows (multi-assigned CPU subtasks).
+pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs,
+                                bool private_writes = false);
 // Reduction helpers
 bool is_builtin_reduction(pmDataReductionCallback cb, pmReductionOpType* op, pmReductionDataType* dt);
 void reduce_host(void* dst, const void* src, size_t bytes, pmReductionOpType op, pmReductionDataType dt);

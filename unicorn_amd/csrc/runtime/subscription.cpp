@@ -174,7 +174,8 @@ void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
 
 // Host view of a subtask for the CPU stubs. NATURAL views point into the address space (or a
 // shadow buffer for reduced spaces); COMPACT views are packed into compact_bufs.
-pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs) {
+pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs,
+                                bool private_writes) {
     pmSubtaskInfo si;
     si.subtaskId = st.id;
     si.memCount = (uint)t->spaces.size();
@@ -195,7 +196,7 @@ pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<ui
                 pmSubscriptionInfo u = ms.unified();
                 if (u.length) {
                     uint8_t* base;
-                    if (t->is_reduced(i)) {
+                    if (t->is_reduced(i) || private_writes) {
                         if (!st.shadow[i]) {
                             st.shadow[i] = malloc(u.length);
                             if (!st.shadow[i]) raise(pmMemoryError, "shadow allocation");
