@@ -560,7 +560,7 @@ extern "C" int ua_lu_solve(void* h, const double* b, int nrhs, int max_steps, in
     pmTaskMem mem[3];
     mem[0] = pmTaskMem(in->mat, READ_ONLY, SUBSCRIPTION_NATURAL);
     mem[1] = pmTaskMem(in->orig, READ_ONLY, SUBSCRIPTION_NATURAL);
-    mem[2] = pmTaskMem(in->state, READ_WRITE, SUBSCRIPTION_NATURAL);
+    mem[2] = pmTaskMem(in->state, READ_WRITE, SUBSCRIPTION_NATURAL, true);   // one subtask: disjoint
     double t = 0, s = 0;
     conf.taskStep = REFINE_INIT;
     pmTaskDetails d(&conf, sizeof(conf), mem, 3, g_cb[3].h[device], 1);

@@ -353,7 +353,9 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
     // kinds and whole subtasks; a reduced task's copies would need a second shadow per subtask
     t->multi_assign = d.multiAssignEnabled && t->cpu_allowed && t->gpu_allowed && !t->split_device &&
                       !cs->cb.dataReduction;
-    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN;
+    for (uint i = 0; i < d.taskMemCount; ++i) t->has_deferred |= t->deferred(i);
+    // held-back writes are kept on the host: such a task stages its GPU subtasks the PLAIN way
+    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN && !t->has_deferred;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
         AddressSpace* as = t->spaces[i];

@@ -83,6 +83,18 @@ bool Task::write_only(uint i) const {
 bool Task::is_reduced(uint i) const {
     return callbacks->cb.dataReduction != nullptr && writable(i);
 }
+bool Task::deferred(uint i) const {
+    return writable(i) && !write_only(i) && !is_reduced(i) && !mem[i].disjointReadWritesAcrossSubtasks;
+}
+void Task::defer_write(ulong subtask, uint m, size_t offset, const void* src, size_t len) {
+    DeferredWrite w;
+    w.subtask = subtask;
+    w.mem = m;
+    w.offset = offset;
+    w.bytes.assign((const uint8_t*)src, (const uint8_t*)src + len);
+    std::lock_guard<std::mutex> g(deferred_mtx);
+    deferred_writes.push_back(std::move(w));
+}
 pmSubscriptionVisibilityType Task::visibility(uint i, bool gpu) const {
     pmSubscriptionVisibilityType v = mem[i].subscriptionVisibilityType;
     if (v == SUBSCRIPTION_OPTIMAL) {
@@ -576,6 +588,17 @@ void Runtime::complete(Task* t) {
         std::lock_guard<std::mutex> g(ma_list_mtx);
         ma_tasks.erase(std::remove(ma_tasks.begin(), ma_tasks.end(), t), ma_tasks.end());
     }
+    if (t->has_deferred) {
+        // held-back writes land in subtask order (stable within a subtask)
+        std::lock_guard<std::mutex> g(t->deferred_mtx);
+        auto& w = t->deferred_writes;
+        std::stable_sort(w.begin(), w.end(), [](const Task::DeferredWrite& a, const Task::DeferredWrite& b) {
+            return a.subtask < b.subtask;
+        });
+        if (t->status.load() == pmSuccess)
+            for (auto& x : w) memcpy(t->spaces[x.mem]->host + x.offset, x.bytes.data(), x.bytes.size());
+        std::vector<Task::DeferredWrite>().swap(w);
+    }
     try {
         commit_reduction(t);
     } catch (Error& e) {
@@ -685,20 +708,22 @@ void Runtime::cpu_loop(int idx) {
                     }
                     if (commit) {
                         for (uint i = 0; i < t->spaces.size(); ++i) {
-                            if (!t->writable(i)) continue;
+                            if (!t->writable(i) || t->is_reduced(i)) continue;
+                            const bool hold = t->deferred(i);
+                            auto put = [&](size_t off, const uint8_t* src, size_t len) {
+                                if (hold) t->defer_write(s, i, off, src, len);
+                                else memcpy(t->spaces[i]->host + off, src, len);
+                            };
                             if (si.memInfo[i].visibilityType == SUBSCRIPTION_COMPACT) {
                                 // COMPACT views: scatter written records back
                                 CompactLayout cl = make_compact(st->mem[i]);
-                                for (auto& kv : st->mem[i].write().map()) {
-                                    size_t co = cl.compact_offset(kv.first);
-                                    memcpy(t->spaces[i]->host + kv.first, compact[i].data() + co, kv.second);
-                                }
-                            } else if (ma && st->shadow[i]) {
+                                for (auto& kv : st->mem[i].write().map())
+                                    put(kv.first, compact[i].data() + cl.compact_offset(kv.first), kv.second);
+                            } else if (st->shadow[i]) {
                                 // private NATURAL view: copy its written records
                                 const size_t base = st->mem[i].unified().offset;
                                 for (auto& kv : st->mem[i].write().map())
-                                    memcpy(t->spaces[i]->host + kv.first, (uint8_t*)st->shadow[i] + (kv.first - base),
-                                           kv.second);
+                                    put(kv.first, (uint8_t*)st->shadow[i] + (kv.first - base), kv.second);
                             }
                         }
                     }
@@ -712,10 +737,10 @@ void Runtime::cpu_loop(int idx) {
                 if (commit) fail_task(t, e);
             }
             if (ma && !claimed) commit = t->claim_commit(s, false);
+            if (st && !t->callbacks->cb.dataReduction)
+                for (uint i = 0; i < MAX_MEM_SECTIONS_PER_TASK; ++i)
+                    if (st->shadow[i] && !st->shadow_on_device[i]) { free(st->shadow[i]); st->shadow[i] = nullptr; }
             if (ma) {
-                if (st)
-                    for (uint i = 0; i < MAX_MEM_SECTIONS_PER_TASK; ++i)
-                        if (st->shadow[i] && !st->shadow_on_device[i]) { free(st->shadow[i]); st->shadow[i] = nullptr; }
                 std::lock_guard<std::mutex> g(t->ma_mtx);
                 t->cpu_runs.erase(s);
                 if (commit) {
@@ -1169,6 +1194,8 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
                         s.st->shadow_on_device[sc.mem] = false;
                     }
                     memcpy((uint8_t*)s.st->shadow[sc.mem] + (sc.src_off - u.offset), s.pinned + sc.pinned_off, sc.len);
+                } else if (t->deferred(sc.mem)) {
+                    t->defer_write(sid, sc.mem, sc.src_off, s.pinned + sc.pinned_off, sc.len);
                 } else {
                     memcpy(t->spaces[sc.mem]->host + sc.src_off, s.pinned + sc.pinned_off, sc.len);
                 }

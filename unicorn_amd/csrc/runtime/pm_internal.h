@@ -235,6 +235,21 @@ struct Task {
     std::unique_ptr<Subtask> pending_reduce;
     std::mutex reduce_mtx;
     std::vector<std::unique_ptr<Subtask>> reduce_queue;
+    // Writes to a READ_WRITE space whose subtasks may read what other subtasks write
+    // (pmTaskMem::disjointReadWritesAcrossSubtasks false, the default) are held back until the
+    // task ends, so that every subtask reads the task's input (reference
+    // pmTask::DoSubtasksNeedShadowMemory, pmTask.cpp:749-754, committed as pmTask.cpp:1173-1191).
+    bool deferred(uint i) const;
+    bool has_deferred = false;
+    struct DeferredWrite {
+        ulong subtask;
+        uint mem;
+        size_t offset;
+        std::vector<uint8_t> bytes;
+    };
+    std::mutex deferred_mtx;
+    std::vector<DeferredWrite> deferred_writes;
+    void defer_write(ulong subtask, uint mem, size_t offset, const void* src, size_t len);
     pmTaskInfo info();
     bool is_reduced(uint i) const;
     bool writable(uint i) const;

@@ -196,7 +196,7 @@ pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<ui
                 pmSubscriptionInfo u = ms.unified();
                 if (u.length) {
                     uint8_t* base;
-                    if (t->is_reduced(i) || private_writes) {
+                    if (t->is_reduced(i) || private_writes || t->deferred(i)) {
                         if (!st.shadow[i]) {
                             st.shadow[i] = malloc(u.length);
                             if (!st.shadow[i]) raise(pmMemoryError, "shadow allocation");
