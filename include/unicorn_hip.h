@@ -88,8 +88,14 @@ int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, i
 int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const uint64_t* dst_off,
                     const uint64_t* len, uint32_t n_records, uint64_t max_len, void* stream);
 
-/* ---- built-in reduction: dst[i] += src[i] (REDUCE_ADD over floats; reference
- *      pmReducer.cpp:593-605) ---- */
+/* ---- built-in reductions (reference pmReducer::ReduceMemories, pmReducer.cpp:593-703) ----
+ * dst[i] = dst[i] (op) src[i] for i < count elements; op is a pmReductionOpType value
+ * (REDUCE_ADD .. REDUCE_BITWISE_XOR), type one of the UH_REDUCE_* element types below (in the
+ * order of pmReductionDataType). Both buffers are device memory, 16-byte aligned. */
+enum { UH_REDUCE_INT32 = 0, UH_REDUCE_UINT32 = 1, UH_REDUCE_INT64 = 2, UH_REDUCE_UINT64 = 3,
+       UH_REDUCE_FLOAT32 = 4, UH_REDUCE_FLOAT64 = 5 };
+int uh_reduce(void* dst, const void* src, size_t count, int op, int type, void* stream);
+/* REDUCE_ADD over floats (uh_reduce with op 0, UH_REDUCE_FLOAT32) */
 int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream);
 
 /* ---- imageFiltering (reference imageFilteringCUDA.cu:86-342) ----

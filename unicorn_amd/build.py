@@ -83,7 +83,7 @@ def build_all(verbose: bool = True) -> None:
     rt_srcs = sorted((CSRC / "runtime").glob("*.cpp"))
     app_srcs = sorted((CSRC / "apps").glob("*.cpp"))
     hip_deps = _headers(INC, CSRC / "hip")
-    cxx_deps = _headers(INC, CSRC / "runtime", CSRC / "apps")
+    cxx_deps = _headers(INC, CSRC / "runtime", CSRC / "apps", CSRC / "hip")
     jobs = max(1, min(16, os.cpu_count() or 4))
     with cf.ThreadPoolExecutor(jobs) as ex:
         hip_f = [ex.submit(_compile, s, HIPCC, HIP_FLAGS, hip_deps) for s in hip_srcs]

@@ -96,20 +96,6 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* dst, uint32_t v, siz
     for (; i < n; i += stride) dst[i] = v;
 }
 
-__global__ void __launch_bounds__(256) k_reduce_add_f32(float* dst, const float* src, size_t n) {
-    size_t n4 = n >> 2;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    float4* d4 = (float4*)dst;
-    const float4* s4 = (const float4*)src;
-    for (size_t k = i; k < n4; k += stride) {
-        float4 a = d4[k], b = s4[k];
-        a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
-        d4[k] = a;
-    }
-    for (size_t k = (n4 << 2) + i; k < n; k += stride) dst[k] += src[k];
-}
-
 }  // namespace uh
 
 using namespace uh;
@@ -311,22 +297,6 @@ int uh_copy_records(void* dst, const void* src, const uint64_t* src_off, const u
     UH_LAUNCH(k_copy_records, dim3(n_records, (unsigned)chunks), dim3(256), 0,
                        s, (uint8_t*)dst, (const uint8_t*)src, src_off, dst_off, len);
     UH_LAUNCH_CHECK("k_copy_records");
-    g_plain_launches++;
-    return UH_OK;
-}
-
-int uh_reduce_add_f32(float* dst, const float* src, size_t n, void* stream) {
-    if (!n) return UH_OK;
-    int rc;
-    hipStream_t s = plain_stream_of(stream, &rc);
-    if (rc) return rc;
-    if (((uintptr_t)dst | (uintptr_t)src) & 15u) return fail(UH_ERR_ARG, "uh_reduce_add_f32: unaligned");
-    size_t blocks = (n / 4 + 255) / 256;
-    size_t cap = (size_t)cu_count() * 8;
-    if (blocks > cap) blocks = cap;
-    if (blocks == 0) blocks = 1;
-    UH_LAUNCH(k_reduce_add_f32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
-    UH_LAUNCH_CHECK("k_reduce_add_f32");
     g_plain_launches++;
     return UH_OK;
 }

@@ -279,9 +279,30 @@ pmStatus pmSubscribeToMemory(pmTaskHandle th, pmDeviceHandle, ulong sid, pmSplit
     PM_API_END
 }
 
-pmStatus pmRedistributeData(pmTaskHandle, pmDeviceHandle, ulong, pmSplitInfo&, uint, size_t, size_t, uint) {
-    rt::set_last_error("pmRedistributeData: redistribution is not part of this build");
-    return pmFatalError;
+// Registers a range of the subtask's output for redistribution (reference
+// pmSubscriptionManager::RegisterRedistribution, pmSubscriptionManager.cpp:861-877): pOffset is
+// relative to the subtask's consolidated write subscription of space pMemIndex. Only from the
+// task's dataRedistribution callback.
+pmStatus pmRedistributeData(pmTaskHandle th, pmDeviceHandle, ulong sid, pmSplitInfo&, uint mi, size_t off, size_t len,
+                            uint order) {
+    PM_API_BEGIN
+    DistributionContext* c = current_context();
+    if (!c || !c->redistribution || (Task*)th != c->task || c->subtask->id != sid)
+        raise(pmInvalidCommand, "pmRedistributeData outside the subtask's dataRedistribution callback");
+    Task* t = c->task;
+    if (mi >= t->spaces.size()) raise(pmInvalidIndex, "memory index out of range");
+    if (!len) return pmSuccess;
+    const MemSubscription& ms = c->subtask->mem[mi];
+    if (!ms.has_write) raise(pmInvalidCommand, "pmRedistributeData: the subtask has no write subscription on this space");
+    const size_t global = ms.cwrite.offset + off;
+    if (global >= t->spaces[mi]->length || len > t->spaces[mi]->length - global)
+        raise(pmInvalidOffset, "pmRedistributeData: range beyond the address space");
+    std::lock_guard<std::mutex> g(t->redist_mtx);
+    auto& v = t->redist[mi];
+    uint seq = 0;
+    for (auto it = v.rbegin(); it != v.rend() && it->subtask == sid; ++it) ++seq;
+    v.push_back({order, sid, seq, global, len});
+    PM_API_END
 }
 
 pmStatus pmSetGpuLaunchConf(pmTaskHandle th, pmDeviceHandle, ulong sid, pmSplitInfo&, pmGpuLaunchConf& lc) {
@@ -354,8 +375,10 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
     t->multi_assign = d.multiAssignEnabled && t->cpu_allowed && t->gpu_allowed && !t->split_device &&
                       !cs->cb.dataReduction;
     for (uint i = 0; i < d.taskMemCount; ++i) t->has_deferred |= t->deferred(i);
-    // held-back writes are kept on the host: such a task stages its GPU subtasks the PLAIN way
-    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN && !t->has_deferred;
+    // held-back writes and redistribution work on the host copy: such a task stages its GPU
+    // subtasks the PLAIN way
+    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN && !t->has_deferred &&
+                      !cs->cb.dataRedistribution;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
         AddressSpace* as = t->spaces[i];
@@ -435,9 +458,18 @@ pmStatus pmReleaseScratchBuffer(pmTaskHandle th, pmDeviceHandle, ulong sid, pmSp
 
 void* pmGetLastReductionScratchBuffer(pmTaskHandle) { return NULL; }
 
-pmRedistributionMetadata* pmGetRedistributionMetadata(pmTaskHandle, uint, ulong* count) {
+// (order, bytes) of every order of a redistributed space, ascending (reference
+// pmRedistributor::GetRedistributionMetadata, pmRedistributor.cpp:295-323); valid until the task
+// is released.
+pmRedistributionMetadata* pmGetRedistributionMetadata(pmTaskHandle th, uint mi, ulong* count) {
     if (count) *count = 0;
-    return NULL;
+    Task* t = (Task*)th;
+    if (!t || !t->finished.load(std::memory_order_acquire)) return NULL;
+    std::lock_guard<std::mutex> g(t->redist_mtx);
+    auto it = t->redist_meta.find(mi);
+    if (it == t->redist_meta.end() || it->second.empty()) return NULL;
+    if (count) *count = it->second.size();
+    return it->second.data();
 }
 
 // ---------------------------------------------------------------- utilities: mapped files

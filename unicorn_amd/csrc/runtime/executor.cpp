@@ -521,10 +521,10 @@ static void reduce_into(Task* t, std::unique_ptr<Subtask> st) {
         if (acc->shadow_on_device[i] != st->shadow_on_device[i])
             raise(pmFatalError, "mixed host/device reduction shadows");
         if (acc->shadow_on_device[i]) {
-            if (!builtin || op != REDUCE_ADD || dt != REDUCE_FLOATS)
-                raise(pmFatalError, "device reduction supports REDUCE_ADD over floats");
-            if (uh_reduce_add_f32((float*)acc->shadow[i], (const float*)st->shadow[i],
-                                  acc->shadow_len[i] / 4, Runtime::get()->copy_stream) ||
+            if (!builtin) raise(pmFatalError, "HBM shadows take built-in reductions only");
+            static const size_t kElem[] = {4, 4, 8, 8, 4, 8};    // pmReductionDataType sizes
+            if (uh_reduce(acc->shadow[i], st->shadow[i], acc->shadow_len[i] / kElem[dt], (int)op, (int)dt,
+                          Runtime::get()->copy_stream) ||
                 uh_stream_sync(Runtime::get()->copy_stream))
                 raise(pmGraphicsCardError, uh_last_error());
             Memory::get().device_free(st->shadow[i], st->shadow_len[i]);
@@ -601,6 +601,7 @@ void Runtime::complete(Task* t) {
     }
     try {
         commit_reduction(t);
+        if (t->status.load() == pmSuccess && t->callbacks->cb.dataRedistribution) commit_redistribution(t);
     } catch (Error& e) {
         int expected = pmSuccess;
         t->status.compare_exchange_strong(expected, (int)e.status);
@@ -727,6 +728,7 @@ void Runtime::cpu_loop(int idx) {
                             }
                         }
                     }
+                    if (commit && t->callbacks->cb.dataRedistribution) run_redistribution(t, *st, dev);
                     if (t->callbacks->cb.dataReduction) reduce_into(t, std::move(st));
                 }
             } catch (Error& e) {
@@ -784,6 +786,8 @@ struct GpuSlot {
     bool busy = false;
     bool ma_copy = false;   // a multi-assigned copy of a CPU stub's subtask
     bool copy_failed = false;
+    void* scratch_dev = nullptr;   // device copy of the subtask's scratch buffer
+    int scratch_type = -1;
 };
 
 // Read-only device buffers of the PLAIN mode, keyed by (address space, exact records)
@@ -881,15 +885,30 @@ void upload_small(void* dst, const void* src, size_t bytes, const char* what) {
     hipchk(uh_stream_sync(r->copy_stream), what);
 }
 
-// The subtask's scratch buffer, filled on the host by its data-distribution callback, becomes the
-// device buffer gpuContext.scratchBuffer (what pmGetScratchBuffer returns in device code).
+// The subtask's scratch buffer, allocated (and possibly filled) on the host by its
+// data-distribution callback, becomes the device buffer gpuContext.scratchBuffer (what
+// pmGetScratchBuffer returns in device code). A buffer that outlives the subtask
+// (SUBTASK_TO_POST_SUBTASK, PRE_SUBTASK_TO_POST_SUBTASK) is copied back after the callback's
+// work (download_scratch), for the dataRedistribution callback and other post-subtask readers.
 void upload_scratch(Subtask& st, GpuSlot& s, pmSubtaskInfo& si) {
-    auto it = st.scratch.find(PRE_SUBTASK_TO_SUBTASK);
-    if (it == st.scratch.end()) it = st.scratch.find(PRE_SUBTASK_TO_POST_SUBTASK);
-    if (it == st.scratch.end() || it->second.empty()) return;
-    void* d = slot_device_alloc(s, it->second.size());
-    upload_small(d, it->second.data(), it->second.size(), "scratch upload");
-    si.gpuContext.scratchBuffer = d;
+    s.scratch_dev = nullptr;
+    s.scratch_type = -1;
+    for (int type : {(int)PRE_SUBTASK_TO_SUBTASK, (int)PRE_SUBTASK_TO_POST_SUBTASK, (int)SUBTASK_TO_POST_SUBTASK}) {
+        auto it = st.scratch.find(type);
+        if (it == st.scratch.end() || it->second.empty()) continue;
+        void* d = slot_device_alloc(s, it->second.size());
+        upload_small(d, it->second.data(), it->second.size(), "scratch upload");
+        si.gpuContext.scratchBuffer = d;
+        s.scratch_dev = d;
+        s.scratch_type = type;
+        return;
+    }
+}
+
+void download_scratch(Subtask& st, GpuSlot& s, void* stream) {
+    if (!s.scratch_dev || s.scratch_type == (int)PRE_SUBTASK_TO_SUBTASK) return;
+    std::vector<uint8_t>& h = st.scratch[s.scratch_type];
+    hipchk(uh_memcpy_d2h(h.data(), s.scratch_dev, h.size(), stream), "scratch download");
 }
 
 // PLAIN mode: stage every address space of the subtask as the reference stub does.
@@ -1201,6 +1220,8 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
                 }
             }
         }
+        if (win && t->callbacks->cb.dataRedistribution && t->status.load() == pmSuccess)
+            run_redistribution(t, *s.st, r->gpu_device);
         if (t->callbacks->cb.dataReduction && t->status.load() == pmSuccess) {
             reduce_into(t, std::move(s.st));
         }
@@ -1465,6 +1486,7 @@ void Runtime::gpu_loop() {
                             upload_scratch(*sp->st, *sp, si);
                             pmStatus rs = run_gpu_callback(tt, *sp->st, si, batch, gpu_device, kBatchStatusSlot);
                             if (rs != pmSuccess) raise(rs, "GPU callback failed");
+                            download_scratch(*sp->st, *sp, batch);
                         } catch (Error& e) { fail_task(tt, e); }
                     }
                     run.slots.push_back(std::move(sp));
@@ -1546,6 +1568,7 @@ void Runtime::gpu_loop() {
                     upload_scratch(*s.st, s, info);
                     pmStatus rs = run_gpu_callback(t, *s.st, info, s.stream, gpu_device, si_idx);
                     if (rs != pmSuccess) raise(rs, "GPU callback failed");
+                    download_scratch(*s.st, s, s.stream);
                     if (plain) {
                         // D2H of written records into the pinned buffer (pmCudaInterface.cu:510-527)
                         uint64_t bytes = 0;

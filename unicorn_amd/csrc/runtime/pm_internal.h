@@ -250,6 +250,21 @@ struct Task {
     std::mutex deferred_mtx;
     std::vector<DeferredWrite> deferred_writes;
     void defer_write(ulong subtask, uint mem, size_t offset, const void* src, size_t len);
+    // Data redistribution (reference pmRedistributor.cpp, pmSubscriptionManager.cpp:861-877): the
+    // dataRedistribution callback of every completed subtask registers ranges of the subtask's
+    // output (offsets relative to its consolidated write subscription) with an order number; at
+    // the end of the task the address space is rewritten as the concatenation of all ranges in
+    // ascending order (within an order by subtask, then in registration order), and the bytes
+    // after them are zeroed.
+    struct RedistRecord {
+        uint order;
+        ulong subtask;
+        uint seq;
+        size_t offset, length;
+    };
+    std::mutex redist_mtx;
+    std::map<uint, std::vector<RedistRecord>> redist;                  // per task memory index
+    std::map<uint, std::vector<pmRedistributionMetadata>> redist_meta; // (order, bytes) after the task
     pmTaskInfo info();
     bool is_reduced(uint i) const;
     bool writable(uint i) const;
@@ -265,6 +280,7 @@ struct DistributionContext {
     Subtask* subtask = nullptr;
     pmDeviceHandle device = nullptr;
     bool distribution = true;    // false: a subtask callback (scratch buffers only, no subscribing)
+    bool redistribution = false; // a dataRedistribution callback (pmRedistributeData allowed)
 };
 DistributionContext*& current_context();
 
@@ -364,6 +380,10 @@ private:
 
 // Runs the dataDistribution callback of one subtask and fills its subscriptions.
 void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev);
+// Runs the dataRedistribution callback of a completed subtask (its output committed).
+void run_redistribution(Task* t, Subtask& st, pmDeviceInfo* dev);
+// Rewrites the task's redistributed address spaces (at task completion).
+void commit_redistribution(Task* t);
 // Builds the pmSubtaskInfo of a CPU (host, NATURAL/COMPACT) subtask; may allocate shadows.
 // private_writes: writable NATURAL views point into shadows (multi-assigned CPU subtasks).
 pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs,

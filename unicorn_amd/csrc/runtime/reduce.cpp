@@ -1,8 +1,10 @@
 // Built-in subtask reductions (reference pmPublicUtilities.h:45-75, pmReducer.cpp:593-710,
 // pmPublicDefinitions.cpp:750-846). The callback returned by pmGetSubtaskReductionCallbackImpl
 // is a marker: the runtime recognises it and folds the subtasks' private outputs itself, on the
-// host or, for REDUCE_ADD over floats on the HIP stub, in HBM.
+// host, or in HBM (uh_reduce) when the HIP stub keeps the shadows there. Both folds take their
+// element operations from reduce_ops.h.
 #include "pm_internal.h"
+#include "../hip/reduce_ops.h"
 
 #include <cstring>
 
@@ -29,33 +31,28 @@ pmDataReductionCallback pick_type(pmReductionDataType dt) {
     }
 }
 
-template <typename T, typename F>
-void fold(void* dst, const void* src, size_t bytes, F f) {
+template <int OP, typename T>
+void fold(void* dst, const void* src, size_t bytes) {
     T* d = (T*)dst;
     const T* s = (const T*)src;
-    size_t n = bytes / sizeof(T);
-    for (size_t i = 0; i < n; ++i) d[i] = f(d[i], s[i]);
+    const size_t n = bytes / sizeof(T);
+    for (size_t i = 0; i < n; ++i) d[i] = redop::apply<OP, T>(d[i], s[i]);
 }
 
-template <typename T, bool INTEGRAL>
+template <typename T>
 void reduce_typed(void* dst, const void* src, size_t bytes, pmReductionOpType op) {
-    switch (op) {
-        case REDUCE_ADD: fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a + b); }); break;
-        case REDUCE_MIN: fold<T>(dst, src, bytes, [](T a, T b) { return b < a ? b : a; }); break;
-        case REDUCE_MAX: fold<T>(dst, src, bytes, [](T a, T b) { return b > a ? b : a; }); break;
-        case REDUCE_PRODUCT: fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a * b); }); break;
-        case REDUCE_LOGICAL_AND: fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a && b); }); break;
-        case REDUCE_LOGICAL_OR: fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a || b); }); break;
-        case REDUCE_LOGICAL_XOR: fold<T>(dst, src, bytes, [](T a, T b) { return (T)((!a) != (!b)); }); break;
-        default:
-            if constexpr (INTEGRAL) {
-                if (op == REDUCE_BITWISE_AND) fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a & b); });
-                else if (op == REDUCE_BITWISE_OR) fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a | b); });
-                else if (op == REDUCE_BITWISE_XOR) fold<T>(dst, src, bytes, [](T a, T b) { return (T)(a ^ b); });
-                else rt::raise(pmInvalidCommand, "unknown reduction");
-            } else {
-                rt::raise(pmInvalidCommand, "bitwise reduction over floating point");
-            }
+    switch ((int)op) {
+        case redop::ADD: fold<redop::ADD, T>(dst, src, bytes); break;
+        case redop::MIN: fold<redop::MIN, T>(dst, src, bytes); break;
+        case redop::MAX: fold<redop::MAX, T>(dst, src, bytes); break;
+        case redop::PRODUCT: fold<redop::PRODUCT, T>(dst, src, bytes); break;
+        case redop::LOGICAL_AND: fold<redop::LOGICAL_AND, T>(dst, src, bytes); break;
+        case redop::BITWISE_AND: fold<redop::BITWISE_AND, T>(dst, src, bytes); break;
+        case redop::LOGICAL_OR: fold<redop::LOGICAL_OR, T>(dst, src, bytes); break;
+        case redop::BITWISE_OR: fold<redop::BITWISE_OR, T>(dst, src, bytes); break;
+        case redop::LOGICAL_XOR: fold<redop::LOGICAL_XOR, T>(dst, src, bytes); break;
+        case redop::BITWISE_XOR: fold<redop::BITWISE_XOR, T>(dst, src, bytes); break;
+        default: rt::raise(pmInvalidCommand, "unknown reduction");
     }
 }
 
@@ -100,12 +97,12 @@ bool is_builtin_reduction(pmDataReductionCallback cb, pmReductionOpType* op, pmR
 
 void reduce_host(void* dst, const void* src, size_t bytes, pmReductionOpType op, pmReductionDataType dt) {
     switch (dt) {
-        case REDUCE_INTS: reduce_typed<int, true>(dst, src, bytes, op); break;
-        case REDUCE_UNSIGNED_INTS: reduce_typed<unsigned int, true>(dst, src, bytes, op); break;
-        case REDUCE_LONGS: reduce_typed<long, true>(dst, src, bytes, op); break;
-        case REDUCE_UNSIGNED_LONGS: reduce_typed<unsigned long, true>(dst, src, bytes, op); break;
-        case REDUCE_FLOATS: reduce_typed<float, false>(dst, src, bytes, op); break;
-        case REDUCE_DOUBLES: reduce_typed<double, false>(dst, src, bytes, op); break;
+        case REDUCE_INTS: reduce_typed<int>(dst, src, bytes, op); break;
+        case REDUCE_UNSIGNED_INTS: reduce_typed<unsigned int>(dst, src, bytes, op); break;
+        case REDUCE_LONGS: reduce_typed<long>(dst, src, bytes, op); break;
+        case REDUCE_UNSIGNED_LONGS: reduce_typed<unsigned long>(dst, src, bytes, op); break;
+        case REDUCE_FLOATS: reduce_typed<float>(dst, src, bytes, op); break;
+        case REDUCE_DOUBLES: reduce_typed<double>(dst, src, bytes, op); break;
         default: raise(pmInvalidCommand, "unknown reduction data type");
     }
 }

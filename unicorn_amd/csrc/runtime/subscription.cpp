@@ -172,6 +172,64 @@ void run_distribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
     if (s != pmSuccess) raise(s, "dataDistribution callback failed");
 }
 
+void run_redistribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
+    pmDataRedistributionCallback cb = t->callbacks->cb.dataRedistribution;
+    if (!cb) return;
+    DistributionContext ctx;
+    ctx.task = t;
+    ctx.subtask = &st;
+    ctx.device = dev->deviceHandle;
+    ctx.distribution = false;
+    ctx.redistribution = true;
+    DistributionContext* saved = current_context();
+    current_context() = &ctx;
+    // the subtask's views of the committed host copy
+    pmSubtaskInfo si;
+    si.subtaskId = st.id;
+    si.memCount = (uint)t->spaces.size();
+    si.splitInfo = st.split;
+    for (uint i = 0; i < t->spaces.size(); ++i) {
+        pmSubscriptionInfo u = st.mem[i].unified();
+        if (!u.length) continue;
+        pmMemInfo& mi = si.memInfo[i];
+        mi.ptr = t->spaces[i]->host + u.offset;
+        mi.length = u.length;
+        mi.visibilityType = SUBSCRIPTION_NATURAL;
+        if (st.mem[i].has_read) mi.readPtr = t->spaces[i]->host + st.mem[i].cread.offset;
+        if (st.mem[i].has_write) mi.writePtr = t->spaces[i]->host + st.mem[i].cwrite.offset;
+    }
+    pmStatus s = cb(t->info(), *dev, si);
+    current_context() = saved;
+    if (s != pmSuccess) raise(s, "dataRedistribution callback failed");
+}
+
+void commit_redistribution(Task* t) {
+    std::lock_guard<std::mutex> g(t->redist_mtx);
+    for (auto& kv : t->redist) {
+        const uint i = kv.first;
+        auto& recs = kv.second;
+        std::stable_sort(recs.begin(), recs.end(), [](const Task::RedistRecord& a, const Task::RedistRecord& b) {
+            if (a.order != b.order) return a.order < b.order;
+            if (a.subtask != b.subtask) return a.subtask < b.subtask;
+            return a.seq < b.seq;
+        });
+        AddressSpace* as = t->spaces[i];
+        std::vector<uint8_t> out(as->length, 0);
+        size_t pos = 0;
+        std::vector<pmRedistributionMetadata>& meta = t->redist_meta[i];
+        meta.clear();
+        for (const auto& r : recs) {
+            if (pos + r.length > as->length) raise(pmInvalidOffset, "redistributed ranges exceed the address space");
+            memcpy(out.data() + pos, as->host + r.offset, r.length);
+            pos += r.length;
+            if (meta.empty() || meta.back().order != r.order) meta.push_back(pmRedistributionMetadata(r.order, 0));
+            meta.back().count += (uint)r.length;
+        }
+        memcpy(as->host, out.data(), as->length);
+        Memory::get().mark_host_written(as);
+    }
+}
+
 // Host view of a subtask for the CPU stubs. NATURAL views point into the address space (or a
 // shadow buffer for reduced spaces); COMPACT views are packed into compact_bufs.
 pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs,
