@@ -128,7 +128,7 @@ def build_test_support(verbose: bool = True) -> None:
                   [f"--offload-arch={ARCH}", "-fPIC"] + rpath + ["-lpmlib", "-lunicorn_hip"])
         else:
             obj = _compile(src, CXX, CXX_FLAGS, deps)
-            _link(TEST_NATIVE / f"lib{src.stem}.so", CXX, [obj], rpath + ["-lpmlib", "-pthread"])
+            _link(TEST_NATIVE / f"lib{src.stem}.so", CXX, [obj], rpath + ["-lpmlib", "-lunicorn_hip", "-pthread"])
     if verbose:
         print(f"built test support in {TEST_NATIVE}", flush=True)
 

@@ -567,14 +567,15 @@ static void commit_reduction(Task* t) {
         pmSubscriptionInfo u = acc->mem[i].unified();
         if (acc->shadow_on_device[i]) {
             Memory::get().ensure_device(as, Runtime::get()->copy_stream);
-            if (uh_memcpy_d2d(as->dev + u.offset, acc->shadow[i], acc->shadow_len[i], Runtime::get()->copy_stream) ||
+            if (uh_memcpy_d2d(as->dev + u.offset, acc->shadow[i], std::min(acc->shadow_len[i], span_in_space(as, u)),
+                              Runtime::get()->copy_stream) ||
                 uh_stream_sync(Runtime::get()->copy_stream))
                 raise(pmGraphicsCardError, uh_last_error());
             Memory::get().mark_device_written(as);
             Memory::get().device_free(acc->shadow[i], acc->shadow_len[i]);
         } else {
             Memory::get().sync_host(as);
-            memcpy(as->host + u.offset, acc->shadow[i], acc->shadow_len[i]);
+            memcpy(as->host + u.offset, acc->shadow[i], std::min(acc->shadow_len[i], span_in_space(as, u)));
             Memory::get().mark_host_written(as);
             free(acc->shadow[i]);
         }
@@ -1065,7 +1066,7 @@ pmSubtaskInfo stage_resident(Task* t, GpuSlot& s, void* stream) {
                 uint8_t* base = as->dev + u.offset;
                 if (t->is_reduced(i)) {
                     void* sh = Memory::get().device_alloc(u.length);
-                    hipchk(uh_memcpy_d2d(sh, as->dev + u.offset, u.length, stream), "shadow init");
+                    hipchk(uh_memcpy_d2d(sh, as->dev + u.offset, span_in_space(as, u), stream), "shadow init");
                     st.shadow[i] = sh;
                     st.shadow_len[i] = u.length;
                     st.shadow_on_device[i] = true;

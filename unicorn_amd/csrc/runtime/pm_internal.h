@@ -105,6 +105,14 @@ struct AddressSpace {
     std::condition_variable runners_cv;
 };
 
+// Bytes of the span u that lie inside the address space. The consolidated extent of a scattered
+// subscription is step * count bytes from its offset (the reference's definition), which can end
+// past the space when the last record is shorter than the step; copies of a whole span stop at the
+// space's end.
+inline size_t span_in_space(const AddressSpace* as, const pmSubscriptionInfo& u) {
+    return u.offset >= as->length ? 0 : (u.length < as->length - u.offset ? u.length : as->length - u.offset);
+}
+
 class Memory {
 public:
     static Memory& get();

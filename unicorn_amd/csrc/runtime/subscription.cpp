@@ -260,7 +260,9 @@ pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<ui
                             if (!st.shadow[i]) raise(pmMemoryError, "shadow allocation");
                             st.shadow_len[i] = u.length;
                             st.shadow_on_device[i] = false;
-                            memcpy(st.shadow[i], as->host + u.offset, u.length);
+                            const size_t n = span_in_space(as, u);
+                            memcpy(st.shadow[i], as->host + u.offset, n);
+                            if (n < u.length) memset((uint8_t*)st.shadow[i] + n, 0, u.length - n);
                         }
                         base = (uint8_t*)st.shadow[i];
                     } else {
