@@ -87,6 +87,15 @@ int ua_mm_fetch(void* h, uint64_t* checksum);
 int ua_mm_get_inputs(void* h, float* A, float* B);
 int ua_mm_destroy(void* h);
 
+/* ---- matrixTranspose (reference testSuite/matrixTranspose, square blocks of
+ *      min(rows, cols, 2048)): int32 rows x cols, out of place (COMPACT views) or in place (one
+ *      READ_WRITE space) ---- */
+void* ua_mt_create(uint64_t rows, uint64_t cols, int inplace);
+int ua_mt_set_input(void* h, const int32_t* data);   /* rows x cols, row-major */
+int ua_mt_run(void* h, int device, double* seconds);
+int ua_mt_get_output(void* h, int32_t* out);          /* cols x rows, row-major */
+int ua_mt_destroy(void* h);
+
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
  * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
  * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).

@@ -81,6 +81,12 @@ int uh_graph_destroy(void* graph_exec);
 int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, int bz,
                      size_t shared_mem, void* stream, void** args);
 
+/* ---- matrixTranspose (reference matrixTranspose_cuda, matrixTransposeCUDA.cu:46-69) ----
+ * out[c * ld_out + r] = in[r * ld_in + c] for a rows x cols block of 32-bit elements (leading
+ * dimensions in elements; in and out must not alias). */
+int uh_transpose32(const void* in, uint64_t ld_in, void* out, uint64_t ld_out, uint32_t rows, uint32_t cols,
+                   void* stream);
+
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
  * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to

@@ -23,6 +23,13 @@
  *   or_lu_f64             the same factorisation in double
  *   or_lu_solve_f64       forward/backward substitution with those factors (FP64)
  *   or_matvec_f64         r = b - A x in double
+ *   or_transpose_i32      testSuite/matrixTranspose/source/code/src/matrixTranspose.cpp:187-261
+ *                         (serialMatrixTranspose): the element at output index i of the
+ *                         cols x rows result is input index (i * cols) mod (rows * cols - 1),
+ *                         the map its cycle-following loop walks; the last element stays
+ *   or_transpose_blocks   matrixTranspose.cpp:294-378 (data distribution + matrixTranspose_cpu,
+ *                         square blocks): every bs x bs block of the input moves to its
+ *                         transposed position, one subtask at a time
  *
  * Parity pins (tests/test_oracle.py): golden vectors produced by the reference's own serial
  * functions compiled from its sources (oracle/Makefile, oracle/_ref), closed forms and brute force.
@@ -304,5 +311,26 @@ void or_matvec_f64(const float* A, int n, int ld, const double* x, const double*
             const float* col = A + (size_t)j * ld;
             for (int i = 0; i < n; ++i) rv[i] -= (double)col[i] * xj;
         }
+    }
+}
+
+/* ------------------------------------------------------------------ matrix transpose */
+
+void or_transpose_i32(const int32_t* in, int32_t* out, size_t rows, size_t cols) {
+    const size_t n = rows * cols;
+    if (!n) return;
+    if (n == 1) { out[0] = in[0]; return; }
+    for (size_t i = 0; i + 1 < n; ++i) out[i] = in[(size_t)(((unsigned __int128)i * cols) % (n - 1))];
+    out[n - 1] = in[n - 1];
+}
+
+void or_transpose_blocks(const int32_t* in, int32_t* out, size_t rows, size_t cols, size_t bs) {
+    const size_t bcc = cols / bs;
+    const size_t subtasks = (rows / bs) * bcc;
+    for (size_t s = 0; s < subtasks; ++s) {
+        const size_t bir = s / bcc, bic = s % bcc;
+        for (size_t i = 0; i < bs; ++i)          /* output row bic * bs + i, input column */
+            for (size_t j = 0; j < bs; ++j)      /* output column bir * bs + j, input row */
+                out[(bic * bs + i) * rows + bir * bs + j] = in[(bir * bs + j) * cols + bic * bs + i];
     }
 }

@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
         _lib.or_lu_f64.argtypes = [vp, i32, i32]
         _lib.or_lu_solve_f64.argtypes = [vp, i32, i32, vp, i32]
         _lib.or_matvec_f64.argtypes = [vp, i32, i32, vp, vp, i32, vp]
+        _lib.or_transpose_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
+        _lib.or_transpose_blocks.argtypes = [vp, vp, C.c_size_t, C.c_size_t, C.c_size_t]
     return _lib
 
 
@@ -147,6 +149,24 @@ def matvec_residual_f64(A_colmajor, x, b):
 
 # ---------------------------------------------------------------- reference runner
 
+def transpose(M: np.ndarray) -> np.ndarray:
+    """serialMatrixTranspose's index map over a rows x cols int32 matrix."""
+    M = np.ascontiguousarray(M, dtype=np.int32)
+    rows, cols = M.shape
+    out = np.empty((cols, rows), dtype=np.int32)
+    lib().or_transpose_i32(_p(M), _p(out), rows, cols)
+    return out
+
+
+def transpose_blocks(M: np.ndarray, bs: int) -> np.ndarray:
+    """The app's block decomposition (one bs x bs block per subtask)."""
+    M = np.ascontiguousarray(M, dtype=np.int32)
+    rows, cols = M.shape
+    out = np.empty((cols, rows), dtype=np.int32)
+    lib().or_transpose_blocks(_p(M), _p(out), rows, cols, bs)
+    return out
+
+
 def ref_available() -> bool:
     return REF_RUNNER.exists()
 
@@ -206,3 +226,23 @@ def ref_matmul_int(A: np.ndarray, B: np.ndarray) -> np.ndarray:
         np.ascontiguousarray(B, dtype=np.int32).tofile(pb)
         _run_ref(["mm", n, pa, pb, pc])
         return np.fromfile(pc, dtype=np.int32).reshape(n, n)
+
+
+def ref_transpose(M: np.ndarray, inplace: bool) -> np.ndarray:
+    """serialMatrixTranspose of the reference (out of place, or in place on a copy)."""
+    rows, cols = M.shape
+    with tempfile.TemporaryDirectory() as d:
+        pa, po = Path(d, "a"), Path(d, "o")
+        np.ascontiguousarray(M, dtype=np.int32).tofile(pa)
+        _run_ref(["mt", int(inplace), rows, cols, pa, po])
+        return np.fromfile(po, dtype=np.int32).reshape(cols, rows)
+
+
+def ref_transpose_block(B: np.ndarray, inplace: bool) -> np.ndarray:
+    """transposeMatrixBlock of the reference over one square block."""
+    bs = B.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        pa, po = Path(d, "a"), Path(d, "o")
+        np.ascontiguousarray(B, dtype=np.int32).tofile(pa)
+        _run_ref(["mtblock", int(inplace), bs, pa, po])
+        return np.fromfile(po, dtype=np.int32).reshape(bs, bs)

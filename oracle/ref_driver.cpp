@@ -8,6 +8,8 @@
 //   imageFiltering::imageFilter_cpu     imageFiltering.cpp:379-472 (one call per tile, NATURAL views)
 //   pageRank::serialPageRank            testSuite/pageRank/source/code/src/pageRank.cpp:237-276 (2 iterations)
 //   matrixMultiply::serialMatrixMultiply testSuite/matrixMultiply/source/code/src/matrixMultiply.cpp:47-55 (int)
+//   matrixTranspose::serialMatrixTranspose testSuite/matrixTranspose/source/code/src/matrixTranspose.cpp:187-261
+//   matrixTranspose::transposeMatrixBlock  matrixTranspose.cpp:90-110 (square blocks, both forms)
 // The pm:: structures passed to imageFilter_cpu are laid out from the reference's own header;
 // nothing of the reference runtime is called (those references are discarded by --gc-sections).
 #include <cstdio>
@@ -27,6 +29,11 @@ extern unsigned int gTotalWebPages;
 extern unsigned int gMaxOutlinksPerWebPage;
 extern float* gSerialOutput;
 void serialPageRank(unsigned int* pWebDump);
+}
+namespace matrixTranspose {
+void serialMatrixTranspose(bool pInplace, int* pInputMatrix, int* pOutputMatrix, size_t pInputDimRows, size_t pInputDimCols);
+void transposeMatrixBlock(int* pBlock, size_t pBlockDim);
+void transposeMatrixBlock(int* pInputBlock, int* pOutputBlock, size_t pBlockDim);
 }
 namespace matrixMultiply {
 void serialMatrixMultiply(int* pMatrixA, int* pMatrixB, int* pMatrixC, int pDim1, int pDim2, int pDim3);
@@ -141,6 +148,36 @@ int main(int argc, char** argv) {
         std::vector<int> c((size_t)n * n, 0);
         matrixMultiply::serialMatrixMultiply((int*)a.data(), (int*)b.data(), c.data(), n, n, n);
         spit(argv[5], c.data(), c.size() * 4);
+        return 0;
+    }
+    if (!strcmp(mode, "mt") && argc == 7) {
+        // mt inplace rows cols in.bin out.bin : serialMatrixTranspose (in place: on a copy of in)
+        const bool inplace = atoi(argv[2]) != 0;
+        size_t rows = strtoul(argv[3], 0, 10), cols = strtoul(argv[4], 0, 10);
+        std::vector<char> a = slurp(argv[5]);
+        std::vector<int> out(rows * cols, 0);
+        if (inplace) {
+            memcpy(out.data(), a.data(), out.size() * 4);
+            matrixTranspose::serialMatrixTranspose(true, nullptr, out.data(), rows, cols);
+        } else {
+            matrixTranspose::serialMatrixTranspose(false, (int*)a.data(), out.data(), rows, cols);
+        }
+        spit(argv[6], out.data(), out.size() * 4);
+        return 0;
+    }
+    if (!strcmp(mode, "mtblock") && argc == 6) {
+        // mtblock inplace bs in.bin out.bin : transposeMatrixBlock of one bs x bs block
+        const bool inplace = atoi(argv[2]) != 0;
+        size_t bs = strtoul(argv[3], 0, 10);
+        std::vector<char> a = slurp(argv[4]);
+        std::vector<int> out(bs * bs, 0);
+        if (inplace) {
+            memcpy(out.data(), a.data(), out.size() * 4);
+            matrixTranspose::transposeMatrixBlock(out.data(), bs);
+        } else {
+            matrixTranspose::transposeMatrixBlock((int*)a.data(), out.data(), bs);
+        }
+        spit(argv[5], out.data(), out.size() * 4);
         return 0;
     }
     fprintf(stderr, "bad arguments\n");

@@ -3,7 +3,8 @@
 The reference ships no fixtures (its tests are runtime differential checks with random
 time-seeded inputs), so these vectors are produced here: small seeded inputs are fed to
 oracle/_ref/ref_runner, which is built by oracle/Makefile from the reference's unmodified
-sources (serialImageFilter, serialPageRank, int serialMatrixMultiply). Run in a container that
+sources (serialImageFilter, serialPageRank, int serialMatrixMultiply, serialMatrixTranspose,
+transposeMatrixBlock). Run in a container that
 has the reference sources:  make -C oracle ref && python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -61,6 +62,16 @@ def main():
         b = rng.integers(-16, 17, (n, n)).astype(np.int32)
         c = O.ref_matmul_int(a, b)
         np.savez_compressed(HERE / f"{name}.npz", a=a, b=b, c=c)
+    # matrixTranspose: serialMatrixTranspose (both forms) and transposeMatrixBlock (own seed, so the
+    # vectors above stay as they were)
+    rt = np.random.default_rng(20250114)
+    for name, (rows, cols, inplace) in {"mt_a": (6, 10, False), "mt_b": (12, 4, True), "mt_c": (1, 7, False),
+                                         "mt_d": (16, 16, True)}.items():
+        m = rt.integers(-(1 << 31), 1 << 31, (rows, cols), dtype=np.int64).astype(np.int32)
+        np.savez_compressed(HERE / f"{name}.npz", a=m, inplace=np.int32(inplace), out=O.ref_transpose(m, inplace))
+    blk = rt.integers(-1000, 1000, (9, 9)).astype(np.int32)
+    np.savez_compressed(HERE / "mt_block.npz", a=blk, out_copy=O.ref_transpose_block(blk, False),
+                        out_inplace=O.ref_transpose_block(blk, True))
     print("golden vectors written to", HERE)
 
 

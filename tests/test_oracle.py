@@ -273,3 +273,43 @@ def test_lu_solve_and_residual():
     assert np.abs(xs - x).max() <= 1e-12
     r = O.matvec_residual_f64(np.asfortranarray(A), xs, b)
     assert np.abs(r).max() <= 1e-11
+
+
+# ---------------------------------------------------------------- matrix transpose
+
+@pytest.mark.parametrize("name", ["mt_a", "mt_b", "mt_c", "mt_d"])
+def test_transpose_golden(name):
+    g = _load(name)
+    np.testing.assert_array_equal(O.transpose(g["a"]), g["out"])
+    # the fixture is the closed form too: serialMatrixTranspose's cycles move every element
+    np.testing.assert_array_equal(g["out"], g["a"].T)
+
+
+def test_transpose_block_golden():
+    g = _load("mt_block")
+    np.testing.assert_array_equal(g["out_copy"], g["a"].T)
+    np.testing.assert_array_equal(g["out_inplace"], g["a"].T)
+
+
+def test_transpose_index_map_catches_planted_mistake():
+    # the map (i * cols) mod (n - 1) with rows in place of cols is another permutation
+    m = np.arange(6 * 10, dtype=np.int32).reshape(6, 10)
+    n = m.size
+    wrong = np.array([m.reshape(-1)[(i * 6) % (n - 1)] for i in range(n - 1)] + [m.reshape(-1)[-1]], dtype=np.int32)
+    assert not np.array_equal(wrong.reshape(10, 6), O.transpose(m))
+
+
+@pytest.mark.parametrize("rows,cols,bs", [(8, 4, 4), (12, 12, 4), (6, 18, 6), (5, 5, 5)])
+def test_transpose_blocks_equal_whole_matrix(rows, cols, bs):
+    m = np.random.default_rng(rows * cols).integers(-999, 999, (rows, cols)).astype(np.int32)
+    np.testing.assert_array_equal(O.transpose_blocks(m, bs), O.transpose(m))
+
+
+def test_transpose_reference_fresh_inputs():
+    if not O.ref_available():
+        pytest.skip("oracle/_ref/ref_runner is built only where the reference sources are")
+    rng = np.random.default_rng(77)
+    for rows, cols in [(3, 11), (32, 8), (9, 9)]:
+        m = rng.integers(-(1 << 31), 1 << 31, (rows, cols), dtype=np.int64).astype(np.int32)
+        for inplace in (False, True):
+            np.testing.assert_array_equal(O.ref_transpose(m, inplace), O.transpose(m))

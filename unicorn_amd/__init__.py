@@ -94,6 +94,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_mm_create": (vp, [i32, i32, i32]), "ua_mm_set_inputs": (i32, [vp, vp, vp]),
         "ua_mm_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_mm_get_output": (i32, [vp, vp]),
         "ua_mm_destroy": (i32, [vp]),
+        "ua_mt_create": (vp, [C.c_uint64, C.c_uint64, i32]), "ua_mt_set_input": (i32, [vp, vp]),
+        "ua_mt_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_mt_get_output": (i32, [vp, vp]),
+        "ua_mt_destroy": (i32, [vp]),
         "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
                             C.POINTER(C.c_uint64)]),
         "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
@@ -573,6 +576,44 @@ class MatMul:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_mm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MatTranspose:
+    """matrixTranspose task: int32 rows x cols -> cols x rows, one subtask per square block of
+    min(rows, cols, 2048); in place uses one READ_WRITE address space."""
+
+    def __init__(self, rows: int, cols: int, inplace: bool = False):
+        init()
+        self.rows, self.cols = rows, cols
+        self._h = apps_lib().ua_mt_create(rows, cols, int(inplace))
+        if not self._h:
+            _check(-1, "ua_mt_create")
+
+    def set_input(self, M: np.ndarray) -> None:
+        M = np.ascontiguousarray(M, dtype=np.int32)
+        assert M.shape == (self.rows, self.cols)
+        _check(apps_lib().ua_mt_set_input(self._h, _ptr(M)), "ua_mt_set_input")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_mt_run(self._h, device, C.byref(t)), "matrixTranspose task")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty((self.cols, self.rows), dtype=np.int32)
+        _check(apps_lib().ua_mt_get_output(self._h, _ptr(out)), "ua_mt_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_mt_destroy(self._h)
             self._h = None
 
     def __del__(self):
