@@ -96,6 +96,14 @@ int ua_mt_run(void* h, int device, double* seconds);
 int ua_mt_get_output(void* h, int32_t* out);          /* cols x rows, row-major */
 int ua_mt_destroy(void* h);
 
+/* ---- prefixSum (reference testSuite/prefixSum): inclusive int32 prefix sum (wrap-around) by a
+ *      chunk-scan task, a host scan of the chunk totals and an add task ---- */
+void* ua_ps_create(uint32_t n, uint32_t elems_per_subtask);   /* reference: 1,000,000 per subtask */
+int ua_ps_set_input(void* h, const int32_t* data);
+int ua_ps_run(void* h, int device, double* seconds);
+int ua_ps_get_output(void* h, int32_t* out);
+int ua_ps_destroy(void* h);
+
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
  * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
  * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).

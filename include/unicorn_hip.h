@@ -87,6 +87,12 @@ int uh_launch_kernel(const void* func, int gx, int gy, int gz, int bx, int by, i
 int uh_transpose32(const void* in, uint64_t ld_in, void* out, uint64_t ld_out, uint32_t rows, uint32_t cols,
                    void* stream);
 
+/* ---- prefixSum (reference prefixSum_cuda / elemAdd_cuda, prefixSumCUDA.cu:68-165) ----
+ * uh_scan_i32: out[i] = in[0] + ... + in[i] modulo 2^32, n <= 2^24 (in may equal out).
+ * uh_add_scalar_i32: data[i] += *value modulo 2^32, value a device pointer. */
+int uh_scan_i32(const int32_t* in, int32_t* out, uint64_t n, void* stream);
+int uh_add_scalar_i32(int32_t* data, uint64_t n, const int32_t* value, void* stream);
+
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
  * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to

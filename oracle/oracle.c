@@ -27,6 +27,9 @@
  *                         (serialMatrixTranspose): the element at output index i of the
  *                         cols x rows result is input index (i * cols) mod (rows * cols - 1),
  *                         the map its cycle-following loop walks; the last element stays
+ *   or_prefix_sum_i32     testSuite/prefixSum/source/code/src/prefixSum.cpp:47-53 (serialPrefixSum)
+ *                         over int (wrap-around), and the chunked form of 225-290 (prefixSum_cpu
+ *                         per chunk, PopulateAuxMem, elemAdd_cpu) when chunk > 0
  *   or_transpose_blocks   matrixTranspose.cpp:294-378 (data distribution + matrixTranspose_cpu,
  *                         square blocks): every bs x bs block of the input moves to its
  *                         transposed position, one subtask at a time
@@ -332,5 +335,26 @@ void or_transpose_blocks(const int32_t* in, int32_t* out, size_t rows, size_t co
         for (size_t i = 0; i < bs; ++i)          /* output row bic * bs + i, input column */
             for (size_t j = 0; j < bs; ++j)      /* output column bir * bs + j, input row */
                 out[(bic * bs + i) * rows + bir * bs + j] = in[(bir * bs + j) * cols + bic * bs + i];
+    }
+}
+
+/* ------------------------------------------------------------------ prefix sum */
+
+void or_prefix_sum_i32(const int32_t* in, int32_t* out, size_t n, size_t chunk) {
+    if (!n) return;
+    if (!chunk) chunk = n;
+    const size_t chunks = (n + chunk - 1) / chunk;
+    /* per-chunk inclusive scans */
+    for (size_t c = 0; c < chunks; ++c) {
+        const size_t a = c * chunk, b = a + chunk < n ? a + chunk : n;
+        uint32_t acc = 0;
+        for (size_t i = a; i < b; ++i) { acc += (uint32_t)in[i]; out[i] = (int32_t)acc; }
+    }
+    /* scanned chunk totals added to the chunks after the first */
+    for (size_t c = 1; c < chunks; ++c) {
+        /* the previous chunk already holds the global prefix: its last element is the carry */
+        const uint32_t carry = (uint32_t)out[c * chunk - 1];
+        const size_t a = c * chunk, b = a + chunk < n ? a + chunk : n;
+        for (size_t i = a; i < b; ++i) out[i] = (int32_t)((uint32_t)out[i] + carry);
     }
 }

@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
         _lib.or_matvec_f64.argtypes = [vp, i32, i32, vp, vp, i32, vp]
         _lib.or_transpose_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
         _lib.or_transpose_blocks.argtypes = [vp, vp, C.c_size_t, C.c_size_t, C.c_size_t]
+        _lib.or_prefix_sum_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
     return _lib
 
 
@@ -164,6 +165,14 @@ def transpose_blocks(M: np.ndarray, bs: int) -> np.ndarray:
     rows, cols = M.shape
     out = np.empty((cols, rows), dtype=np.int32)
     lib().or_transpose_blocks(_p(M), _p(out), rows, cols, bs)
+    return out
+
+
+def prefix_sum(x: np.ndarray, chunk: int = 0) -> np.ndarray:
+    """serialPrefixSum (chunk 0), or the app's chunk scan + aux + add decomposition."""
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    out = np.empty_like(x)
+    lib().or_prefix_sum_i32(_p(x), _p(out), x.size, chunk)
     return out
 
 

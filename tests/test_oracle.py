@@ -313,3 +313,25 @@ def test_transpose_reference_fresh_inputs():
         m = rng.integers(-(1 << 31), 1 << 31, (rows, cols), dtype=np.int64).astype(np.int32)
         for inplace in (False, True):
             np.testing.assert_array_equal(O.ref_transpose(m, inplace), O.transpose(m))
+
+
+# ---------------------------------------------------------------- prefix sum
+
+def test_prefix_sum_closed_form_and_chunks():
+    # serialPrefixSum over int wraps modulo 2^32: the closed form is a 64-bit cumsum cast back
+    rng = np.random.default_rng(31)
+    x = rng.integers(-(1 << 31), 1 << 31, 10007, dtype=np.int64).astype(np.int32)
+    want = np.cumsum(x.astype(np.int64)).astype(np.int32)
+    np.testing.assert_array_equal(O.prefix_sum(x), want)
+    for chunk in (1, 7, 1000, 10007, 20000):
+        np.testing.assert_array_equal(O.prefix_sum(x, chunk), want)
+
+
+def test_prefix_sum_planted_mistakes():
+    x = np.arange(1, 11, dtype=np.int32)
+    assert O.prefix_sum(x)[-1] == 55
+    exclusive = np.concatenate([[0], np.cumsum(x)[:-1]]).astype(np.int32)
+    assert not np.array_equal(O.prefix_sum(x), exclusive)
+    # a chunked result without the carried totals differs after the first chunk
+    no_carry = np.concatenate([np.cumsum(x[:4]), np.cumsum(x[4:8]), np.cumsum(x[8:])]).astype(np.int32)
+    assert not np.array_equal(O.prefix_sum(x, 4), no_carry)

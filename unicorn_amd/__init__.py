@@ -97,6 +97,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_mt_create": (vp, [C.c_uint64, C.c_uint64, i32]), "ua_mt_set_input": (i32, [vp, vp]),
         "ua_mt_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_mt_get_output": (i32, [vp, vp]),
         "ua_mt_destroy": (i32, [vp]),
+        "ua_ps_create": (vp, [C.c_uint32, C.c_uint32]), "ua_ps_set_input": (i32, [vp, vp]),
+        "ua_ps_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_ps_get_output": (i32, [vp, vp]),
+        "ua_ps_destroy": (i32, [vp]),
         "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
                             C.POINTER(C.c_uint64)]),
         "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
@@ -614,6 +617,43 @@ class MatTranspose:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_mt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PrefixSum:
+    """prefixSum task pair: inclusive int32 prefix sum (wrap-around) of n elements in chunks."""
+
+    def __init__(self, n: int, elems_per_subtask: int = 1_000_000):
+        init()
+        self.n = n
+        self._h = apps_lib().ua_ps_create(n, elems_per_subtask)
+        if not self._h:
+            _check(-1, "ua_ps_create")
+
+    def set_input(self, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        assert x.shape == (self.n,)
+        _check(apps_lib().ua_ps_set_input(self._h, _ptr(x)), "ua_ps_set_input")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_ps_run(self._h, device, C.byref(t)), "prefixSum tasks")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(apps_lib().ua_ps_get_output(self._h, _ptr(out)), "ua_ps_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_ps_destroy(self._h)
             self._h = None
 
     def __del__(self):
