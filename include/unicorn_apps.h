@@ -104,6 +104,14 @@ int ua_ps_run(void* h, int device, double* seconds);
 int ua_ps_get_output(void* h, int32_t* out);
 int ua_ps_destroy(void* h);
 
+/* ---- sparseSolver (reference testSuite/sparseSolver): C = A B, A sparse n x n in COO order
+ *      sorted by (row, col), B dense, one subtask per square block of min(n, 2048) ---- */
+void* ua_ss_create(uint64_t n, uint64_t nnz, const int32_t* rows, const int32_t* cols, const float* vals,
+                   const float* B);
+int ua_ss_run(void* h, int device, double* seconds);
+int ua_ss_get_output(void* h, float* C);
+int ua_ss_destroy(void* h);
+
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
  * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
  * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).

@@ -93,6 +93,13 @@ int uh_transpose32(const void* in, uint64_t ld_in, void* out, uint64_t ld_out, u
 int uh_scan_i32(const int32_t* in, int32_t* out, uint64_t n, void* stream);
 int uh_add_scalar_i32(int32_t* data, uint64_t n, const int32_t* value, void* stream);
 
+/* ---- sparseSolver (reference cusparseXcoo2csr + csrmm2, sparseSolverCUDA.cu:206-290) ----
+ * entries: nnz records {int32 row, int32 col, float value}, sorted by row. C[r][j] = sum over the
+ * entries k of row row0 + r (in order) of value[k] * B[col[k]][j], for r < rows, j < cols.
+ * Products and sums are rounded separately (no fused multiply-add). */
+int uh_spmm_coo_f32(const void* entries, uint32_t nnz, int32_t row0, uint32_t rows, const float* B, uint64_t ldb,
+                    float* C, uint64_t ldc, uint32_t cols, void* stream);
+
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
  * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to

@@ -30,6 +30,10 @@
  *   or_prefix_sum_i32     testSuite/prefixSum/source/code/src/prefixSum.cpp:47-53 (serialPrefixSum)
  *                         over int (wrap-around), and the chunked form of 225-290 (prefixSum_cpu
  *                         per chunk, PopulateAuxMem, elemAdd_cpu) when chunk > 0
+ *   or_spmm_coo           testSuite/sparseSolver/source/code/src/sparseSolver.cpp:78-85, 189-263
+ *                         (serialSparseMatrixMultiply: MKL coomm, C = A B, beta 0) restated as a
+ *                         loop over the COO entries in order, in float (round each product and
+ *                         sum), or in double when dbl != 0
  *   or_transpose_blocks   matrixTranspose.cpp:294-378 (data distribution + matrixTranspose_cpu,
  *                         square blocks): every bs x bs block of the input moves to its
  *                         transposed position, one subtask at a time
@@ -356,5 +360,24 @@ void or_prefix_sum_i32(const int32_t* in, int32_t* out, size_t n, size_t chunk) 
         const uint32_t carry = (uint32_t)out[c * chunk - 1];
         const size_t a = c * chunk, b = a + chunk < n ? a + chunk : n;
         for (size_t i = a; i < b; ++i) out[i] = (int32_t)((uint32_t)out[i] + carry);
+    }
+}
+
+/* ------------------------------------------------------------------ sparse x dense */
+
+void or_spmm_coo(const int32_t* rows, const int32_t* cols, const float* vals, size_t nnz, const float* B, size_t n,
+                 float* Cf, double* Cd, int dbl) {
+    if (dbl) memset(Cd, 0, n * n * sizeof(double));
+    else memset(Cf, 0, n * n * sizeof(float));
+    for (size_t k = 0; k < nnz; ++k) {
+        const size_t r = (size_t)rows[k], c = (size_t)cols[k];
+        for (size_t j = 0; j < n; ++j) {
+            if (dbl) {
+                Cd[r * n + j] += (double)vals[k] * (double)B[c * n + j];
+            } else {
+                volatile float p = vals[k] * B[c * n + j];   /* no fused multiply-add */
+                Cf[r * n + j] = Cf[r * n + j] + p;
+            }
+        }
     }
 }

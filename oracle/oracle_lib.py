@@ -47,6 +47,7 @@ def lib() -> C.CDLL:
         _lib.or_transpose_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
         _lib.or_transpose_blocks.argtypes = [vp, vp, C.c_size_t, C.c_size_t, C.c_size_t]
         _lib.or_prefix_sum_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
+        _lib.or_spmm_coo.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, i32]
     return _lib
 
 
@@ -174,6 +175,20 @@ def prefix_sum(x: np.ndarray, chunk: int = 0) -> np.ndarray:
     out = np.empty_like(x)
     lib().or_prefix_sum_i32(_p(x), _p(out), x.size, chunk)
     return out
+
+
+def spmm_coo(rows, cols, vals, B, f64=False):
+    """C = A B for COO A (entries in order) and dense B: float with each product and sum rounded,
+    or double."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    n = B.shape[0]
+    Cf = np.zeros((n, n), np.float32)
+    Cd = np.zeros((n, n), np.float64)
+    lib().or_spmm_coo(_p(rows), _p(cols), _p(vals), rows.size, _p(B), n, _p(Cf), _p(Cd), int(f64))
+    return Cd if f64 else Cf
 
 
 def ref_available() -> bool:

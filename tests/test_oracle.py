@@ -335,3 +335,36 @@ def test_prefix_sum_planted_mistakes():
     # a chunked result without the carried totals differs after the first chunk
     no_carry = np.concatenate([np.cumsum(x[:4]), np.cumsum(x[4:8]), np.cumsum(x[8:])]).astype(np.int32)
     assert not np.array_equal(O.prefix_sum(x, 4), no_carry)
+
+
+# ---------------------------------------------------------------- sparse x dense
+
+def _coo(n, density, rng):
+    k = max(1, int(n * n * density))
+    idx = np.unique(rng.choice(n * n, size=k, replace=False))
+    return (idx // n).astype(np.int32), (idx % n).astype(np.int32)
+
+
+def test_spmm_coo_equals_dense_product():
+    rng = np.random.default_rng(41)
+    n = 64
+    r, c = _coo(n, 0.05, rng)
+    v = rng.integers(-9, 10, r.size).astype(np.float32)
+    B = rng.integers(-9, 10, (n, n)).astype(np.float32)
+    A = np.zeros((n, n), np.float64)
+    A[r, c] = v
+    want = A @ B.astype(np.float64)
+    # small integers: float and double forms are both exact
+    np.testing.assert_array_equal(O.spmm_coo(r, c, v, B, f64=True), want)
+    np.testing.assert_array_equal(O.spmm_coo(r, c, v, B).astype(np.float64), want)
+
+
+def test_spmm_coo_planted_mistake():
+    rng = np.random.default_rng(42)
+    n = 32
+    r, c = _coo(n, 0.1, rng)
+    v = rng.standard_normal(r.size).astype(np.float32)
+    B = rng.standard_normal((n, n)).astype(np.float32)
+    good = O.spmm_coo(r, c, v, B, f64=True)
+    assert not np.allclose(O.spmm_coo(c, r, v, B, f64=True), good)          # A transposed
+    assert np.allclose(O.spmm_coo(r, c, v, B), good, rtol=1e-5, atol=1e-5)

@@ -100,6 +100,8 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_ps_create": (vp, [C.c_uint32, C.c_uint32]), "ua_ps_set_input": (i32, [vp, vp]),
         "ua_ps_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_ps_get_output": (i32, [vp, vp]),
         "ua_ps_destroy": (i32, [vp]),
+        "ua_ss_create": (vp, [C.c_uint64, C.c_uint64, vp, vp, vp, vp]), "ua_ss_run": (i32, [vp, i32, C.POINTER(dbl)]),
+        "ua_ss_get_output": (i32, [vp, vp]), "ua_ss_destroy": (i32, [vp]),
         "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
                             C.POINTER(C.c_uint64)]),
         "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
@@ -654,6 +656,42 @@ class PrefixSum:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_ps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SparseMatMul:
+    """sparseSolver task: C = A B with A sparse (COO sorted by row, then column), B dense, n x n."""
+
+    def __init__(self, n: int, rows: np.ndarray, cols: np.ndarray, vals: np.ndarray, B: np.ndarray):
+        init()
+        self.n = n
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        self._h = apps_lib().ua_ss_create(n, rows.size, _ptr(rows), _ptr(cols), _ptr(vals), _ptr(B))
+        if not self._h:
+            _check(-1, "ua_ss_create")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_ss_run(self._h, device, C.byref(t)), "sparseSolver task")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty((self.n, self.n), dtype=np.float32)
+        _check(apps_lib().ua_ss_get_output(self._h, _ptr(out)), "ua_ss_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_ss_destroy(self._h)
             self._h = None
 
     def __del__(self):
