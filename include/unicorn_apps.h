@@ -112,6 +112,16 @@ int ua_ss_run(void* h, int device, double* seconds);
 int ua_ss_get_output(void* h, float* C);
 int ua_ss_destroy(void* h);
 
+/* ---- fft (reference testSuite/fft, 2-D without transposes): forward unnormalised 2-D DFT of a
+ *      rows x cols complex float matrix (interleaved float pairs) by a row task and a column task
+ *      of rows_per_subtask rows / columns per subtask (0: the reference's 512); rows and cols are
+ *      powers of two in [2, 8192] ---- */
+void* ua_fft_create(uint32_t rows, uint32_t cols, uint32_t rows_per_subtask);
+int ua_fft_set_input(void* h, const float* data);
+int ua_fft_run(void* h, int device, double* seconds);
+int ua_fft_get_output(void* h, float* out);
+int ua_fft_destroy(void* h);
+
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
  * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
  * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).

@@ -100,6 +100,14 @@ int uh_add_scalar_i32(int32_t* data, uint64_t n, const int32_t* value, void* str
 int uh_spmm_coo_f32(const void* entries, uint32_t nnz, int32_t row0, uint32_t rows, const float* B, uint64_t ldb,
                     float* C, uint64_t ldc, uint32_t cols, void* stream);
 
+/* ---- fft (reference cufftPlan1d / cufftPlanMany + cufftExecC2C, fftCUDA.cu:112-200) ----
+ * Forward, unnormalised DFTs X[k] = sum_j x[j] exp(-2 pi i j k / n) of `count` sequences of n
+ * single-precision complex points (float pairs), n a power of two in [2, 8192]. Element j of
+ * sequence s is at complex index s * seq_stride + j * elem_stride of its buffer. in may equal out
+ * when both stride pairs are equal; otherwise the buffers must not overlap. */
+int uh_fft_c2c_f32(const void* in, uint64_t in_seq_stride, uint64_t in_elem_stride, void* out,
+                   uint64_t out_seq_stride, uint64_t out_elem_stride, uint32_t n, uint32_t count, void* stream);
+
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
  * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to

@@ -37,6 +37,12 @@
  *   or_transpose_blocks   matrixTranspose.cpp:294-378 (data distribution + matrixTranspose_cpu,
  *                         square blocks): every bs x bs block of the input moves to its
  *                         transposed position, one subtask at a time
+ *   or_fft2_f64           testSuite/fft/source/code/src/fft.cpp:115-155 (fftSerial, FFT_2D: FFTW
+ *                         forward row transforms, transpose, row transforms, transpose), i.e. the
+ *                         unnormalised 2-D DFT X[u][v] = sum x[r][c] exp(-2 pi i (r u/R + c v/C));
+ *                         FFTW 3.3.3 (vendored under testSuite/fft/thirdparty, not built: it needs
+ *                         its own configure step) is restated by its definition, in double, as
+ *                         radix-2 transforms of the rows, then of the columns
  *
  * Parity pins (tests/test_oracle.py): golden vectors produced by the reference's own serial
  * functions compiled from its sources (oracle/Makefile, oracle/_ref), closed forms and brute force.
@@ -380,4 +386,44 @@ void or_spmm_coo(const int32_t* rows, const int32_t* cols, const float* vals, si
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------ 2-D DFT */
+
+/* One forward DFT of n = 2^k points in double, in place over x (interleaved re, im) read and
+ * written with stride s (complex elements): bit reversal, then radix-2 butterflies with
+ * w = exp(-2 pi i j / len) from cos / sin at each length. */
+static void fft1_f64(double* x, size_t n, size_t s) {
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double* a = x + 2 * i * s;
+            double* b = x + 2 * j * s;
+            const double t0 = a[0], t1 = a[1];
+            a[0] = b[0]; a[1] = b[1];
+            b[0] = t0; b[1] = t1;
+        }
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t h = len / 2;
+        for (size_t j = 0; j < h; ++j) {
+            const double wr = cos(two_pi * (double)j / (double)len), wi = -sin(two_pi * (double)j / (double)len);
+            for (size_t b = 0; b < n; b += len) {
+                double* p = x + 2 * (b + j) * s;
+                double* q = x + 2 * (b + j + h) * s;
+                const double tr = wr * q[0] - wi * q[1], ti = wr * q[1] + wi * q[0];
+                q[0] = p[0] - tr; q[1] = p[1] - ti;
+                p[0] += tr; p[1] += ti;
+            }
+        }
+    }
+}
+
+void or_fft2_f64(const float* in, double* out, size_t rows, size_t cols) {
+    for (size_t i = 0; i < 2 * rows * cols; ++i) out[i] = (double)in[i];
+    for (size_t r = 0; r < rows; ++r) fft1_f64(out + 2 * r * cols, cols, 1);      /* row transforms */
+    for (size_t c = 0; c < cols; ++c) fft1_f64(out + 2 * c, rows, cols);          /* column transforms */
 }

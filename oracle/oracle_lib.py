@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         _lib.or_transpose_blocks.argtypes = [vp, vp, C.c_size_t, C.c_size_t, C.c_size_t]
         _lib.or_prefix_sum_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
         _lib.or_spmm_coo.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, i32]
+        _lib.or_fft2_f64.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
     return _lib
 
 
@@ -189,6 +190,14 @@ def spmm_coo(rows, cols, vals, B, f64=False):
     Cd = np.zeros((n, n), np.float64)
     lib().or_spmm_coo(_p(rows), _p(cols), _p(vals), rows.size, _p(B), n, _p(Cf), _p(Cd), int(f64))
     return Cd if f64 else Cf
+
+
+def fft2(x: np.ndarray) -> np.ndarray:
+    """fftSerial's forward 2-D DFT (rows, then columns) of a complex64 matrix, in double."""
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    out = np.empty(x.shape, np.complex128)
+    lib().or_fft2_f64(_p(x), _p(out), x.shape[0], x.shape[1])
+    return out
 
 
 def ref_available() -> bool:

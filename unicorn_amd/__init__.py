@@ -102,6 +102,9 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_ps_destroy": (i32, [vp]),
         "ua_ss_create": (vp, [C.c_uint64, C.c_uint64, vp, vp, vp, vp]), "ua_ss_run": (i32, [vp, i32, C.POINTER(dbl)]),
         "ua_ss_get_output": (i32, [vp, vp]), "ua_ss_destroy": (i32, [vp]),
+        "ua_fft_create": (vp, [C.c_uint32, C.c_uint32, C.c_uint32]), "ua_fft_set_input": (i32, [vp, vp]),
+        "ua_fft_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_fft_get_output": (i32, [vp, vp]),
+        "ua_fft_destroy": (i32, [vp]),
         "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
                             C.POINTER(C.c_uint64)]),
         "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
@@ -182,7 +185,7 @@ def set_hbm_budget(nbytes: int) -> None:
 
 
 def set_split(cpu: bool = True, gpu: bool = False, cpu_pieces: int = 0, gpu_pieces: int = 0) -> None:
-    """Subtask splitting of imageFiltering / matrixMultiply tasks submitted afterwards: the
+    """Subtask splitting of imageFiltering / matrixMultiply / fft tasks submitted afterwards: the
     canSplitCpuSubtasks / canSplitGpuSubtasks flags and the pieces per split subtask (0 = one per
     stub of the splitting device type). The defaults are the reference apps' settings."""
     init()
@@ -692,6 +695,45 @@ class SparseMatMul:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_ss_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Fft2D:
+    """fft tasks: the forward, unnormalised 2-D DFT of a rows x cols complex64 matrix (a row task,
+    then a column task of rows_per_subtask rows / columns per subtask; 0 = 512)."""
+
+    def __init__(self, rows: int, cols: int, rows_per_subtask: int = 0):
+        init()
+        self.shape = (rows, cols)
+        self._h = apps_lib().ua_fft_create(rows, cols, rows_per_subtask)
+        if not self._h:
+            _check(-1, "ua_fft_create")
+
+    def set_input(self, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        if x.shape != self.shape:
+            raise ValueError(f"input shape {x.shape} != {self.shape}")
+        _check(apps_lib().ua_fft_set_input(self._h, _ptr(x)), "ua_fft_set_input")
+
+    def run(self, device: int = DEVICE_GPU) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_fft_run(self._h, device, C.byref(t)), "fft tasks")
+        return t.value
+
+    def output(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.complex64)
+        _check(apps_lib().ua_fft_get_output(self._h, _ptr(out)), "ua_fft_get_output")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_fft_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -49,9 +49,9 @@ inline int register_variants(const char* key, pmCallbacks cb, CallbackVariants& 
     return 0;
 }
 
-// Subtask splitting requested by the apps that support it (imageFiltering, matrixMultiply), as
-// the reference apps set pmTaskDetails::canSplitCpuSubtasks (imageFiltering.cpp:264,
-// matrixMultiplyBlas.cpp:219). CPU splitting is on by default, GPU splitting off; multi_assign
+// Subtask splitting requested by the apps that support it (imageFiltering, matrixMultiply, fft),
+// as the reference apps set pmTaskDetails::canSplitCpuSubtasks (imageFiltering.cpp:264,
+// matrixMultiplyBlas.cpp:219, fft.cpp:435-438). CPU splitting is on by default, GPU splitting off; multi_assign
 // is pmTaskDetails::multiAssignEnabled of every app's tasks (default true, as the reference).
 struct SplitRequest {
     bool cpu = true;

@@ -32,6 +32,10 @@ __device__ __forceinline__ uint32_t lower_row(const CooEntry* e, uint32_t n, int
 
 __global__ void __launch_bounds__(256) k_spmm_coo(const CooEntry* ent, uint32_t nnz, int32_t row0, const float* B,
                                                    uint64_t ldb, float* C, uint64_t ldc, uint32_t cols) {
+    // HIP contracts a * b + c into one fused multiply-add by default (also through __fmul_rn /
+    // __fadd_rn, whose header bodies allow it); the CPU callback rounds the product and the sum
+    // separately, so contraction is off here and the operators are written out
+#pragma clang fp contract(off)
     __shared__ uint32_t range[2];
     const int32_t r = row0 + (int32_t)blockIdx.x;
     if (threadIdx.x == 0) {
@@ -51,7 +55,7 @@ __global__ void __launch_bounds__(256) k_spmm_coo(const CooEntry* ent, uint32_t 
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const uint32_t j = j0 + threadIdx.x + 256u * e;
-                if (j < cols) acc[e] = __fadd_rn(acc[e], __fmul_rn(a, brow[j]));
+                if (j < cols) acc[e] = acc[e] + a * brow[j];
             }
         }
 #pragma unroll
