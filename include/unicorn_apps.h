@@ -122,6 +122,19 @@ int ua_fft_run(void* h, int device, double* seconds);
 int ua_fft_get_output(void* h, float* out);
 int ua_fft_destroy(void* h);
 
+/* ---- pageRankInvertedIndex (reference testSuite/pageRankInvertedIndex): pageRank over a web
+ *      dump directory (the ua_webdump_* format) by an index-inversion task with data
+ *      redistribution, then `iterations` (0: 2) pageRank tasks over the inverted index;
+ *      pages_per_subtask 0 = the reference's 10,000,000, rounded to whole files ---- */
+void* ua_prii_create(const char* dir, uint32_t pages_per_subtask);
+int ua_prii_run(void* h, int device, uint32_t iterations, double* seconds);
+/* the redistributed index: n_entries entries {src, dest, src out-links} (uint32 x 3) and the
+ * inlink count of each of the subtasks (entries NULL: counts only) */
+int ua_prii_get_index(void* h, uint32_t* entries, uint64_t capacity, uint64_t* n_entries, uint32_t* counts);
+int ua_prii_get_ranks(void* h, float* ranks);
+int ua_prii_info(void* h, uint32_t* pages, uint32_t* pages_per_subtask, uint32_t* subtasks);
+int ua_prii_destroy(void* h);
+
 /* ---- web graph generator (the role of the reference's webGenerator.pl) ----
  * Out-degree 0 with probability `dangling`, else P(k) ~ (k + k0)^-2 on [1, max_out] with k0
  * chosen for an overall mean of `mean_out`; targets by a Polya urn (in-degree + 1).

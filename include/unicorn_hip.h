@@ -14,7 +14,8 @@
  * thread). Streams are opaque `void*` (a hipStream_t). A stream created by
  * uh_batch_create() is a *batch handle*: kernel launchers called with it record a work item
  * in a device work queue instead of launching, and uh_batch_flush() runs all recorded items of
- * a family in one persistent launch. Launchers never synchronise.
+ * a family in one persistent launch. Launchers never synchronise, except uh_invert_index, which
+ * waits for its stream to report malformed input.
  */
 #ifndef UNICORN_HIP_H
 #define UNICORN_HIP_H
@@ -107,6 +108,23 @@ int uh_spmm_coo_f32(const void* entries, uint32_t nnz, int32_t row0, uint32_t ro
  * when both stride pairs are equal; otherwise the buffers must not overlap. */
 int uh_fft_c2c_f32(const void* in, uint64_t in_seq_stride, uint64_t in_elem_stride, void* out,
                    uint64_t out_seq_stride, uint64_t out_elem_stride, uint32_t n, uint32_t count, void* stream);
+
+/* ---- pageRankInvertedIndex (the reference's GPU callbacks invertIndex_cudaLaunchFunc /
+ *      pageRank_cudaLaunchFunc are empty, pageRankInvertedIndexCUDA.cu:92-100; these do what its
+ *      CPU callbacks invertIndex_cpu / pageRank_cpu do, pageRankInvertedIndex.cpp:301-382, 438-472) ----
+ * uh_invert_index: `pages` web-dump records (uint32: out-link count c <= max_out, then c 1-based
+ * targets, record length max_out + 1) of the pages first_page.. become 12-byte entries
+ * {src page, dest page (0-based), src out-links} in out_entries, grouped by dest / pages_per_subtask
+ * in ascending group order, each group in reverse enumeration order (page, then link); totals[g]
+ * receives the entry count of group g (groups <= 8192). Device pointers; the call waits for its
+ * stream and fails if a record or link was malformed (those are skipped).
+ * uh_prii_rank: out[0 .. out_pages) = 0, then out[dest - first_page] += damping * (x / out-links)
+ * for each entry, x = in_ranks[src] (or `initial` when in_ranks is NULL); entries outside the
+ * range are skipped. Float atomics: summation order is not fixed. */
+int uh_invert_index(const uint32_t* dump, uint32_t pages, uint32_t max_out, uint32_t first_page, uint32_t total_pages,
+                    uint32_t pages_per_subtask, uint32_t groups, void* out_entries, uint32_t* totals, void* stream);
+int uh_prii_rank(const void* entries, uint32_t n, const float* in_ranks, float initial, float damping,
+                 uint32_t first_page, float* out, uint32_t out_pages, uint32_t total_pages, void* stream);
 
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----

@@ -43,6 +43,14 @@
  *                         FFTW 3.3.3 (vendored under testSuite/fft/thirdparty, not built: it needs
  *                         its own configure step) is restated by its definition, in double, as
  *                         radix-2 transforms of the rows, then of the columns
+ *   or_prii_invert        testSuite/pageRankInvertedIndex/source/code/src/pageRankInvertedIndex.cpp:
+ *                         301-399 (invertIndex_cpu: per-group counts, each group filled from its
+ *                         top; invertIndexDataRedistribution: group g -> order g) and the runtime's
+ *                         redistribution (pmRedistributor: orders ascending, subtasks ascending)
+ *   or_prii_pagerank      pageRankInvertedIndex.cpp:438-472 (pageRank_cpu) per subtask over the
+ *                         redistributed index, PAGE_RANK_ITERATIONS tasks (InvertIndexTask /
+ *                         ParallelPageRankIteration, 546-637)
+ *   or_prii_serial        pageRankInvertedIndex.cpp:208-247 (serialPageRank)
  *
  * Parity pins (tests/test_oracle.py): golden vectors produced by the reference's own serial
  * functions compiled from its sources (oracle/Makefile, oracle/_ref), closed forms and brute force.
@@ -426,4 +434,97 @@ void or_fft2_f64(const float* in, double* out, size_t rows, size_t cols) {
     for (size_t i = 0; i < 2 * rows * cols; ++i) out[i] = (double)in[i];
     for (size_t r = 0; r < rows; ++r) fft1_f64(out + 2 * r * cols, cols, 1);      /* row transforms */
     for (size_t c = 0; c < cols; ++c) fft1_f64(out + 2 * c, rows, cols);          /* column transforms */
+}
+
+/* ------------------------------------------------------------------ pageRank over an inverted index */
+
+/* records: pages x (max_out + 1) uint32 (count, 1-based targets, padding). Every subtask s of P
+ * pages fills its own window group by group from the top (invertIndex_cpu), then the groups are
+ * concatenated order by order, subtasks ascending within an order (the redistribution).
+ * entries: room for every link (3 uint32 each); counts[g]: entries of group g; returns the total. */
+uint64_t or_prii_invert(const uint32_t* records, uint32_t pages, uint32_t max_out, uint32_t P, uint32_t* entries,
+                        uint32_t* counts) {
+    const uint32_t D = (pages + P - 1) / P;
+    const uint64_t rec = (uint64_t)max_out + 1;
+    uint64_t links = 0;
+    for (uint64_t p = 0; p < pages; ++p) links += records[p * rec];
+    uint32_t* win = (uint32_t*)malloc((links ? links : 1) * 3 * sizeof(uint32_t));
+    uint64_t* per = (uint64_t*)calloc((size_t)D * D, sizeof(uint64_t));     /* per[s * D + g] */
+    uint64_t* wbase = (uint64_t*)calloc(D + 1, sizeof(uint64_t));           /* window start of s */
+    for (uint32_t s = 0; s < D; ++s) {
+        const uint64_t p0 = (uint64_t)s * P, p1 = p0 + P < pages ? p0 + P : pages;
+        uint64_t n = 0;
+        for (uint64_t p = p0; p < p1; ++p)
+            for (uint32_t k = 0; k < records[p * rec]; ++k) {
+                ++per[(uint64_t)s * D + (records[p * rec + 1 + k] - 1) / P];
+                ++n;
+            }
+        wbase[s + 1] = wbase[s] + n;
+        /* top of each group, then fill downwards */
+        uint64_t* loc = (uint64_t*)malloc((size_t)D * sizeof(uint64_t));
+        uint64_t end = 0;
+        for (uint32_t g = 0; g < D; ++g) { end += per[(uint64_t)s * D + g]; loc[g] = end; }
+        uint32_t* w = win + 3 * wbase[s];
+        for (uint64_t p = p0; p < p1; ++p)
+            for (uint32_t k = 0; k < records[p * rec]; ++k) {
+                const uint32_t t = records[p * rec + 1 + k];
+                const uint64_t i = --loc[(t - 1) / P];
+                w[3 * i] = (uint32_t)p;
+                w[3 * i + 1] = t - 1;
+                w[3 * i + 2] = records[p * rec];
+            }
+        free(loc);
+    }
+    uint64_t pos = 0;
+    for (uint32_t g = 0; g < D; ++g) {
+        uint64_t c = 0;
+        for (uint32_t s = 0; s < D; ++s) {
+            uint64_t off = 0;
+            for (uint32_t h = 0; h < g; ++h) off += per[(uint64_t)s * D + h];
+            const uint64_t n = per[(uint64_t)s * D + g];
+            memcpy(entries + 3 * pos, win + 3 * (wbase[s] + off), n * 3 * sizeof(uint32_t));
+            pos += n;
+            c += n;
+        }
+        counts[g] = (uint32_t)c;
+    }
+    free(win);
+    free(per);
+    free(wbase);
+    return pos;
+}
+
+/* `iterations` pageRank tasks over the inverted index (pageRank_cpu per subtask, entries in index
+ * order): out[dest] += 0.85f * (x[src] / outlinks), x = 1.0 on the first iteration. */
+void or_prii_pagerank(const uint32_t* entries, const uint32_t* counts, uint32_t pages, uint32_t P, int iterations,
+                      float* out) {
+    const uint32_t D = (pages + P - 1) / P;
+    float* x = (float*)malloc((size_t)pages * sizeof(float));
+    for (int it = 0; it < iterations; ++it) {
+        memset(out, 0, (size_t)pages * sizeof(float));
+        const uint32_t* e = entries;
+        for (uint32_t g = 0; g < D; ++g)
+            for (uint32_t i = 0; i < counts[g]; ++i, e += 3) {
+                const float r = it == 0 ? 1.0f : x[e[0]];
+                out[e[1]] += 0.85f * (r / (float)e[2]);
+            }
+        memcpy(x, out, (size_t)pages * sizeof(float));
+    }
+    free(x);
+}
+
+/* serialPageRank: pages in order push (0.85f * rank) / outlinks to every target. */
+void or_prii_serial(const uint32_t* records, uint32_t pages, uint32_t max_out, int iterations, float* out) {
+    const uint64_t rec = (uint64_t)max_out + 1;
+    float* x = (float*)malloc((size_t)pages * sizeof(float));
+    for (int it = 0; it < iterations; ++it) {
+        memset(out, 0, (size_t)pages * sizeof(float));
+        for (uint64_t p = 0; p < pages; ++p) {
+            const uint32_t c = records[p * rec];
+            const float incr = (float)(0.85f * (it == 0 ? 1.0f : x[p]) / (float)c);
+            for (uint32_t k = 0; k < c; ++k) out[records[p * rec + 1 + k] - 1] += incr;
+        }
+        memcpy(x, out, (size_t)pages * sizeof(float));
+    }
+    free(x);
 }

@@ -49,6 +49,10 @@ def lib() -> C.CDLL:
         _lib.or_prefix_sum_i32.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
         _lib.or_spmm_coo.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, vp, vp, i32]
         _lib.or_fft2_f64.argtypes = [vp, vp, C.c_size_t, C.c_size_t]
+        _lib.or_prii_invert.argtypes = [vp, u32, u32, u32, vp, vp]
+        _lib.or_prii_invert.restype = C.c_uint64
+        _lib.or_prii_pagerank.argtypes = [vp, vp, u32, u32, i32, vp]
+        _lib.or_prii_serial.argtypes = [vp, u32, u32, i32, vp]
     return _lib
 
 
@@ -197,6 +201,46 @@ def fft2(x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.complex64)
     out = np.empty(x.shape, np.complex128)
     lib().or_fft2_f64(_p(x), _p(out), x.shape[0], x.shape[1])
+    return out
+
+
+def prii_records(row_ptr: np.ndarray, col: np.ndarray, max_out: int) -> np.ndarray:
+    """The web dump's padded records (pages x (max_out + 1) uint32) of an out-link CSR."""
+    pages = row_ptr.size - 1
+    rec = np.zeros((pages, max_out + 1), np.uint32)
+    deg = np.diff(row_ptr.astype(np.int64))
+    rec[:, 0] = deg
+    for p in np.nonzero(deg)[0]:
+        rec[p, 1:1 + deg[p]] = col[row_ptr[p]:row_ptr[p + 1]] + 1
+    return rec
+
+
+def prii_invert(records: np.ndarray, P: int):
+    """invertIndex_cpu per subtask + the redistribution: (entries [n x 3], counts per group)."""
+    records = np.ascontiguousarray(records, dtype=np.uint32)
+    pages, rec = records.shape
+    links = int(records[:, 0].sum())
+    entries = np.zeros((max(links, 1), 3), np.uint32)
+    D = (pages + P - 1) // P
+    counts = np.zeros(D, np.uint32)
+    n = lib().or_prii_invert(_p(records), pages, rec - 1, P, _p(entries), _p(counts))
+    return entries[:n], counts
+
+
+def prii_pagerank(entries: np.ndarray, counts: np.ndarray, pages: int, P: int, iterations: int = 2) -> np.ndarray:
+    """pageRank_cpu over the redistributed index, subtask by subtask, entries in order."""
+    entries = np.ascontiguousarray(entries if entries.size else np.zeros((1, 3), np.uint32), dtype=np.uint32)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    out = np.zeros(pages, np.float32)
+    lib().or_prii_pagerank(_p(entries), _p(counts), pages, P, iterations, _p(out))
+    return out
+
+
+def prii_serial(records: np.ndarray, iterations: int = 2) -> np.ndarray:
+    """serialPageRank of the inverted-index app over the padded records."""
+    records = np.ascontiguousarray(records, dtype=np.uint32)
+    out = np.zeros(records.shape[0], np.float32)
+    lib().or_prii_serial(_p(records), records.shape[0], records.shape[1] - 1, iterations, _p(out))
     return out
 
 

@@ -105,6 +105,11 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_fft_create": (vp, [C.c_uint32, C.c_uint32, C.c_uint32]), "ua_fft_set_input": (i32, [vp, vp]),
         "ua_fft_run": (i32, [vp, i32, C.POINTER(dbl)]), "ua_fft_get_output": (i32, [vp, vp]),
         "ua_fft_destroy": (i32, [vp]),
+        "ua_prii_create": (vp, [C.c_char_p, C.c_uint32]), "ua_prii_run": (i32, [vp, i32, C.c_uint32, C.POINTER(dbl)]),
+        "ua_prii_get_index": (i32, [vp, vp, C.c_uint64, C.POINTER(C.c_uint64), vp]),
+        "ua_prii_get_ranks": (i32, [vp, vp]),
+        "ua_prii_info": (i32, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "ua_prii_destroy": (i32, [vp]),
         "ua_webgen": (i32, [C.c_uint32, C.c_uint64, dbl, C.c_uint32, dbl, vp, vp, C.c_uint64,
                             C.POINTER(C.c_uint64)]),
         "ua_webgen_shift": (dbl, [dbl, C.c_uint32, dbl]),
@@ -734,6 +739,50 @@ class Fft2D:
     def close(self) -> None:
         if self._h:
             apps_lib().ua_fft_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PageRankInvertedIndex:
+    """pageRankInvertedIndex tasks over a web dump directory: the index inversion (with data
+    redistribution), then pageRank iterations over the inverted index."""
+
+    def __init__(self, directory: str, pages_per_subtask: int = 0):
+        init()
+        self._h = apps_lib().ua_prii_create(str(directory).encode(), pages_per_subtask)
+        if not self._h:
+            _check(-1, "ua_prii_create")
+        p, per, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        apps_lib().ua_prii_info(self._h, C.byref(p), C.byref(per), C.byref(n))
+        self.pages, self.pages_per_subtask, self.subtasks = p.value, per.value, n.value
+
+    def run(self, device: int = DEVICE_GPU, iterations: int = 0) -> float:
+        t = C.c_double(0.0)
+        _check(apps_lib().ua_prii_run(self._h, device, iterations, C.byref(t)), "pageRankInvertedIndex tasks")
+        return t.value
+
+    def index(self):
+        """(entries [n x 3] uint32: src, dest, src out-links; inlink counts per subtask)."""
+        n = C.c_uint64()
+        counts = np.zeros(self.subtasks, np.uint32)
+        _check(apps_lib().ua_prii_get_index(self._h, None, 0, C.byref(n), _ptr(counts)), "ua_prii_get_index")
+        e = np.zeros((n.value, 3), np.uint32)
+        _check(apps_lib().ua_prii_get_index(self._h, _ptr(e), n.value, C.byref(n), None), "ua_prii_get_index")
+        return e, counts
+
+    def ranks(self) -> np.ndarray:
+        out = np.empty(self.pages, np.float32)
+        _check(apps_lib().ua_prii_get_ranks(self._h, _ptr(out)), "ua_prii_get_ranks")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            apps_lib().ua_prii_destroy(self._h)
             self._h = None
 
     def __del__(self):
