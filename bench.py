@@ -27,7 +27,11 @@ CPU baseline: the same task on the pm:: runtime's CPU stubs (one thread per core
 share), which run the apps' CPU callbacks (restatements of the reference's CPU callbacks) over
 the same subtask decomposition; luDecomposition is timed at N=4096.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|imagefilter|matmul|pagerank|lu]
+`--workload apps` measures the other apps instead (matrixTranspose, prefixSum, sparseSolver, fft,
+pageRankInvertedIndex at the reference's default sizes), one line with a row per app.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--workload all|imagefilter|matmul|pagerank|lu|apps]
                        [--no-cpu-baseline] [--serving-requests N] [--serving-slots S]
 --steps / --warmup apply to the headline (or the one workload named); defaults per workload:
 imagefilter and matmul 200 / 3, pagerank 10 / 1, lu 3 / 1.
@@ -53,7 +57,7 @@ SEED = 20240601
 # flush of the timed region: each event pair costs the device a few microseconds, which timing
 # every flush would add to the tasks being measured.
 KT_EVERY = 4
-DEFAULTS = {"imagefilter": (200, 3), "matmul": (200, 3), "pagerank": (10, 1), "lu": (3, 1)}
+DEFAULTS = {"imagefilter": (200, 3), "matmul": (200, 3), "pagerank": (10, 1), "lu": (3, 1), "apps": (8, 1)}
 
 
 def log(msg):
@@ -541,6 +545,218 @@ def bench_serving(args, U):
     return out
 
 
+# ---------------------------------------------------------------- the other apps (--workload apps)
+
+def kernel_ms(launch, reps=20):
+    """Average device time of `launch(stream)` over `reps` back-to-back calls, from HIP events on
+    the stream the kernel runs on (after two untimed calls)."""
+    from unicorn_amd import hip as H
+    L = H.lib()
+    st = H.Stream()
+    e0, e1 = C_void(), C_void()
+    H.check(L.uh_event_create(C_ref(e0)), "event")
+    H.check(L.uh_event_create(C_ref(e1)), "event")
+    for _ in range(2):
+        launch(st.h)
+    st.sync()
+    H.check(L.uh_event_record(e0.value, st.h), "record")
+    for _ in range(reps):
+        launch(st.h)
+    H.check(L.uh_event_record(e1.value, st.h), "record")
+    H.check(L.uh_event_sync(e1.value), "event sync")
+    ms = C_float()
+    H.check(L.uh_event_elapsed_ms(C_ref(ms), e0.value, e1.value), "elapsed")
+    L.uh_event_destroy(e0.value)
+    L.uh_event_destroy(e1.value)
+    return ms.value / reps
+
+
+def C_void():
+    import ctypes
+    return ctypes.c_void_p()
+
+
+def C_float():
+    import ctypes
+    return ctypes.c_float()
+
+
+def C_ref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+def app_row(name, workload, run_gpu, steps, parity, cpu_run, cores, kernel, dtype):
+    """tasks/s of one app task on the HIP stub (RESIDENT: per-subtask launches of the app's GPU
+    callback on HBM mirrors), with the dominant kernel's roofline and the CPU-stub baseline."""
+    import unicorn_amd as U
+    U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+    run_gpu()
+    elapsed, lat = time_tasks(run_gpu, steps)
+    cpu = None
+    if cpu_run is not None:
+        t0 = time.perf_counter()
+        cpu_run()
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / dt, 4), "unit": "tasks/s", "cores": cores, "kind": "port",
+               "sample": f"one full task on the pm:: CPU stubs (the app's CPU callbacks), {cores} threads, {dt:.3f} s"}
+    return {"metric": f"tasks/s ({name} task, HIP stub resident)", "value": round(steps / elapsed, 3),
+            "unit": "tasks/s", "steps": steps, "ms_per_step": round(1e3 * elapsed / steps, 4), "dtype": dtype,
+            "config": {"workload": workload, "hip_mode": "resident"}, **lat_stats(lat), "parity": parity,
+            "roofline": kernel, "cpu_baseline": cpu}
+
+
+def hbm_roofline(kernel_name, nbytes, ms, what):
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel_name),
+            "kernel": kernel_name, "kernel_ms": round(ms, 5), "algorithmic_bytes_per_launch": int(nbytes),
+            "per_launch": what}
+
+
+def bench_apps(args, U, steps, warmup, cores):
+    """The apps outside the headline configurations, each at the reference's default size:
+    matrixTranspose 16384 x 4096 int32 (2048-square blocks), prefixSum of 10M int32 (1M per
+    subtask), sparseSolver n = 1024 at density 0.001, fft 4096 x 4096 complex float (512 rows /
+    columns per subtask), pageRankInvertedIndex over a 1M-page web dump (the reference's dump has
+    100M pages; 250K pages per subtask, 2 iterations). One step = one app task (both tasks for fft,
+    the inversion plus both iterations for pageRankInvertedIndex). The headline of the line is
+    fft; the others are under rows."""
+    from unicorn_amd import hip as H
+    from oracle import oracle_lib as O
+    L = H.lib()
+    rows = {}
+    rng = np.random.default_rng(SEED)
+    use_cpu = not args.no_cpu_baseline
+
+    # fft (headline of this line)
+    log("apps: fft 4096^2")
+    n = 4096
+    x = (rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))).astype(np.complex64)
+    f = U.Fft2D(n, n, 512)
+    f.set_input(x)
+    U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+    f.run(U.DEVICE_GPU)
+    got = f.output()
+    ref = O.fft2(x)
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    parity = {"checked": True, "rel_l2_vs_fp64": err, "tolerance": 1e-5, "pass": err <= 1e-5,
+              "against": "oracle 2-D DFT in double"}
+    d_in, d_out = H.DeviceBuffer.from_array(x), H.DeviceBuffer(x.nbytes)
+    ms_row = kernel_ms(lambda s: L.uh_fft_c2c_f32(d_in.ptr, n, 1, d_out.ptr, n, 1, n, 512, s))
+    ms_col = kernel_ms(lambda s: L.uh_fft_c2c_f32(d_in.ptr, 1, n, d_out.ptr, 1, n, n, 512, s))
+    d_in.free()
+    d_out.free()
+    kern = hbm_roofline("k_fft (columns)", 512 * n * 16, ms_col, "512 columns of 4096 points (NATURAL view)")
+    kern["rows_kernel"] = hbm_roofline("k_fft (rows)", 512 * n * 16, ms_row, "512 rows of 4096 points")
+    head = app_row("fft 4096x4096", "fft 4096x4096 complex float, 512 rows / columns per subtask",
+                   lambda: f.run(U.DEVICE_GPU), steps, parity, (lambda: f.run(U.DEVICE_CPU)) if use_cpu else None,
+                   cores, kern, "c64 (f32 pairs)")
+    f.close()
+
+    log("apps: matrixTranspose 16384x4096")
+    R, Cc = 1 << 14, 1 << 12
+    M = rng.integers(-2**31, 2**31 - 1, (R, Cc), dtype=np.int64).astype(np.int32)
+    mt = U.MatTranspose(R, Cc)
+    mt.set_input(M)
+    U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+    mt.run(U.DEVICE_GPU)
+    ok = bool(np.array_equal(mt.output(), M.T))
+    d_in, d_out = H.DeviceBuffer.from_array(M[:2048]), H.DeviceBuffer(2048 * 2048 * 4)
+    ms = kernel_ms(lambda s: L.uh_transpose32(d_in.ptr, Cc, d_out.ptr, 2048, 2048, 2048, s))
+    d_in.free()
+    d_out.free()
+    rows["matrixTranspose"] = app_row(
+        "matrixTranspose 16384x4096", "matrixTranspose 16384x4096 int32 out of place, 2048-square blocks",
+        lambda: mt.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "M.T"},
+        (lambda: mt.run(U.DEVICE_CPU)) if use_cpu else None, cores,
+        hbm_roofline("k_transpose32", 2 * 2048 * 2048 * 4, ms, "one 2048x2048 block"), "i32")
+    mt.close()
+
+    log("apps: prefixSum 10M")
+    N = 10_000_000
+    v = rng.integers(-1000, 1000, N).astype(np.int32)
+    ps = U.PrefixSum(N, 1_000_000)
+    ps.set_input(v)
+    U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+    ps.run(U.DEVICE_GPU)
+    ok = bool(np.array_equal(ps.output(), O.prefix_sum(v)))
+    d_in, d_out = H.DeviceBuffer.from_array(v[:1_000_000]), H.DeviceBuffer(4_000_000)
+    ms = kernel_ms(lambda s: L.uh_scan_i32(d_in.ptr, d_out.ptr, 1_000_000, s))
+    d_in.free()
+    d_out.free()
+    rows["prefixSum"] = app_row(
+        "prefixSum 10M", "prefixSum 10,000,000 int32, 1,000,000 per subtask (scan task, host aux scan, add task)",
+        lambda: ps.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "oracle serialPrefixSum"},
+        (lambda: ps.run(U.DEVICE_CPU)) if use_cpu else None, cores,
+        hbm_roofline("k_scan_totals + k_scan_blocks", 8 * 1_000_000, ms, "one 1,000,000-element chunk (both passes)"),
+        "i32")
+    ps.close()
+
+    log("apps: sparseSolver 1024")
+    n = 1024
+    k = max(1, int(n * n * 0.001))
+    idx = np.unique(rng.choice(n * n, size=k, replace=False))
+    r_, c_ = (idx // n).astype(np.int32), (idx % n).astype(np.int32)
+    vals = rng.standard_normal(r_.size).astype(np.float32)
+    B = rng.standard_normal((n, n)).astype(np.float32)
+    sm = U.SparseMatMul(n, r_, c_, vals, B)
+    U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+    sm.run(U.DEVICE_GPU)
+    ok = bool(np.array_equal(sm.output(), O.spmm_coo(r_, c_, vals, B)))
+    ent = np.zeros((r_.size, 3), np.uint32)
+    ent[:, 0], ent[:, 1], ent[:, 2] = r_.view(np.uint32), c_.view(np.uint32), vals.view(np.uint32)
+    d_e, d_b, d_c = H.DeviceBuffer.from_array(ent), H.DeviceBuffer.from_array(B), H.DeviceBuffer(n * n * 4)
+    ms = kernel_ms(lambda s: L.uh_spmm_coo_f32(d_e.ptr, r_.size, 0, n, d_b.ptr, n, d_c.ptr, n, n, s))
+    d_e.free(); d_b.free(); d_c.free()
+    rows["sparseSolver"] = app_row(
+        "sparseSolver 1024", f"sparseSolver n=1024, {r_.size} non-zeros (density 0.001), one 1024-square block",
+        lambda: sm.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "oracle COO loop (float)"},
+        (lambda: sm.run(U.DEVICE_CPU)) if use_cpu else None, cores,
+        hbm_roofline("k_spmm_coo", 12 * r_.size + 4 * r_.size * n + 4 * n * n, ms,
+                     "one 1024-row block: entries, the B rows they select, C"), "f32")
+    sm.close()
+
+    log("apps: pageRankInvertedIndex 1M pages")
+    import tempfile
+    pages, P = 1_000_000, 250_000
+    rp, col = U.webgen(pages, SEED)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        U.webdump_write(tmp, rp, col, 100, 50_000)
+        a = U.PageRankInvertedIndex(tmp, P)
+        U.set_hip_mode(U.PM_HIP_MODE_RESIDENT)
+        a.run(U.DEVICE_GPU, 2)
+        e, counts = a.index()
+        y = a.ranks()
+        recs = O.prii_records(rp, col, 100)
+        eo, co = O.prii_invert(recs, P)
+        yo = O.prii_pagerank(eo, co, pages, P, 2).astype(np.float64)
+        rel = float(np.abs(y - yo).sum() / np.abs(yo).sum())
+        parity = {"checked": True, "index_bit_exact": bool(np.array_equal(e, eo)), "ranks_rel_l1_vs_oracle": rel,
+                  "tolerance": 1e-5, "pass": bool(np.array_equal(e, eo)) and rel <= 1e-5,
+                  "against": "oracle invertIndex_cpu + redistribution, pageRank_cpu"}
+        sub = np.ascontiguousarray(recs[:P])
+        links = int(sub[:, 0].sum())
+        d_d, d_o, d_t = H.DeviceBuffer.from_array(sub), H.DeviceBuffer(P * 100 * 12), H.DeviceBuffer(4 * 4)
+        ms_inv = kernel_ms(lambda s: L.uh_invert_index(d_d.ptr, P, 100, 0, pages, P, 4, d_o.ptr, d_t.ptr, s), reps=5)
+        d_d.free(); d_o.free(); d_t.free()
+        n0 = int(co[0])
+        d_e, d_x, d_y = H.DeviceBuffer.from_array(eo[:n0]), H.DeviceBuffer.from_array(y), H.DeviceBuffer(P * 4)
+        ms_rank = kernel_ms(lambda s: L.uh_prii_rank(d_e.ptr, n0, d_x.ptr, 1.0, 0.85, 0, d_y.ptr, P, pages, s))
+        d_e.free(); d_x.free(); d_y.free()
+        kern = hbm_roofline("uh_invert_index (4 kernels)", 4 * P + 4 * links + 12 * links, ms_inv,
+                            f"one 250K-page subtask: counts and {links} links read, {links} entries written")
+        kern["rank_kernel"] = hbm_roofline("k_prii_rank", 12 * n0 + 4 * n0 + 4 * P, ms_rank,
+                                           f"one subtask's {n0} inlinks: entries, source ranks, output ranks")
+        rows["pageRankInvertedIndex"] = app_row(
+            "pageRankInvertedIndex 1M pages", f"pageRankInvertedIndex {pages} pages ({col.size} links) from a web "
+            f"dump, 250K pages per subtask, inversion + 2 iterations", lambda: a.run(U.DEVICE_GPU, 2), max(2, steps // 4),
+            parity, (lambda: a.run(U.DEVICE_CPU, 2)) if use_cpu else None, cores, kern, "u32 index, f32 ranks")
+        a.close()
+    head.update({"warmup": 1, "rows": rows})
+    return head
+
+
 def copy_calibration(nbytes, reps=3):
     """D2D copies of `nbytes` (the size of one image): the reference point for the HBM counters
     of a profiled run (FETCH_SIZE / WRITE_SIZE of a copy of known size)."""
@@ -554,7 +770,8 @@ def copy_calibration(nbytes, reps=3):
     dst.free()
 
 
-BENCHES = {"imagefilter": bench_imagefilter, "matmul": bench_matmul, "pagerank": bench_pagerank, "lu": bench_lu}
+BENCHES = {"imagefilter": bench_imagefilter, "matmul": bench_matmul, "pagerank": bench_pagerank, "lu": bench_lu,
+           "apps": bench_apps}
 
 
 def main():

@@ -433,7 +433,20 @@ static void fft1_f64(double* x, size_t n, size_t s) {
 void or_fft2_f64(const float* in, double* out, size_t rows, size_t cols) {
     for (size_t i = 0; i < 2 * rows * cols; ++i) out[i] = (double)in[i];
     for (size_t r = 0; r < rows; ++r) fft1_f64(out + 2 * r * cols, cols, 1);      /* row transforms */
-    for (size_t c = 0; c < cols; ++c) fft1_f64(out + 2 * c, rows, cols);          /* column transforms */
+    /* column transforms, each on a contiguous copy of the column */
+    double* col = (double*)malloc(2 * rows * sizeof(double));
+    for (size_t c = 0; c < cols; ++c) {
+        for (size_t r = 0; r < rows; ++r) {
+            col[2 * r] = out[2 * (r * cols + c)];
+            col[2 * r + 1] = out[2 * (r * cols + c) + 1];
+        }
+        fft1_f64(col, rows, 1);
+        for (size_t r = 0; r < rows; ++r) {
+            out[2 * (r * cols + c)] = col[2 * r];
+            out[2 * (r * cols + c) + 1] = col[2 * r + 1];
+        }
+    }
+    free(col);
 }
 
 /* ------------------------------------------------------------------ pageRank over an inverted index */

@@ -210,8 +210,9 @@ def prii_records(row_ptr: np.ndarray, col: np.ndarray, max_out: int) -> np.ndarr
     rec = np.zeros((pages, max_out + 1), np.uint32)
     deg = np.diff(row_ptr.astype(np.int64))
     rec[:, 0] = deg
-    for p in np.nonzero(deg)[0]:
-        rec[p, 1:1 + deg[p]] = col[row_ptr[p]:row_ptr[p + 1]] + 1
+    src = np.repeat(np.arange(pages), deg)
+    slot = np.arange(deg.sum()) - np.repeat(row_ptr[:-1].astype(np.int64), deg)
+    rec[src, 1 + slot] = np.asarray(col[:deg.sum()], dtype=np.uint32) + 1
     return rec
 
 

@@ -368,3 +368,21 @@ def test_spmm_coo_planted_mistake():
     good = O.spmm_coo(r, c, v, B, f64=True)
     assert not np.allclose(O.spmm_coo(c, r, v, B, f64=True), good)          # A transposed
     assert np.allclose(O.spmm_coo(r, c, v, B), good, rtol=1e-5, atol=1e-5)
+
+
+
+def test_spmm_float_order_brute_force():
+    """The float form rounds each product, then each sum, entry by entry in COO order: a Python
+    loop over np.float32 scalars gives the same bits; a transposed B is caught."""
+    n = 12
+    rng = np.random.default_rng(4)
+    idx = np.unique(rng.choice(n * n, size=40, replace=False))
+    r, c = (idx // n).astype(np.int32), (idx % n).astype(np.int32)
+    v = rng.standard_normal(r.size).astype(np.float32)
+    B = rng.standard_normal((n, n)).astype(np.float32)
+    want = np.zeros((n, n), np.float32)
+    for k in range(r.size):
+        for j in range(n):
+            want[r[k], j] = np.float32(want[r[k], j] + np.float32(v[k] * B[c[k], j]))
+    np.testing.assert_array_equal(O.spmm_coo(r, c, v, B), want)
+    assert not np.array_equal(O.spmm_coo(r, c, v, np.ascontiguousarray(B.T)), want)

@@ -66,6 +66,9 @@ def lib() -> C.CDLL:
             "uh_batch_destroy": (i32, [vp]),
             "uh_batch_flush": (i32, [vp, i32]),
             "uh_launch_counters": (i32, [C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+            "uh_transpose32": (i32, [vp, u64, vp, u64, u32, u32, vp]),
+            "uh_scan_i32": (i32, [vp, vp, u64, vp]),
+            "uh_spmm_coo_f32": (i32, [vp, u32, C.c_int32, u32, vp, u64, vp, u64, u32, vp]),
             "uh_fft_c2c_f32": (i32, [vp, u64, u64, vp, u64, u64, u32, u32, vp]),
             "uh_invert_index": (i32, [vp, u32, u32, u32, u32, u32, u32, vp, vp, vp]),
             "uh_prii_rank": (i32, [vp, u32, vp, f32, f32, u32, vp, u32, u32, vp]),
