@@ -56,6 +56,7 @@ typedef struct pmHipStubStats
     uint64_t ns_wake;           /* task completion -> blocked waiter resumed, per wait */
     uint64_t ma_copies;         /* multi-assigned copies of CPU subtasks run by the HIP stub */
     uint64_t ma_wins;           /* of those, copies that finished first and committed */
+    uint64_t oom_waits;         /* device allocations retried after retiring the oldest in-flight subtask */
 } pmHipStubStats;
 
 #ifdef __cplusplus

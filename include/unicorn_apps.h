@@ -28,7 +28,7 @@ typedef struct ua_stats {
     uint64_t subtasks, batches, h2d_bytes, d2h_bytes, d2d_bytes, cache_hits, evictions;
     uint64_t plain_launches, persistent_launches, queue_items;
     uint64_t ns_queue, ns_stage, ns_flush, ns_wait, ns_retire, ns_wake;   /* see pmHipStubStats */
-    uint64_t ma_copies, ma_wins;
+    uint64_t ma_copies, ma_wins, oom_waits;
 } ua_stats;
 
 int ua_init(int cpu_stubs);                   /* pmInitialize; cpu_stubs 0 = one per core */

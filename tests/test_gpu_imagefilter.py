@@ -29,6 +29,17 @@ def _case(rng, H, W, r, mode):
     return img, w
 
 
+@pytest.fixture(params=[1, 0], ids=["mfma", "dot4"])
+def engine(gpu_runtime, request):
+    """The kernel engine of radius 1 and 2 (uh_imagefilter_engine): int8 MFMA or v_dot4."""
+    from unicorn_amd import hip as H
+    import ctypes as C
+    prev = C.c_int()
+    H.check(H.lib().uh_imagefilter_engine(request.param, C.byref(prev)), "engine")
+    yield request.param
+    H.check(H.lib().uh_imagefilter_engine(prev.value, None), "engine")
+
+
 @pytest.mark.parametrize("name,hip_mode,compact,graphs", MODES)
 @pytest.mark.parametrize("H,W,r,mode,tile", [
     (300, 517, 2, 0, 128),     # ragged tiles, tile not a multiple of the kernel tile
@@ -37,8 +48,9 @@ def _case(rng, H, W, r, mode):
     (4, 3, 2, 1, 2),           # image smaller than the filter window
     (257, 1030, 3, 0, 256),    # radius 3: direct kernel
     (1030, 2100, 2, 1, 512),
+    (333, 1001, 1, 1, 300),    # radius 1, odd widths: staged rows at every byte alignment
 ])
-def test_imagefilter_gpu_matches_oracle(gpu_runtime, name, hip_mode, compact, graphs, H, W, r, mode, tile):
+def test_imagefilter_gpu_matches_oracle(gpu_runtime, engine, name, hip_mode, compact, graphs, H, W, r, mode, tile):
     U = gpu_runtime
     U.set_hip_mode(hip_mode, compact=compact, graphs=graphs)
     rng = np.random.default_rng(H * 7 + W * 13 + r + 100 * mode + tile)
@@ -54,7 +66,7 @@ def test_imagefilter_gpu_matches_oracle(gpu_runtime, name, hip_mode, compact, gr
     assert np.array_equal(got, want), f"{name}: {np.count_nonzero(got != want)} bytes differ"
 
 
-def test_imagefilter_gpu_repeat_resident(gpu_runtime):
+def test_imagefilter_gpu_repeat_resident(gpu_runtime, engine):
     """The same task twice on a resident input (second run reuses the HBM mirror)."""
     U = gpu_runtime
     U.set_hip_mode(2, graphs=True)
@@ -75,7 +87,7 @@ def test_imagefilter_gpu_repeat_resident(gpu_runtime):
     assert s["h2d_bytes"] <= 2 * 2048 * 2048 * 3 * 2
 
 
-def test_imagefilter_gpu_cpu_gpu_mix(gpu_runtime):
+def test_imagefilter_gpu_cpu_gpu_mix(gpu_runtime, engine):
     """CPU and GPU stubs on one task (the reference's CPU+GPU mode)."""
     U = gpu_runtime
     U.set_hip_mode(1)

@@ -33,8 +33,10 @@ def test_builtin_reductions_on_gpu(gpu_runtime, lib, hip_mode, ti):
 
 
 @pytest.mark.parametrize("device", [1, 2])
-@pytest.mark.parametrize("hip_mode", [0, 2])
+@pytest.mark.parametrize("hip_mode", [0, 1, 2])
 def test_redistribution_on_gpu(gpu_runtime, lib, device, hip_mode):
+    """PLAIN stages the subtasks through host memory and redistributes there; a GPU-only task in
+    the resident modes keeps its output in HBM and is redistributed there (one copy kernel)."""
     gpu_runtime.set_hip_mode(hip_mode)
     subtasks, n, orders = 8, 1000, 5
     rc, out, meta = run_redistribute(lib, subtasks, n, orders, device)

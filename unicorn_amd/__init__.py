@@ -59,7 +59,8 @@ class ua_stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "subtasks", "batches", "h2d_bytes", "d2h_bytes", "d2d_bytes", "cache_hits", "evictions",
         "plain_launches", "persistent_launches", "queue_items",
-        "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake", "ma_copies", "ma_wins")]
+        "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake", "ma_copies", "ma_wins",
+        "oom_waits")]
 
 
 class ua_if_config(C.Structure):

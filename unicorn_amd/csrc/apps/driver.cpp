@@ -75,6 +75,7 @@ int ua_get_stats(ua_stats* s) {
     s->ns_wake = h.ns_wake;
     s->ma_copies = h.ma_copies;
     s->ma_wins = h.ma_wins;
+    s->oom_waits = h.oom_waits;
     s->plain_launches = s->persistent_launches = s->queue_items = 0;
     if (pmHasHipDevice()) uh_launch_counters(&s->plain_launches, &s->persistent_launches, &s->queue_items);
     return 0;

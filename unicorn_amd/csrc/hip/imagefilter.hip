@@ -25,6 +25,7 @@
 #include "uh_internal.h"
 
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 
 namespace uh {
@@ -333,6 +334,289 @@ __device__ __forceinline__ void filter_epilogue(const FilterArgs& a, int x0, int
 }
 
 
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }   // vmcnt(0)
+
+// ---------------------------------------------------------------- MFMA engine (radius 1 and 2)
+//
+// For one filter row dy, the output bytes of an image row are a banded matrix product over the
+// raw interleaved input bytes: output byte j (pixel x = j / 3, channel c = j % 3) takes input
+// byte 3 (x + dx) + 2 - c = j + 3 dx + 2 - 2c for dx in [-R, R] (the B<->R swap included), so
+//   out[y][j] = sum_dy sum_k in[y + dy][k] T_dy[k][j],   T_dy[j + 3dx + 2 - 2c(j)][j] = w[dy][dx].
+// v_mfma_i32_32x32x32_i8 computes it for 32 output bytes (M) x 32 image rows (N): A = the band of
+// T (32 outputs x 32 input bytes), B = 32 staged input bytes of 32 rows, read straight from the
+// raw window in LDS (no de-interleave). A block of 32 outputs needs the 48 input bytes from 8
+// before it, covered by two K steps of 32; the three blocks of a wave's 96 bytes (32 pixels) share
+// their middle steps, so a filter row costs 4 B reads and 6 MFMAs per 32 rows x 96 bytes. The band
+// depends on the block's byte phase mod 3 (blocks at 0, 32, 64 have phases 0, 2, 1): A holds
+// (2R+1) rows x 3 phases x 2 steps operands, built once per work item from the packed weights
+// with one v_perm per dword (byte selectors from a table fixed at compile time).
+// Input bytes are biased (XOR 0x80) into int8 on the way to the MFMA; the accumulators start at
+// 128 * sum(w), which undoes the bias, as in the dot4 engine. Tiles are 128 pixels x 32 rows, one
+// wave per 32 pixels; the raw window of the next tile streams into the other of two LDS buffers
+// while a tile computes. At a view edge the out-of-view pixels of the window are overwritten with
+// the edge pixel (the reference clamps rows and columns to the image) before the products.
+static constexpr int FM_W = 128;                  // tile width (pixels)
+static constexpr int FM_H = 32;                   // tile height (rows)
+static constexpr int FM_ROW_CHUNKS = 28;          // staged 16-byte chunks per row (from pixel x0 - 4)
+// LDS row pitch: one unused chunk per row. The 32 lanes of a B read take the same column of 32
+// rows; with a 28-chunk pitch (112 dwords) the rows fell on 2-4 bank pairs, with 29 chunks
+// (58 qwords) on 16.
+static constexpr int FM_PITCH_CHUNKS = 29;
+static constexpr int FM_ROW = FM_PITCH_CHUNKS * 16;
+
+template <int R>
+struct FmGeom {
+    static constexpr int LROWS = FM_H + 2 * R;
+    static constexpr int CHUNKS = LROWS * FM_PITCH_CHUNKS;   // chunk slots, the pad slots included
+};
+// whole 1 KB wave destinations (64 lanes x 16 bytes) per buffer
+static constexpr int FM_BUF = (FmGeom<2>::CHUNKS + 63) / 64 * 1024;
+
+struct FmLds {
+    uint8_t buf[2][FM_BUF];
+};
+
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
+
+// Byte selectors of the A operands. Lane l holds output byte m = l % 32 of a block and the K bytes
+// k = 16 (l / 32) + 0..15 of a step (the B operand uses the same K numbering for its 16 bytes).
+// Byte k of step s of a block of phase p is tap dx of output m when
+//   k + 32 s - 8 = m + 3 dx + 2 - 2 c,  c = (p + m) mod 3
+// (input byte relative to the block's first output byte, the window starting 8 bytes before it).
+// Selector byte = dx + R (a byte of the packed weight row: bytes 0-3 of lo, byte 4 = byte 0 of
+// hi), or 0x0c (zero).
+struct FmSel {
+    uint32_t v[2][64][3][2][4];   // [R - 1][lane][block][step][dword]
+};
+constexpr FmSel fm_make_sel() {
+    FmSel t{};
+    for (int r = 1; r <= 2; ++r)
+        for (int l = 0; l < 64; ++l)
+            for (int b = 0; b < 3; ++b)
+                for (int st = 0; st < 2; ++st)
+                    for (int d = 0; d < 4; ++d) {
+                        uint32_t sel = 0;
+                        const int m = l % 32, h = l / 32, p = (32 * b) % 3, c = (p + m) % 3;
+                        for (int bb = 0; bb < 4; ++bb) {
+                            const int k = 16 * h + 4 * d + bb;
+                            const int u = k + 32 * st - 8 - m - 2 + 2 * c;    // 3 dx
+                            uint32_t byte = 0x0c;
+                            if (u >= -3 * r && u <= 3 * r && (u + 30) % 3 == 0) byte = (uint32_t)(u / 3 + r);
+                            sel |= byte << (8 * bb);
+                        }
+                        t.v[r - 1][l][b][st][d] = sel;
+                    }
+    return t;
+}
+__device__ const FmSel g_fm_sel = fm_make_sel();
+
+template <int R>
+struct FmA {
+    i32x4 op[2 * R + 1][3][2];
+};
+
+template <int R>
+__device__ __forceinline__ void fm_build_a(const FilterWeights& w, FmA<R>& A) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            uint32_t sel[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) sel[d] = g_fm_sel.v[R - 1][lane][b][st][d];
+#pragma unroll
+            for (int dy = 0; dy < 2 * R + 1; ++dy) {
+                const uint32_t lo = w.wv[(dy * 4 + R) * 2], hi = w.wv[(dy * 4 + R) * 2 + 1];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) A.op[dy][b][st][d] = (int32_t)__builtin_amdgcn_perm(hi, lo, sel[d]);
+            }
+        }
+}
+
+__device__ __forceinline__ void fm_origin(const FilterArgs& a, uint32_t tile, int& x0, int& y0) {
+    const uint32_t tiles_y = (uint32_t)((a.rows + FM_H - 1) / FM_H);
+    x0 = a.col0 + (int)(tile / tiles_y) * FM_W;
+    y0 = a.row0 + (int)(tile % tiles_y) * FM_H;
+}
+
+// Raw window of a tile: staged row lr holds the FM_ROW bytes from the 16-byte boundary at or
+// below pixel x0 - 4 of image row y0 - R + lr (rows clamped to the view), as 16-byte
+// direct-to-LDS buffer loads (see filter_dma).
+template <int R>
+__device__ __forceinline__ void fm_dma(const FilterArgs& a, int x0, int y0, uint8_t* buf) {
+    constexpr int CHUNKS = FmGeom<R>::CHUNKS;
+    const FilterView v = filter_view(a);
+    const int mis = (int)((uintptr_t)a.in & 15u);
+    const int extent = (int)__umul24((uint32_t)(a.view_last_row - v.first_row), (uint32_t)a.in_stride) + a.pad * 3;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
+        const int c0 = i * FT_THREADS + wave * 64;
+        if (c0 >= CHUNKS) break;
+        const int c = c0 + lane;
+        const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
+        if (c < CHUNKS && ch < FM_ROW_CHUNKS) {
+            const int st = filter_row_start<R>(a, v, x0, y0, lr) + mis;
+            const int off = (st & ~15) + ch * 16;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)(buf + c0 * 16), 16, (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
+        }
+    }
+}
+
+// Byte of pixel x0 - 4 in staged row lr.
+template <int R>
+__device__ __forceinline__ int fm_row_byte(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr, int mis) {
+    return lr * FM_ROW + ((filter_row_start<R>(a, v, x0, y0, lr) + mis) & 15);
+}
+
+// Out-of-view pixels of the window that valid outputs read get the edge pixel's bytes.
+template <int R>
+__device__ __forceinline__ bool fm_needs_patch(const FilterArgs& a, const FilterView& v, int x0) {
+    const int cend = a.col0 + a.cols;
+    const int xl = (x0 + FM_W < cend ? x0 + FM_W : cend) - 1;
+    return x0 - R < v.first_col || xl + R > v.last_col;
+}
+
+template <int R>
+__device__ __forceinline__ void fm_patch(const FilterArgs& a, int x0, int y0, uint8_t* buf) {
+    const FilterView v = filter_view(a);
+    const int mis = (int)((uintptr_t)a.in & 15u);
+    constexpr int LROWS = FmGeom<R>::LROWS;
+    // (row, side, k): pixel first_col - 1 - k or last_col + 1 + k, k < R
+    for (int t = threadIdx.x; t < LROWS * 2 * R; t += FT_THREADS) {
+        const int lr = t / (2 * R), side = (t / R) & 1, k = t % R;
+        const int px = side ? v.last_col + 1 + k : v.first_col - 1 - k;
+        const int src = side ? v.last_col : v.first_col;
+        if (px < x0 - R || px > x0 + FM_W - 1 + R) continue;
+        const int rb = fm_row_byte<R>(a, v, x0, y0, lr, mis);
+        uint8_t* d = buf + rb + (px - (x0 - 4)) * 3;
+        const uint8_t* sp = buf + rb + (src - (x0 - 4)) * 3;
+        const uint8_t b0 = sp[0], b1 = sp[1], b2 = sp[2];
+        d[0] = b0; d[1] = b1; d[2] = b2;
+    }
+}
+
+// 16 bytes of LDS at byte address q, as an int8 MFMA operand biased by XOR 0x80.
+template <bool AL8>
+__device__ __forceinline__ i32x4 fm_load_b(const uint8_t* base, int q) {
+    uint32_t d[4];
+    if constexpr (AL8) {
+        const uint2 u0 = *(const uint2*)(base + q), u1 = *(const uint2*)(base + q + 8);
+        d[0] = u0.x; d[1] = u0.y; d[2] = u1.x; d[3] = u1.y;
+    } else {
+        const uint32_t* w = (const uint32_t*)(base + (q & ~3));
+        const uint32_t sh = (uint32_t)q & 3u;
+        const uint32_t e0 = w[0], e1 = w[1], e2 = w[2], e3 = w[3], e4 = w[4];
+        d[0] = __builtin_amdgcn_alignbyte(e1, e0, sh);
+        d[1] = __builtin_amdgcn_alignbyte(e2, e1, sh);
+        d[2] = __builtin_amdgcn_alignbyte(e3, e2, sh);
+        d[3] = __builtin_amdgcn_alignbyte(e4, e3, sh);
+    }
+    i32x4 r;
+    r.x = (int32_t)(d[0] ^ 0x80808080u);
+    r.y = (int32_t)(d[1] ^ 0x80808080u);
+    r.z = (int32_t)(d[2] ^ 0x80808080u);
+    r.w = (int32_t)(d[3] ^ 0x80808080u);
+    return r;
+}
+
+// Products of one tile for this wave's 96 bytes, then the stores.
+template <int R, int MODE, bool AL8>
+__device__ __forceinline__ void fm_tile(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A, int x0, int y0,
+                                        const uint8_t* buf) {
+    const FilterView v = filter_view(a);
+    const int mis = (int)((uintptr_t)a.in & 15u);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, h = lane >> 5;
+    const int bias = 128 * w.wsum;
+    i32x16 acc[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][r] = bias;
+#pragma unroll
+    for (int dy = 0; dy < 2 * R + 1; ++dy) {
+        // tile byte 96 wave + 32 q - 8 + 16 h of staged row n + dy (pixel x0 - 4 is 12 bytes before tile byte 0)
+        const int q0 = fm_row_byte<R>(a, v, x0, y0, n + dy, mis) + 4 + 96 * wave + 16 * h;
+        const i32x4 b0 = fm_load_b<AL8>(buf, q0);
+        const i32x4 b1 = fm_load_b<AL8>(buf, q0 + 32);
+        const i32x4 b2 = fm_load_b<AL8>(buf, q0 + 64);
+        const i32x4 b3 = fm_load_b<AL8>(buf, q0 + 96);
+        acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][0], b0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][0], b1, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][0], b2, acc[2], 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][1], b1, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][1], b2, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][1], b3, acc[2], 0, 0, 0);
+    }
+    // D[m][n]: column n = lane % 32 (tile row), row m = 8 (r / 4) + 4 (lane / 32) + r % 4 (byte of the block)
+    const int y = y0 + n;
+    if (y >= a.row0 + a.rows) return;
+    const int nbytes = a.cols * 3;
+    const int jt = (x0 - a.col0) * 3 + 96 * wave + 4 * h;    // tile-relative byte of this lane's first dword
+    UH_GLOBAL uint8_t* row = as_global(a.out) + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride);
+    const int shift = a.shift;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t o[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = acc[b][4 * i + e];
+                o[e] = MODE == 0 ? (uint32_t)s : (uint32_t)clampi(s >> shift, 0, 255);
+            }
+            const int j = jt + 32 * b + 8 * i;
+            UH_GLOBAL uint8_t* dst = row + j;
+            if (j + 4 <= nbytes && (((uintptr_t)dst) & 3u) == 0) {
+                *(UH_GLOBAL uint32_t*)dst = pack4(o[0], o[1], o[2], o[3]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (j + e < nbytes) dst[e] = (uint8_t)o[e];
+            }
+        }
+}
+
+// Tiles l0 .. l1-1 of one item (all tiles in one item share its arguments and weights).
+template <int R, int MODE>
+__device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const FilterWeights& w, uint32_t l0, uint32_t l1) {
+    if (l0 >= l1) return;
+    FmA<R> A;
+    fm_build_a<R>(w, A);
+    __syncthreads();              // a previous item's last tile may still be reading a buffer
+    const FilterView v = filter_view(a);
+    // every staged row starts at the same byte offset mod 8 when the stride is a multiple of 8
+    const int mis = (int)((uintptr_t)a.in & 15u);
+    int x0, y0;
+    fm_origin(a, l0, x0, y0);
+    fm_dma<R>(a, x0, y0, sm.buf[0]);
+    int cur = 0;
+    for (uint32_t t = l0; t < l1; ++t) {
+        wait_vm_all();
+        __syncthreads();          // tile t's window is in buf[cur]; every wave has left buf[cur ^ 1]
+        int x1 = 0, y1 = 0;
+        if (t + 1 < l1) {
+            fm_origin(a, t + 1, x1, y1);
+            fm_dma<R>(a, x1, y1, sm.buf[cur ^ 1]);
+        }
+        if (fm_needs_patch<R>(a, v, x0)) {
+            fm_patch<R>(a, x0, y0, sm.buf[cur]);
+            __syncthreads();
+        }
+        const bool al8 = (a.in_stride & 7u) == 0 && ((fm_row_byte<R>(a, v, x0, y0, 0, mis) + 4) & 7) == 0;
+        if (al8) fm_tile<R, MODE, true>(a, w, A, x0, y0, sm.buf[cur]);
+        else fm_tile<R, MODE, false>(a, w, A, x0, y0, sm.buf[cur]);
+        x0 = x1; y0 = y1;
+        cur ^= 1;
+    }
+}
+
 // Direct path for any radius <= 15: one output pixel per thread.
 __device__ __forceinline__ void filter_pixel_direct(const FilterArgs& a, const int8_t* weights, int i, int j) {
     const int r = a.radius, D = 2 * r + 1;
@@ -368,13 +652,20 @@ __device__ __forceinline__ void filter_tile_direct(const FilterArgs& a, const in
     }
 }
 
-// Kernel variants. Variant 0 dispatches on each item's radius and mode (uniform per tile; the
-// direct path stages nothing); variants 1..4 are specialised for one radius and mode
-// (V = 1 + 2 (R - 1) + MODE) and are launched only on items of that kind, which keeps the
-// register allocation of the common case free of the other paths.
-__host__ __device__ constexpr int filter_variant(int radius, int mode) {
-    return (radius == 1 || radius == 2) ? 1 + 2 * (radius - 1) + mode : 0;
+// Kernel variants. Variant 0 dispatches on each item's engine, radius and mode (uniform per
+// tile; the direct path stages nothing); variants 1..4 (dot4 engine) and 5..8 (MFMA engine) are
+// specialised for one radius and mode (V = 1 + 2 (R - 1) + MODE, plus 4 for the MFMA engine) and
+// are launched only on items of that kind, which keeps the register allocation of the common case
+// free of the other paths. An item's engine (FilterWeights::pad[0], 1 = MFMA) also fixes its tile
+// geometry (256 x 16 or 128 x 32 pixels), which every variant follows.
+__host__ __device__ constexpr int filter_variant(int radius, int mode, int mfma) {
+    return (radius == 1 || radius == 2) ? 1 + 2 * (radius - 1) + mode + (mfma ? 4 : 0) : 0;
 }
+
+union FilterShared {
+    FilterLds d;
+    FmLds m;
+};
 
 template <int V>
 __device__ __forceinline__ bool filter_staged(const FilterArgs& a) {
@@ -427,7 +718,6 @@ __device__ __forceinline__ void filter_epilogue_v(const FilterArgs& a, int x0, i
     }
 }
 
-__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }   // vmcnt(0)
 
 // The tile pipeline shared by both kernels: tiles l0 .. l1-1 of one item, whose arguments and
 // weights stay in scalar registers. Per tile n: the dot products of n from the LDS planes, then
@@ -482,12 +772,42 @@ __device__ __forceinline__ void filter_run(FilterLds& sm, const FilterArgs& a, c
 }
 
 // Plain launch: workgroup b computes tiles 2b and 2b + 1 of one item.
+// Tiles l0 .. l1-1 of one item with the engine its variant (or, for variant 0, the item) names.
 template <int V>
-__global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
+__device__ __forceinline__ void filter_item_run(FilterShared& sm, const FilterArgs& a, const FilterWeights& w,
+                                                const int8_t* table, uint32_t l0, uint32_t l1) {
+    if constexpr (V >= 5) {
+        fm_run<(V - 3) / 2, (V - 5) % 2>(sm.m, a, w, l0, l1);
+    } else if constexpr (V == 0) {
+        if (w.pad[0] == 1 && (a.radius == 1 || a.radius == 2)) {
+            if (a.radius == 1) {
+                if (a.mode == 0) fm_run<1, 0>(sm.m, a, w, l0, l1);
+                else fm_run<1, 1>(sm.m, a, w, l0, l1);
+            } else {
+                if (a.mode == 0) fm_run<2, 0>(sm.m, a, w, l0, l1);
+                else fm_run<2, 1>(sm.m, a, w, l0, l1);
+            }
+        } else {
+            filter_run<0>(sm.d, a, w, table, l0, l1);
+        }
+    } else {
+        filter_run<V>(sm.d, a, w, table, l0, l1);
+    }
+}
+
+// Waves per SIMD the variant is compiled for: the MFMA engine (also reachable from variant 0)
+// holds its A operands in registers.
+template <int V>
+struct FilterOcc {
+    static constexpr int value = (V >= 5 || V == 0) ? 2 : FT_WAVES_PER_EU;
+};
+
+template <int V>
+__global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FilterOcc<V>::value)))
 k_imagefilter(FilterItem it, uint32_t tiles) {
-    __shared__ FilterLds sm;
+    __shared__ FilterShared sm;
     const uint32_t l0 = blockIdx.x * 2u;
-    filter_run<V>(sm, head_of(it.a), it.w, it.a.weights, l0, l0 + 2u < tiles ? l0 + 2u : tiles);
+    filter_item_run<V>(sm, head_of(it.a), it.w, it.a.weights, l0, l0 + 2u < tiles ? l0 + 2u : tiles);
 }
 
 // Persistent launch over a device work queue with a static schedule: worker w runs a contiguous
@@ -499,9 +819,9 @@ k_imagefilter(FilterItem it, uint32_t tiles) {
 static constexpr uint32_t FT_PREFIX_LDS = 1024;   // items whose tile prefix is searched in LDS
 
 template <int V>
-__global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
+__global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FilterOcc<V>::value)))
 k_imagefilter_persistent(const uint8_t* queue) {
-    __shared__ FilterLds sm;
+    __shared__ FilterShared sm;
     __shared__ uint32_t s_prefix[FT_PREFIX_LDS + 1];
     const QueueHeader hd = uniform_load((const QueueHeader*)queue);   // n_items, total_tiles, offsets
     const FilterItem* items = (const FilterItem*)(queue + hd.items_offset);
@@ -523,7 +843,7 @@ k_imagefilter_persistent(const uint8_t* queue) {
         if (n) {
             const FilterArgs a = uniform_load((const FilterArgs*)&items[item].a);
             const FilterWeights w = uniform_load(&items[item].w);
-            filter_run<V>(sm, a, w, items[item].a.weights, local, local + n);
+            filter_item_run<V>(sm, a, w, items[item].a.weights, local, local + n);
         }
         t += n;
         ++item;
@@ -532,6 +852,7 @@ k_imagefilter_persistent(const uint8_t* queue) {
 }
 
 uint32_t filter_tiles(const FilterItem& it) {
+    if (it.w.pad[0] == 1) return (uint32_t)(((it.a.cols + FM_W - 1) / FM_W) * ((it.a.rows + FM_H - 1) / FM_H));
     return (uint32_t)(((it.a.cols + FT_W - 1) / FT_W) * ((it.a.rows + FT_H - 1) / FT_H));
 }
 
@@ -540,20 +861,23 @@ int filter_variant_for(uint32_t kinds) {
     return (kinds && (kinds & (kinds - 1)) == 0 && !(kinds & 1u)) ? __builtin_ctz(kinds) : 0;
 }
 
+static constexpr int FILTER_VARIANTS = 9;
 typedef void (*FilterPersistentFn)(const uint8_t*);
-static const FilterPersistentFn kFilterPersistent[5] = {
+static const FilterPersistentFn kFilterPersistent[FILTER_VARIANTS] = {
     k_imagefilter_persistent<0>, k_imagefilter_persistent<1>, k_imagefilter_persistent<2>,
-    k_imagefilter_persistent<3>, k_imagefilter_persistent<4>};
+    k_imagefilter_persistent<3>, k_imagefilter_persistent<4>, k_imagefilter_persistent<5>,
+    k_imagefilter_persistent<6>, k_imagefilter_persistent<7>, k_imagefilter_persistent<8>};
 typedef void (*FilterPlainFn)(FilterItem, uint32_t);
-static const FilterPlainFn kFilterPlain[5] = {
-    k_imagefilter<0>, k_imagefilter<1>, k_imagefilter<2>, k_imagefilter<3>, k_imagefilter<4>};
+static const FilterPlainFn kFilterPlain[FILTER_VARIANTS] = {
+    k_imagefilter<0>, k_imagefilter<1>, k_imagefilter<2>, k_imagefilter<3>, k_imagefilter<4>,
+    k_imagefilter<5>, k_imagefilter<6>, k_imagefilter<7>, k_imagefilter<8>};
 
 // Workgroups of a persistent filter kernel resident per CU (occupancy queries, cached).
-static int g_filter_wg[5] = {};
+static int g_filter_wg[FILTER_VARIANTS] = {};
 
 int filter_wg_per_cu() {
     if (!g_filter_wg[0]) {
-        for (int v = 0; v < 5; ++v) {
+        for (int v = 0; v < FILTER_VARIANTS; ++v) {
             int b = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kFilterPersistent[v], FT_THREADS, 0) != hipSuccess || b < 1)
                 b = 2;
@@ -607,6 +931,28 @@ static int filter_validate(const uh_imagefilter_args* a, int* view_cols) {
     return UH_OK;
 }
 
+// Engine of the staged radii: 0 = dot4 (default), 1 = MFMA; UH_FILTER_ENGINE=mfma selects the
+// MFMA engine at start.
+static std::atomic<int> g_filter_engine{-1};
+
+static int filter_engine() {
+    int e = g_filter_engine.load();
+    if (e < 0) {
+        const char* s = getenv("UH_FILTER_ENGINE");
+        e = (s && strcmp(s, "mfma") == 0) ? 1 : 0;
+        g_filter_engine.store(e);
+    }
+    return e;
+}
+
+extern "C" int uh_imagefilter_engine(int engine, int* previous) {
+    if (engine < -1 || engine > 1) return fail(UH_ERR_ARG, "uh_imagefilter_engine: engine %d", engine);
+    const int prev = filter_engine();
+    if (previous) *previous = prev;
+    if (engine >= 0) g_filter_engine.store(engine);
+    return UH_OK;
+}
+
 extern "C" int uh_imagefilter(const uh_imagefilter_args* args, void* stream) {
     int vcols = 0;
     int rc = filter_validate(args, &vcols);
@@ -615,9 +961,12 @@ extern "C" int uh_imagefilter(const uh_imagefilter_args* args, void* stream) {
     memset(&it, 0, sizeof(it));
     it.a = *args;
     it.a.pad = vcols;   // the kernels read the view width from pad
-    if (it.a.radius == 1 || it.a.radius == 2) pack_filter_weights(it.a.weights, it.a.radius, &it.w);
+    const bool staged = it.a.radius == 1 || it.a.radius == 2;
+    if (staged) pack_filter_weights(it.a.weights, it.a.radius, &it.w);
+    const int mfma = staged && filter_engine() == 1;
+    it.w.pad[0] = mfma;
     const uint32_t tiles = filter_tiles(it);
-    const int v = filter_variant(it.a.radius, it.a.mode);
+    const int v = filter_variant(it.a.radius, it.a.mode, mfma);
     if (Batch* b = as_batch(stream)) {
         record_item(b, FAM_FILTER, &it, sizeof(it), tiles, 1u << v);
         return UH_OK;

@@ -375,10 +375,9 @@ pmStatus pmSubmitTaskWithOptions(pmTaskDetails d, const pmTaskSubmitOptions* opt
     t->multi_assign = d.multiAssignEnabled && t->cpu_allowed && t->gpu_allowed && !t->split_device &&
                       !cs->cb.dataReduction;
     for (uint i = 0; i < d.taskMemCount; ++i) t->has_deferred |= t->deferred(i);
-    // held-back writes and redistribution work on the host copy: such a task stages its GPU
-    // subtasks the PLAIN way
-    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN && !t->has_deferred &&
-                      !cs->cb.dataRedistribution;
+    // held-back writes work on the host copy: such a task stages its GPU subtasks the PLAIN way
+    // (a redistributing task runs resident and is redistributed in HBM)
+    t->gpu_resident = t->gpu_allowed && !t->cpu_allowed && t->hip_mode != PM_HIP_MODE_PLAIN && !t->has_deferred;
     if (d.timeOutInSecs >= 0 && d.timeOutInSecs != INT_MAX) t->deadline = now_secs() + d.timeOutInSecs;
     for (uint i = 0; i < t->spaces.size(); ++i) {
         AddressSpace* as = t->spaces[i];

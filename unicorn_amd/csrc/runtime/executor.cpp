@@ -768,6 +768,7 @@ void hipchk(int rc, const char* what) {
 // One GPU subtask in flight.
 struct GpuSlot {
     Task* task = nullptr;
+    uint64_t seq = 0;              // staging order (the OOM back-off retires the oldest first)
     std::unique_ptr<Subtask> st;
     void* stream = nullptr;
     void* event = nullptr;
@@ -852,12 +853,25 @@ void cache_shrink(size_t need) {
     }
 }
 
+// Out-of-memory back-off of the per-subtask path (reference pmExecutionStub.cpp:2403-2415: on a
+// failed device allocation the stub waits for its oldest in-flight subtask and retries): while
+// staging a subtask, the stub thread installs a hook that retires its oldest other in-flight
+// subtask, which frees that subtask's buffers; it returns false when none is left.
+static thread_local std::function<bool()>* tl_oom_backoff = nullptr;
+
 void* slot_device_alloc(GpuSlot& s, size_t bytes) {
     if (!bytes) return nullptr;
-    cache_shrink(bytes);
-    void* p = Memory::get().device_alloc(bytes);
-    s.dev_bufs.push_back({p, bytes});
-    return p;
+    for (;;) {
+        cache_shrink(bytes);
+        try {
+            void* p = Memory::get().device_alloc(bytes);
+            s.dev_bufs.push_back({p, bytes});
+            return p;
+        } catch (Error& e) {
+            if (e.status != pmMemoryError || !tl_oom_backoff || !(*tl_oom_backoff)()) throw;
+            stat_add(&pmHipStubStats::oom_waits, 1);
+        }
+    }
 }
 
 uint8_t* slot_pinned(GpuSlot& s, size_t bytes) {
@@ -1117,17 +1131,32 @@ pmSubtaskInfo stage_resident(Task* t, GpuSlot& s, void* stream) {
 }
 
 // After a RESIDENT/BATCHED subtask with COMPACT writable views: scatter written records back
-// into the mirror (device to device).
+// into the mirror (device to device), all records of a view in one copy kernel (a copy call per
+// record costs a few microseconds of launch each: 2048 per 2048-row block).
 void scatter_resident(Task* t, GpuSlot& s, void* stream) {
     for (auto& d : s.dev_scatters) {
         MemSubscription* ms = nullptr;
         for (uint i = 0; i < t->spaces.size(); ++i) if (t->spaces[i] == d.as) ms = &s.st->mem[i];
         if (!ms) continue;
-        for (auto& kv : ms->write().map()) {
-            size_t co = d.cl.compact_offset(kv.first);
-            hipchk(uh_memcpy_d2d(d.as->dev + kv.first, (uint8_t*)d.buf + co, kv.second, stream), "device scatter");
-            stat_add(&pmHipStubStats::d2d_bytes, kv.second);
+        const auto& recs = ms->write().map();
+        const size_t n = recs.size();
+        if (!n) continue;
+        std::vector<uint64_t> tab(3 * n);
+        uint64_t maxlen = 0, bytes = 0;
+        size_t r = 0;
+        for (auto& kv : recs) {
+            tab[r] = d.cl.compact_offset(kv.first);   // source: packed position in the view
+            tab[n + r] = kv.first;                    // destination: offset in the address space
+            tab[2 * n + r] = kv.second;
+            maxlen = std::max<uint64_t>(maxlen, kv.second);
+            bytes += kv.second;
+            ++r;
         }
+        uint64_t* dtab = (uint64_t*)slot_device_alloc(s, tab.size() * 8);
+        upload_small(dtab, tab.data(), tab.size() * 8, "record table upload");
+        hipchk(uh_copy_records(d.as->dev, d.buf, dtab, dtab + n, dtab + 2 * n, (uint32_t)n, maxlen, stream),
+               "device scatter");
+        stat_add(&pmHipStubStats::d2d_bytes, bytes);
     }
 }
 
@@ -1370,6 +1399,7 @@ void Runtime::gpu_loop() {
         admit_queued();
         drain_ready();      // tasks released by a cancellation or time-out under the queue lock
         const int depth = std::max(1, std::min(16, pipeline_depth.load()));
+        static thread_local uint64_t slot_seq = 0;
         Task* t = nullptr;
         ulong first = 0, count = 0;
         pmSplitInfo split;
@@ -1544,6 +1574,28 @@ void Runtime::gpu_loop() {
             }
             next_slot = (si_idx + 1) % depth;
             GpuSlot& s = gs.slots[si_idx];
+            s.seq = ++slot_seq;
+            std::function<bool()> backoff = [&]() {
+                int oldest = -1;
+                for (int k = 0; k < depth; ++k)
+                    if (k != si_idx && gs.slots[k].busy && (oldest < 0 || gs.slots[k].seq < gs.slots[oldest].seq))
+                        oldest = k;
+                if (oldest < 0) return false;
+                GpuSlot& o = gs.slots[oldest];
+                uh_event_sync(o.event);
+                const bool op = o.plain;
+                if (!op && !o.dev_scatters.empty()) {
+                    try { scatter_resident(o.task, o, o.stream); hipchk(uh_stream_sync(o.stream), "scatter"); }
+                    catch (Error& e) { fail_task(o.task, e); }
+                }
+                retire_slot(this, o, op);
+                inflight--;
+                return true;
+            };
+            struct BackoffScope {
+                explicit BackoffScope(std::function<bool()>* f) { tl_oom_backoff = f; }
+                ~BackoffScope() { tl_oom_backoff = nullptr; }
+            } backoff_scope(&backoff);
             s.task = t;
             s.st.reset(new Subtask());
             s.st->id = sid;

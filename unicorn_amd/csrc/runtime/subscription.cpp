@@ -14,10 +14,13 @@
 //     (GetCompactedSubscription, 1978-2031); readPtr / writePtr point at the packed position of
 //     the first read / write record.
 #include "pm_internal.h"
+#include "unicorn_hip.h"
 
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <memory>
 
 namespace pm {
 namespace rt {
@@ -183,7 +186,9 @@ void run_redistribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
     ctx.redistribution = true;
     DistributionContext* saved = current_context();
     current_context() = &ctx;
-    // the subtask's views of the committed host copy
+    // the subtask's views of the committed host copy; a task that ran resident on the HIP stub
+    // keeps its bytes in HBM, and its callback sees the lengths only (the reference's
+    // redistribution callbacks read their scratch buffers, not the data)
     pmSubtaskInfo si;
     si.subtaskId = st.id;
     si.memCount = (uint)t->spaces.size();
@@ -192,15 +197,70 @@ void run_redistribution(Task* t, Subtask& st, pmDeviceInfo* dev) {
         pmSubscriptionInfo u = st.mem[i].unified();
         if (!u.length) continue;
         pmMemInfo& mi = si.memInfo[i];
-        mi.ptr = t->spaces[i]->host + u.offset;
         mi.length = u.length;
         mi.visibilityType = SUBSCRIPTION_NATURAL;
+        if (t->gpu_resident) continue;
+        mi.ptr = t->spaces[i]->host + u.offset;
         if (st.mem[i].has_read) mi.readPtr = t->spaces[i]->host + st.mem[i].cread.offset;
         if (st.mem[i].has_write) mi.writePtr = t->spaces[i]->host + st.mem[i].cwrite.offset;
     }
     pmStatus s = cb(t->info(), *dev, si);
     current_context() = saved;
     if (s != pmSuccess) raise(s, "dataRedistribution callback failed");
+}
+
+// Runs body(0 .. n-1) on up to `threads` threads (the calling one included), in contiguous ranges.
+template <typename F>
+static void parallel_for(size_t n, unsigned threads, F body) {
+    if (threads <= 1 || n <= 1) {
+        for (size_t k = 0; k < n; ++k) body(k);
+        return;
+    }
+    threads = (unsigned)std::min<size_t>(threads, n);
+    std::vector<std::thread> th;
+    const size_t per = (n + threads - 1) / threads;
+    for (unsigned j = 1; j < threads; ++j)
+        th.emplace_back([&, j] {
+            for (size_t k = j * per; k < std::min(n, (j + 1) * per); ++k) body(k);
+        });
+    for (size_t k = 0; k < std::min(n, per); ++k) body(k);
+    for (auto& x : th) x.join();
+}
+
+// The same concatenation when the address space's current copy is its HBM mirror: the records
+// are copied by one kernel into a zeroed device buffer, which then replaces the mirror's bytes;
+// the host copy stays stale until something reads it.
+static void redistribute_on_device(AddressSpace* as, const std::vector<Task::RedistRecord>& recs,
+                                   const std::vector<size_t>& dst, size_t total) {
+    const size_t n = recs.size();
+    const size_t words = (as->length + 3) / 4;
+    std::vector<uint64_t> tab(3 * n);
+    uint64_t maxlen = 0;
+    for (size_t k = 0; k < n; ++k) {
+        tab[k] = recs[k].offset;
+        tab[n + k] = dst[k];
+        tab[2 * n + k] = recs[k].length;
+        maxlen = std::max<uint64_t>(maxlen, recs[k].length);
+    }
+    Memory& M = Memory::get();
+    void* tmp = M.device_alloc(words * 4);
+    void* dtab = n ? M.device_alloc(tab.size() * 8) : nullptr;
+    void* stream = nullptr;
+    int rc = uh_stream_create(&stream);
+    if (!rc && n) rc = uh_memcpy_h2d(dtab, tab.data(), tab.size() * 8, stream);
+    if (!rc) rc = uh_fill_u32(tmp, 0, words, stream);
+    if (!rc && n)
+        rc = uh_copy_records(tmp, as->dev, (const uint64_t*)dtab, (const uint64_t*)dtab + n, (const uint64_t*)dtab + 2 * n,
+                             (uint32_t)n, maxlen, stream);
+    if (!rc) rc = uh_memcpy_d2d(as->dev, tmp, as->length, stream);
+    if (!rc) rc = uh_stream_sync(stream);
+    const std::string err = rc ? std::string(uh_last_error()) : std::string();
+    if (stream) uh_stream_destroy(stream);
+    M.device_free(tmp, words * 4);
+    if (dtab) M.device_free(dtab, tab.size() * 8);
+    if (rc) raise(pmGraphicsCardError, "redistribution in HBM: " + err);
+    (void)total;
+    M.mark_device_written(as);
 }
 
 void commit_redistribution(Task* t) {
@@ -214,18 +274,37 @@ void commit_redistribution(Task* t) {
             return a.seq < b.seq;
         });
         AddressSpace* as = t->spaces[i];
-        std::vector<uint8_t> out(as->length, 0);
+        // destination of every record (its position in the concatenation) and the metadata
+        std::vector<size_t> dst(recs.size());
         size_t pos = 0;
         std::vector<pmRedistributionMetadata>& meta = t->redist_meta[i];
         meta.clear();
-        for (const auto& r : recs) {
+        for (size_t k = 0; k < recs.size(); ++k) {
+            const auto& r = recs[k];
             if (pos + r.length > as->length) raise(pmInvalidOffset, "redistributed ranges exceed the address space");
-            memcpy(out.data() + pos, as->host + r.offset, r.length);
+            dst[k] = pos;
             pos += r.length;
             if (meta.empty() || meta.back().order != r.order) meta.push_back(pmRedistributionMetadata(r.order, 0));
             meta.back().count += (uint)r.length;
         }
-        memcpy(as->host, out.data(), as->length);
+        if (as->dev && as->dev_valid && !as->host_valid) {
+            redistribute_on_device(as, recs, dst, pos);
+            continue;
+        }
+        // gather into a buffer of the concatenation's size, then write it back and clear the rest
+        // (the reference builds a fresh address space of the same length); large spaces are
+        // copied by several threads
+        std::unique_ptr<uint8_t[]> out(new (std::nothrow) uint8_t[pos ? pos : 1]);
+        if (!out) raise(pmMemoryError, "redistribution buffer");
+        const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>(16, (as->length >> 26) + 1));
+        parallel_for(recs.size(), nt, [&](size_t k) { memcpy(out.get() + dst[k], as->host + recs[k].offset, recs[k].length); });
+        const size_t piece = ((as->length + nt - 1) / nt + 4095) & ~(size_t)4095;
+        parallel_for(nt, nt, [&](size_t j) {
+            const size_t a = j * piece, b = std::min(as->length, a + piece);
+            if (a >= b) return;
+            if (a < pos) memcpy(as->host + a, out.get() + a, std::min(b, pos) - a);
+            if (b > pos) memset(as->host + std::max(a, pos), 0, b - std::max(a, pos));
+        });
         Memory::get().mark_host_written(as);
     }
 }

@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
             "uh_batch_destroy": (i32, [vp]),
             "uh_batch_flush": (i32, [vp, i32]),
             "uh_launch_counters": (i32, [C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+            "uh_imagefilter_engine": (i32, [i32, C.POINTER(i32)]),
             "uh_transpose32": (i32, [vp, u64, vp, u64, u32, u32, vp]),
             "uh_scan_i32": (i32, [vp, vp, u64, vp]),
             "uh_spmm_coo_f32": (i32, [vp, u32, C.c_int32, u32, vp, u64, vp, u64, u32, vp]),
