@@ -22,6 +22,7 @@
 // The per-item arguments and weight vectors are scalar loads (SGPRs); image stores use
 // global-space pointers, so LDS waits never wait for HBM traffic.
 // Radii 1 and 2 take this path; other radii use a direct per-pixel path.
+#include <type_traits>
 #include "uh_internal.h"
 
 #include <cstddef>
@@ -347,9 +348,9 @@ __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F7
 // raw window in LDS (no de-interleave). A block of 32 outputs needs the 48 input bytes from 8
 // before it, covered by two K steps of 32; the three blocks of a wave's 96 bytes (32 pixels) share
 // their middle steps, so a filter row costs 4 B reads and 6 MFMAs per 32 rows x 96 bytes. The band
-// depends on the block's byte phase mod 3 (blocks at 0, 32, 64 have phases 0, 2, 1): A holds
-// (2R+1) rows x 3 phases x 2 steps operands, built once per work item from the packed weights
-// with one v_perm per dword (byte selectors from a table fixed at compile time).
+// depends on the block's byte phase mod 3 (blocks at 0, 32, 64 have phases 0, 2, 1): the A
+// operand of a (row, phase, step) is built from the packed weight row with one v_perm per dword
+// (byte selectors from a table fixed at compile time).
 // Input bytes are biased (XOR 0x80) into int8 on the way to the MFMA; the accumulators start at
 // 128 * sum(w), which undoes the bias, as in the dot4 engine. Tiles are 128 pixels x 32 rows, one
 // wave per 32 pixels; the raw window of the next tile streams into the other of two LDS buffers
@@ -372,8 +373,16 @@ struct FmGeom {
 // whole 1 KB wave destinations (64 lanes x 16 bytes) per buffer
 static constexpr int FM_BUF = (FmGeom<2>::CHUNKS + 63) / 64 * 1024;
 
+// Output staging: a tile's bytes (32 rows x 384) at a 400-byte row pitch, written by the waves
+// from their accumulators and read back as whole 16-byte row segments, so that a store
+// instruction covers 1 KB of consecutive rows (3 full lines per row) instead of 8 bytes in each
+// of 32 rows. Two areas: one is read (and stored) while the next tile's products fill the other.
+static constexpr int FM_OUT_PITCH = 400;
+static constexpr int FM_OUT_BYTES = FM_H * FM_OUT_PITCH;
+
 struct FmLds {
     uint8_t buf[2][FM_BUF];
+    uint8_t out[2][FM_OUT_BYTES];
 };
 
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
@@ -411,6 +420,8 @@ constexpr FmSel fm_make_sel() {
 }
 __device__ const FmSel g_fm_sel = fm_make_sel();
 
+// This lane's A operands, (2R + 1) rows x 3 blocks x 2 K steps, built once per work item with one
+// v_perm per dword from the packed weight row of each filter row.
 template <int R>
 struct FmA {
     i32x4 op[2 * R + 1][3][2];
@@ -441,30 +452,47 @@ __device__ __forceinline__ void fm_origin(const FilterArgs& a, uint32_t tile, in
     y0 = a.row0 + (int)(tile % tiles_y) * FM_H;
 }
 
-// Raw window of a tile: staged row lr holds the FM_ROW bytes from the 16-byte boundary at or
-// below pixel x0 - 4 of image row y0 - R + lr (rows clamped to the view), as 16-byte
-// direct-to-LDS buffer loads (see filter_dma).
+// Raw window of a tile: staged row lr holds the FM_ROW_CHUNKS 16-byte chunks from the 16-byte
+// boundary at or below pixel x0 - 4 of image row y0 - R + lr (rows clamped to the view), at row
+// pitch FM_ROW. The chunks come through a buffer resource spanning the view (chunks past its end
+// read as zero, a chunk wholly before it is redirected to offset 0; see filter_dma) into
+// registers, and go to LDS after the current tile's products: a wave waits for them only there.
+// (Direct-to-LDS loads would need no registers, but the compiler then waits for them before the
+// first LDS read that follows, i.e. before the current tile's products.)
+static constexpr int FM_FETCH = (FmGeom<2>::CHUNKS + FT_THREADS - 1) / FT_THREADS;   // chunk slots per thread
+
+struct FmFetch {
+    u32x4 v[FM_FETCH];
+};
+
 template <int R>
-__device__ __forceinline__ void fm_dma(const FilterArgs& a, int x0, int y0, uint8_t* buf) {
+__device__ __forceinline__ void fm_fetch(const FilterArgs& a, int x0, int y0, FmFetch& f) {
     constexpr int CHUNKS = FmGeom<R>::CHUNKS;
     const FilterView v = filter_view(a);
     const int mis = (int)((uintptr_t)a.in & 15u);
     const int extent = (int)__umul24((uint32_t)(a.view_last_row - v.first_row), (uint32_t)a.in_stride) + a.pad * 3;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(a.in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
-        const int c0 = i * FT_THREADS + wave * 64;
-        if (c0 >= CHUNKS) break;
-        const int c = c0 + lane;
+    for (int i = 0; i < FM_FETCH; ++i) {
+        const int c = i * FT_THREADS + (int)threadIdx.x;
         const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
         if (c < CHUNKS && ch < FM_ROW_CHUNKS) {
             const int st = filter_row_start<R>(a, v, x0, y0, lr) + mis;
             const int off = (st & ~15) + ch * 16;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rs, (__attribute__((address_space(3))) void*)(buf + c0 * 16), 16, (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
+            f.v[i] = __builtin_amdgcn_raw_buffer_load_b128(rs, off < 0 ? 0 : off, 0, 0);
         }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void fm_put(const FmFetch& f, uint8_t* buf) {
+    constexpr int CHUNKS = FmGeom<R>::CHUNKS;
+#pragma unroll
+    for (int i = 0; i < FM_FETCH; ++i) {
+        const int c = i * FT_THREADS + (int)threadIdx.x;
+        const int ch = c % FM_PITCH_CHUNKS;
+        if (c < CHUNKS && ch < FM_ROW_CHUNKS) *(u32x4*)(buf + c * 16) = f.v[i];
     }
 }
 
@@ -525,20 +553,23 @@ __device__ __forceinline__ i32x4 fm_load_b(const uint8_t* base, int q) {
     return r;
 }
 
-// Products of one tile for this wave's 96 bytes, then the stores.
-template <int R, int MODE, bool AL8>
-__device__ __forceinline__ void fm_tile(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A, int x0, int y0,
-                                        const uint8_t* buf) {
+// Products of one tile for this wave's 96 bytes (three 32-byte blocks x 32 rows).
+struct FmAcc {
+    i32x16 v[3];
+};
+
+template <int R, bool AL8>
+__device__ __forceinline__ void fm_compute(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A, int x0, int y0,
+                                           const uint8_t* buf, FmAcc& acc) {
     const FilterView v = filter_view(a);
     const int mis = (int)((uintptr_t)a.in & 15u);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 31, h = lane >> 5;
     const int bias = 128 * w.wsum;
-    i32x16 acc[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[b][r] = bias;
+        for (int r = 0; r < 16; ++r) acc.v[b][r] = bias;
 #pragma unroll
     for (int dy = 0; dy < 2 * R + 1; ++dy) {
         // tile byte 96 wave + 32 q - 8 + 16 h of staged row n + dy (pixel x0 - 4 is 12 bytes before tile byte 0)
@@ -547,20 +578,24 @@ __device__ __forceinline__ void fm_tile(const FilterArgs& a, const FilterWeights
         const i32x4 b1 = fm_load_b<AL8>(buf, q0 + 32);
         const i32x4 b2 = fm_load_b<AL8>(buf, q0 + 64);
         const i32x4 b3 = fm_load_b<AL8>(buf, q0 + 96);
-        acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][0], b0, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][0], b1, acc[1], 0, 0, 0);
-        acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][0], b2, acc[2], 0, 0, 0);
-        acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][1], b1, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][1], b2, acc[1], 0, 0, 0);
-        acc[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][1], b3, acc[2], 0, 0, 0);
+        acc.v[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][0], b0, acc.v[0], 0, 0, 0);
+        acc.v[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][0], b1, acc.v[1], 0, 0, 0);
+        acc.v[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][0], b2, acc.v[2], 0, 0, 0);
+        acc.v[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][1], b1, acc.v[0], 0, 0, 0);
+        acc.v[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][1], b2, acc.v[1], 0, 0, 0);
+        acc.v[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][1], b3, acc.v[2], 0, 0, 0);
     }
-    // D[m][n]: column n = lane % 32 (tile row), row m = 8 (r / 4) + 4 (lane / 32) + r % 4 (byte of the block)
-    const int y = y0 + n;
-    if (y >= a.row0 + a.rows) return;
-    const int nbytes = a.cols * 3;
-    const int jt = (x0 - a.col0) * 3 + 96 * wave + 4 * h;    // tile-relative byte of this lane's first dword
-    UH_GLOBAL uint8_t* row = as_global(a.out) + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride);
+}
+
+// A tile's products as bytes into an output staging area. D[m][n]: column n = lane % 32 (tile
+// row), row m = 8 (r / 4) + 4 (lane / 32) + r % 4 (byte of the block): each lane holds 4
+// consecutive bytes at 12 places of its row.
+template <int MODE>
+__device__ __forceinline__ void fm_stage_out(const FilterArgs& a, const FmAcc& acc, uint8_t* out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 31, h = lane >> 5;
     const int shift = a.shift;
+    uint8_t* row = out + n * FM_OUT_PITCH + 96 * wave + 4 * h;
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -568,53 +603,83 @@ __device__ __forceinline__ void fm_tile(const FilterArgs& a, const FilterWeights
             uint32_t o[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int s = acc[b][4 * i + e];
+                const int s = acc.v[b][4 * i + e];
                 o[e] = MODE == 0 ? (uint32_t)s : (uint32_t)clampi(s >> shift, 0, 255);
             }
-            const int j = jt + 32 * b + 8 * i;
-            UH_GLOBAL uint8_t* dst = row + j;
-            if (j + 4 <= nbytes && (((uintptr_t)dst) & 3u) == 0) {
-                *(UH_GLOBAL uint32_t*)dst = pack4(o[0], o[1], o[2], o[3]);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (j + e < nbytes) dst[e] = (uint8_t)o[e];
-            }
+            *(uint32_t*)(row + 32 * b + 8 * i) = pack4(o[0], o[1], o[2], o[3]);
         }
 }
 
-// Tiles l0 .. l1-1 of one item (all tiles in one item share its arguments and weights).
+// Stores of a staged tile: thread t moves row t / 8, bytes 16 (t % 8) + 128 k, k < 3, so a
+// wave's store instruction writes 8 rows x 128 consecutive bytes.
+__device__ __forceinline__ void fm_store_out(const FilterArgs& a, int x0, int y0, const uint8_t* out) {
+    const int r = (int)threadIdx.x >> 3, s16 = 16 * ((int)threadIdx.x & 7);
+    const int y = y0 + r;
+    if (y >= a.row0 + a.rows) return;
+    const int nbytes = a.cols * 3;
+    const int j0 = (x0 - a.col0) * 3 + s16;
+    const uint8_t* src = out + r * FM_OUT_PITCH + s16;
+    UH_GLOBAL uint8_t* dst = as_global(a.out) + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride) + j0;
+    const bool fast = j0 + 256 + 16 <= nbytes && (((uintptr_t)dst) & 15u) == 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const u32x4 v = *(const u32x4*)(src + 128 * k);
+        if (fast) {
+            *(UH_GLOBAL u32x4*)(dst + 128 * k) = v;
+        } else {
+            const int j = j0 + 128 * k;
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                if (j + e < nbytes) dst[128 * k + e] = (uint8_t)(w4[e >> 2] >> (8 * (e & 3)));
+        }
+    }
+}
+
+// Tiles l0 .. l1-1 of one item (all tiles in one item share its arguments and weights). Per tile
+// t (its window in buf[c], written and made visible at the end of tile t-1): fetch tile t+1's
+// window into registers, store tile t-1's bytes from out[c ^ 1], compute tile t, stage its bytes
+// in out[c], write tile t+1's window into the other buffer and meet at a barrier. The fetch and
+// the stores have a whole tile's products to land before anything waits for them.
 template <int R, int MODE>
 __device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const FilterWeights& w, uint32_t l0, uint32_t l1) {
     if (l0 >= l1) return;
     FmA<R> A;
     fm_build_a<R>(w, A);
-    __syncthreads();              // a previous item's last tile may still be reading a buffer
     const FilterView v = filter_view(a);
-    // every staged row starts at the same byte offset mod 8 when the stride is a multiple of 8
     const int mis = (int)((uintptr_t)a.in & 15u);
     int x0, y0;
     fm_origin(a, l0, x0, y0);
-    fm_dma<R>(a, x0, y0, sm.buf[0]);
-    int cur = 0;
+    FmFetch f;
+    fm_fetch<R>(a, x0, y0, f);
+    __syncthreads();              // a previous item's last tile may still be reading the buffers
+    fm_put<R>(f, sm.buf[0]);
+    __syncthreads();
+    int px = 0, py = 0;           // origin of the tile staged in sm.out[c ^ 1]
+    FmAcc acc;
     for (uint32_t t = l0; t < l1; ++t) {
-        wait_vm_all();
-        __syncthreads();          // tile t's window is in buf[cur]; every wave has left buf[cur ^ 1]
+        const uint32_t c = (t - l0) & 1u;
         int x1 = 0, y1 = 0;
-        if (t + 1 < l1) {
+        const bool more = t + 1 < l1;
+        if (more) {
             fm_origin(a, t + 1, x1, y1);
-            fm_dma<R>(a, x1, y1, sm.buf[cur ^ 1]);
+            fm_fetch<R>(a, x1, y1, f);
         }
+        if (t > l0) fm_store_out(a, px, py, sm.out[c ^ 1u]);
         if (fm_needs_patch<R>(a, v, x0)) {
-            fm_patch<R>(a, x0, y0, sm.buf[cur]);
+            fm_patch<R>(a, x0, y0, sm.buf[c]);
             __syncthreads();
         }
         const bool al8 = (a.in_stride & 7u) == 0 && ((fm_row_byte<R>(a, v, x0, y0, 0, mis) + 4) & 7) == 0;
-        if (al8) fm_tile<R, MODE, true>(a, w, A, x0, y0, sm.buf[cur]);
-        else fm_tile<R, MODE, false>(a, w, A, x0, y0, sm.buf[cur]);
+        if (al8) fm_compute<R, true>(a, w, A, x0, y0, sm.buf[c], acc);
+        else fm_compute<R, false>(a, w, A, x0, y0, sm.buf[c], acc);
+        fm_stage_out<MODE>(a, acc, sm.out[c]);
+        if (more) fm_put<R>(f, sm.buf[c ^ 1u]);
+        __syncthreads();          // tile t+1's window and tile t's bytes are visible; every wave has left buf[c]
+        px = x0; py = y0;
         x0 = x1; y0 = y1;
-        cur ^= 1;
     }
+    fm_store_out(a, px, py, sm.out[(l1 - 1 - l0) & 1u]);
 }
 
 // Direct path for any radius <= 15: one output pixel per thread.
@@ -662,10 +727,21 @@ __host__ __device__ constexpr int filter_variant(int radius, int mode, int mfma)
     return (radius == 1 || radius == 2) ? 1 + 2 * (radius - 1) + mode + (mfma ? 4 : 0) : 0;
 }
 
-union FilterShared {
+// Shared memory of a variant: only variant 0 carries both engines' buffers, so the specialised
+// variants keep the LDS footprint (and workgroups per CU) of their own engine.
+union FilterSharedAll {
     FilterLds d;
     FmLds m;
 };
+struct FilterSharedDot4 {
+    FilterLds d;
+};
+struct FilterSharedMfma {
+    FmLds m;
+};
+template <int V>
+using FilterShared = typename std::conditional<V == 0, FilterSharedAll,
+                     typename std::conditional<(V >= 5), FilterSharedMfma, FilterSharedDot4>::type>::type;
 
 template <int V>
 __device__ __forceinline__ bool filter_staged(const FilterArgs& a) {
@@ -774,7 +850,7 @@ __device__ __forceinline__ void filter_run(FilterLds& sm, const FilterArgs& a, c
 // Plain launch: workgroup b computes tiles 2b and 2b + 1 of one item.
 // Tiles l0 .. l1-1 of one item with the engine its variant (or, for variant 0, the item) names.
 template <int V>
-__device__ __forceinline__ void filter_item_run(FilterShared& sm, const FilterArgs& a, const FilterWeights& w,
+__device__ __forceinline__ void filter_item_run(FilterShared<V>& sm, const FilterArgs& a, const FilterWeights& w,
                                                 const int8_t* table, uint32_t l0, uint32_t l1) {
     if constexpr (V >= 5) {
         fm_run<(V - 3) / 2, (V - 5) % 2>(sm.m, a, w, l0, l1);
@@ -796,16 +872,16 @@ __device__ __forceinline__ void filter_item_run(FilterShared& sm, const FilterAr
 }
 
 // Waves per SIMD the variant is compiled for: the MFMA engine (also reachable from variant 0)
-// holds its A operands in registers.
+// keeps its A operands in registers.
 template <int V>
 struct FilterOcc {
-    static constexpr int value = (V >= 5 || V == 0) ? 2 : FT_WAVES_PER_EU;
+    static constexpr int value = (V == 0 || V >= 5) ? 2 : FT_WAVES_PER_EU;
 };
 
 template <int V>
 __global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FilterOcc<V>::value)))
 k_imagefilter(FilterItem it, uint32_t tiles) {
-    __shared__ FilterShared sm;
+    __shared__ FilterShared<V> sm;
     const uint32_t l0 = blockIdx.x * 2u;
     filter_item_run<V>(sm, head_of(it.a), it.w, it.a.weights, l0, l0 + 2u < tiles ? l0 + 2u : tiles);
 }
@@ -821,7 +897,7 @@ static constexpr uint32_t FT_PREFIX_LDS = 1024;   // items whose tile prefix is 
 template <int V>
 __global__ void __launch_bounds__(FT_THREADS) __attribute__((amdgpu_waves_per_eu(FilterOcc<V>::value)))
 k_imagefilter_persistent(const uint8_t* queue) {
-    __shared__ FilterShared sm;
+    __shared__ FilterShared<V> sm;
     __shared__ uint32_t s_prefix[FT_PREFIX_LDS + 1];
     const QueueHeader hd = uniform_load((const QueueHeader*)queue);   // n_items, total_tiles, offsets
     const FilterItem* items = (const FilterItem*)(queue + hd.items_offset);
@@ -931,15 +1007,15 @@ static int filter_validate(const uh_imagefilter_args* a, int* view_cols) {
     return UH_OK;
 }
 
-// Engine of the staged radii: 0 = dot4 (default), 1 = MFMA; UH_FILTER_ENGINE=mfma selects the
-// MFMA engine at start.
+// Engine of the staged radii: 1 = MFMA (default), 0 = dot4; UH_FILTER_ENGINE=dot4 selects the
+// dot4 engine at start.
 static std::atomic<int> g_filter_engine{-1};
 
 static int filter_engine() {
     int e = g_filter_engine.load();
     if (e < 0) {
         const char* s = getenv("UH_FILTER_ENGINE");
-        e = (s && strcmp(s, "mfma") == 0) ? 1 : 0;
+        e = (s && strcmp(s, "dot4") == 0) ? 0 : 1;
         g_filter_engine.store(e);
     }
     return e;
