@@ -358,7 +358,7 @@ __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F7
 // the edge pixel (the reference clamps rows and columns to the image) before the products.
 static constexpr int FM_W = 128;                  // tile width (pixels)
 static constexpr int FM_H = 32;                   // tile height (rows)
-static constexpr int FM_ROW_CHUNKS = 28;          // staged 16-byte chunks per row (from pixel x0 - 4)
+static constexpr int FM_ROW_CHUNKS = 28;          // staged 16-byte chunks per row (from pixel x0 - 8)
 // LDS row pitch: one unused chunk per row. The 32 lanes of a B read take the same column of 32
 // rows; with a 28-chunk pitch (112 dwords) the rows fell on 2-4 bank pairs, with 29 chunks
 // (58 qwords) on 16.
@@ -370,18 +370,22 @@ struct FmGeom {
     static constexpr int LROWS = FM_H + 2 * R;
     static constexpr int CHUNKS = LROWS * FM_PITCH_CHUNKS;   // chunk slots, the pad slots included
 };
-// whole 1 KB wave destinations (64 lanes x 16 bytes) per buffer
-static constexpr int FM_BUF = (FmGeom<2>::CHUNKS + 63) / 64 * 1024;
+// bytes of a window buffer (direct-to-LDS loads write only the lanes of chunks that exist)
+static constexpr int FM_BUF = FmGeom<2>::CHUNKS * 16;
+// Window buffers in the ring: tile t's window is loaded while tiles t - 1 and t - 2 are computed.
+static constexpr int FM_RING = 3;
 
 // Output staging: a tile's bytes (32 rows x 384) at a 400-byte row pitch, written by the waves
 // from their accumulators and read back as whole 16-byte row segments, so that a store
-// instruction covers 1 KB of consecutive rows (3 full lines per row) instead of 8 bytes in each
-// of 32 rows. Two areas: one is read (and stored) while the next tile's products fill the other.
+// instruction covers 8 rows x 128 consecutive bytes instead of 8 bytes in each of 32 rows. Rows
+// 16-31 hold the two 8-byte halves of every 16-byte chunk swapped: a write instruction (one dword
+// of 32 rows, rows r and r + 16 in the same four banks) then touches 64 distinct banks. Two areas:
+// one is read (and stored) while the next tile's products fill the other.
 static constexpr int FM_OUT_PITCH = 400;
 static constexpr int FM_OUT_BYTES = FM_H * FM_OUT_PITCH;
 
 struct FmLds {
-    uint8_t buf[2][FM_BUF];
+    uint8_t buf[FM_RING][FM_BUF];
     uint8_t out[2][FM_OUT_BYTES];
 };
 
@@ -391,8 +395,9 @@ typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 // Byte selectors of the A operands. Lane l holds output byte m = l % 32 of a block and the K bytes
 // k = 16 (l / 32) + 0..15 of a step (the B operand uses the same K numbering for its 16 bytes).
 // Byte k of step s of a block of phase p is tap dx of output m when
-//   k + 32 s - 8 = m + 3 dx + 2 - 2 c,  c = (p + m) mod 3
-// (input byte relative to the block's first output byte, the window starting 8 bytes before it).
+//   k + 32 s - 16 = m + 3 dx + 2 - 2 c,  c = (p + m) mod 3
+// (input byte relative to the block's first output byte, the window starting 16 bytes before it,
+// so that a window is 16-byte aligned in LDS whenever the tile's first byte is).
 // Selector byte = dx + R (a byte of the packed weight row: bytes 0-3 of lo, byte 4 = byte 0 of
 // hi), or 0x0c (zero).
 struct FmSel {
@@ -409,7 +414,7 @@ constexpr FmSel fm_make_sel() {
                         const int m = l % 32, h = l / 32, p = (32 * b) % 3, c = (p + m) % 3;
                         for (int bb = 0; bb < 4; ++bb) {
                             const int k = 16 * h + 4 * d + bb;
-                            const int u = k + 32 * st - 8 - m - 2 + 2 * c;    // 3 dx
+                            const int u = k + 32 * st - 16 - m - 2 + 2 * c;   // 3 dx
                             uint32_t byte = 0x0c;
                             if (u >= -3 * r && u <= 3 * r && (u + 30) % 3 == 0) byte = (uint32_t)(u / 3 + r);
                             sel |= byte << (8 * bb);
@@ -452,54 +457,62 @@ __device__ __forceinline__ void fm_origin(const FilterArgs& a, uint32_t tile, in
     y0 = a.row0 + (int)(tile % tiles_y) * FM_H;
 }
 
-// Raw window of a tile: staged row lr holds the FM_ROW_CHUNKS 16-byte chunks from the 16-byte
-// boundary at or below pixel x0 - 4 of image row y0 - R + lr (rows clamped to the view), at row
-// pitch FM_ROW. The chunks come through a buffer resource spanning the view (chunks past its end
-// read as zero, a chunk wholly before it is redirected to offset 0; see filter_dma) into
-// registers, and go to LDS after the current tile's products: a wave waits for them only there.
-// (Direct-to-LDS loads would need no registers, but the compiler then waits for them before the
-// first LDS read that follows, i.e. before the current tile's products.)
-static constexpr int FM_FETCH = (FmGeom<2>::CHUNKS + FT_THREADS - 1) / FT_THREADS;   // chunk slots per thread
-
-struct FmFetch {
-    u32x4 v[FM_FETCH];
-};
-
+// Offset from the view's origin of pixel x0 - 8 of the window's staged row lr (image row
+// y0 - R + lr clamped to the view).
 template <int R>
-__device__ __forceinline__ void fm_fetch(const FilterArgs& a, int x0, int y0, FmFetch& f) {
+__device__ __forceinline__ int fm_row_start(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr) {
+    const int m = clampi(y0 - R + lr, v.first_row, v.last_row);
+    return (int)__umul24((uint32_t)(a.view_last_row - m), (uint32_t)a.in_stride) + (x0 - 8 - v.first_col) * 3;
+}
+
+// Wave index as a scalar (the offsets built from it stay out of vector registers).
+__device__ __forceinline__ int fm_wave() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+// Raw window of a tile: staged row lr holds the FM_ROW_CHUNKS 16-byte chunks from the 16-byte
+// boundary at or below pixel x0 - 8 of image row y0 - R + lr (rows clamped to the view), at row
+// pitch FM_ROW. The chunks come through a buffer resource spanning the view (chunks past its end
+// read as zero, a chunk wholly before it is redirected to offset 0; see filter_dma) straight into
+// LDS: wave w's load i fills chunk slots 256 i + 64 w + lane (pad slots masked off), so no
+// register holds them and the ring can hold two tiles in flight.
+template <int R>
+__device__ __forceinline__ void fm_dma(const FilterArgs& a, const uint8_t* __restrict__ in, int x0, int y0,
+                                       uint8_t* __restrict__ buf) {
     constexpr int CHUNKS = FmGeom<R>::CHUNKS;
     const FilterView v = filter_view(a);
-    const int mis = (int)((uintptr_t)a.in & 15u);
+    const int mis = (int)((uintptr_t)in & 15u);
     const int extent = (int)__umul24((uint32_t)(a.view_last_row - v.first_row), (uint32_t)a.in_stride) + a.pad * 3;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a.in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
+        (void*)(in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
+    const int wave = fm_wave(), lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < FM_FETCH; ++i) {
-        const int c = i * FT_THREADS + (int)threadIdx.x;
+    for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
+        const int c0 = i * FT_THREADS + wave * 64;
+        if (c0 >= CHUNKS) break;
+        const int c = c0 + lane;
         const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
         if (c < CHUNKS && ch < FM_ROW_CHUNKS) {
-            const int st = filter_row_start<R>(a, v, x0, y0, lr) + mis;
+            const int st = fm_row_start<R>(a, v, x0, y0, lr) + mis;
             const int off = (st & ~15) + ch * 16;
-            f.v[i] = __builtin_amdgcn_raw_buffer_load_b128(rs, off < 0 ? 0 : off, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)(buf + c0 * 16), 16,
+                (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
         }
     }
 }
 
+// Direct-to-LDS loads every wave issues per window (wave 3 has the fewest): the wait that leaves
+// this many outstanding has seen the previous window land, since loads complete in order.
 template <int R>
-__device__ __forceinline__ void fm_put(const FmFetch& f, uint8_t* buf) {
-    constexpr int CHUNKS = FmGeom<R>::CHUNKS;
-#pragma unroll
-    for (int i = 0; i < FM_FETCH; ++i) {
-        const int c = i * FT_THREADS + (int)threadIdx.x;
-        const int ch = c % FM_PITCH_CHUNKS;
-        if (c < CHUNKS && ch < FM_ROW_CHUNKS) *(u32x4*)(buf + c * 16) = f.v[i];
-    }
+__host__ __device__ constexpr int fm_dma_min() {
+    return (FmGeom<R>::CHUNKS - 3 * 64 + FT_THREADS - 1) / FT_THREADS;
 }
 
-// Byte of pixel x0 - 4 in staged row lr.
+// Byte of pixel x0 - 8 in staged row lr.
 template <int R>
 __device__ __forceinline__ int fm_row_byte(const FilterArgs& a, const FilterView& v, int x0, int y0, int lr, int mis) {
-    return lr * FM_ROW + ((filter_row_start<R>(a, v, x0, y0, lr) + mis) & 15);
+    return lr * FM_ROW + ((fm_row_start<R>(a, v, x0, y0, lr) + mis) & 15);
 }
 
 // Out-of-view pixels of the window that valid outputs read get the edge pixel's bytes.
@@ -522,18 +535,23 @@ __device__ __forceinline__ void fm_patch(const FilterArgs& a, int x0, int y0, ui
         const int src = side ? v.last_col : v.first_col;
         if (px < x0 - R || px > x0 + FM_W - 1 + R) continue;
         const int rb = fm_row_byte<R>(a, v, x0, y0, lr, mis);
-        uint8_t* d = buf + rb + (px - (x0 - 4)) * 3;
-        const uint8_t* sp = buf + rb + (src - (x0 - 4)) * 3;
+        uint8_t* d = buf + rb + (px - (x0 - 8)) * 3;
+        const uint8_t* sp = buf + rb + (src - (x0 - 8)) * 3;
         const uint8_t b0 = sp[0], b1 = sp[1], b2 = sp[2];
         d[0] = b0; d[1] = b1; d[2] = b2;
     }
 }
 
 // 16 bytes of LDS at byte address q, as an int8 MFMA operand biased by XOR 0x80.
-template <bool AL8>
+// AL: the alignment every q of the tile has (16, 8, or 1 for none); the 16-byte reads of a B
+// operand (32 rows of one column at an odd number of chunks apart) hit distinct banks.
+template <int AL>
 __device__ __forceinline__ i32x4 fm_load_b(const uint8_t* base, int q) {
     uint32_t d[4];
-    if constexpr (AL8) {
+    if constexpr (AL == 16) {
+        const u32x4 u = *(const u32x4*)(base + q);
+        d[0] = u.x; d[1] = u.y; d[2] = u.z; d[3] = u.w;
+    } else if constexpr (AL == 8) {
         const uint2 u0 = *(const uint2*)(base + q), u1 = *(const uint2*)(base + q + 8);
         d[0] = u0.x; d[1] = u0.y; d[2] = u1.x; d[3] = u1.y;
     } else {
@@ -558,12 +576,12 @@ struct FmAcc {
     i32x16 v[3];
 };
 
-template <int R, bool AL8>
+template <int R, int AL>
 __device__ __forceinline__ void fm_compute(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A, int x0, int y0,
                                            const uint8_t* buf, FmAcc& acc) {
     const FilterView v = filter_view(a);
     const int mis = (int)((uintptr_t)a.in & 15u);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = fm_wave();
     const int n = lane & 31, h = lane >> 5;
     const int bias = 128 * w.wsum;
 #pragma unroll
@@ -572,12 +590,12 @@ __device__ __forceinline__ void fm_compute(const FilterArgs& a, const FilterWeig
         for (int r = 0; r < 16; ++r) acc.v[b][r] = bias;
 #pragma unroll
     for (int dy = 0; dy < 2 * R + 1; ++dy) {
-        // tile byte 96 wave + 32 q - 8 + 16 h of staged row n + dy (pixel x0 - 4 is 12 bytes before tile byte 0)
-        const int q0 = fm_row_byte<R>(a, v, x0, y0, n + dy, mis) + 4 + 96 * wave + 16 * h;
-        const i32x4 b0 = fm_load_b<AL8>(buf, q0);
-        const i32x4 b1 = fm_load_b<AL8>(buf, q0 + 32);
-        const i32x4 b2 = fm_load_b<AL8>(buf, q0 + 64);
-        const i32x4 b3 = fm_load_b<AL8>(buf, q0 + 96);
+        // tile byte 96 wave + 32 q - 16 + 16 h of staged row n + dy (pixel x0 - 8 is 24 bytes before tile byte 0)
+        const int q0 = fm_row_byte<R>(a, v, x0, y0, n + dy, mis) + 8 + 96 * wave + 16 * h;
+        const i32x4 b0 = fm_load_b<AL>(buf, q0);
+        const i32x4 b1 = fm_load_b<AL>(buf, q0 + 32);
+        const i32x4 b2 = fm_load_b<AL>(buf, q0 + 64);
+        const i32x4 b3 = fm_load_b<AL>(buf, q0 + 96);
         acc.v[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][0][0], b0, acc.v[0], 0, 0, 0);
         acc.v[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][1][0], b1, acc.v[1], 0, 0, 0);
         acc.v[2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A.op[dy][2][0], b2, acc.v[2], 0, 0, 0);
@@ -592,10 +610,11 @@ __device__ __forceinline__ void fm_compute(const FilterArgs& a, const FilterWeig
 // consecutive bytes at 12 places of its row.
 template <int MODE>
 __device__ __forceinline__ void fm_stage_out(const FilterArgs& a, const FmAcc& acc, uint8_t* out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = fm_wave();
     const int n = lane & 31, h = lane >> 5;
     const int shift = a.shift;
     uint8_t* row = out + n * FM_OUT_PITCH + 96 * wave + 4 * h;
+    const int sw = (n >> 4) * 8;                  // half swap of rows 16-31
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -606,7 +625,10 @@ __device__ __forceinline__ void fm_stage_out(const FilterArgs& a, const FmAcc& a
                 const int s = acc.v[b][4 * i + e];
                 o[e] = MODE == 0 ? (uint32_t)s : (uint32_t)clampi(s >> shift, 0, 255);
             }
-            *(uint32_t*)(row + 32 * b + 8 * i) = pack4(o[0], o[1], o[2], o[3]);
+            // an opaque offset: merged into pairs, these writes would conflict again
+            int off = 32 * b + 16 * (i >> 1) + ((8 * (i & 1)) ^ sw);
+            asm volatile("" : "+v"(off));
+            *(uint32_t*)(row + off) = pack4(o[0], o[1], o[2], o[3]);
         }
 }
 
@@ -621,9 +643,11 @@ __device__ __forceinline__ void fm_store_out(const FilterArgs& a, int x0, int y0
     const uint8_t* src = out + r * FM_OUT_PITCH + s16;
     UH_GLOBAL uint8_t* dst = as_global(a.out) + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride) + j0;
     const bool fast = j0 + 256 + 16 <= nbytes && (((uintptr_t)dst) & 15u) == 0;
+    const bool swapped = r >= 16;                 // waves 2 and 3: rows 16-31
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const u32x4 v = *(const u32x4*)(src + 128 * k);
+        u32x4 v = *(const u32x4*)(src + 128 * k);
+        if (swapped) v = u32x4{v.z, v.w, v.x, v.y};
         if (fast) {
             *(UH_GLOBAL u32x4*)(dst + 128 * k) = v;
         } else {
@@ -636,50 +660,74 @@ __device__ __forceinline__ void fm_store_out(const FilterArgs& a, int x0, int y0
     }
 }
 
-// Tiles l0 .. l1-1 of one item (all tiles in one item share its arguments and weights). Per tile
-// t (its window in buf[c], written and made visible at the end of tile t-1): fetch tile t+1's
-// window into registers, store tile t-1's bytes from out[c ^ 1], compute tile t, stage its bytes
-// in out[c], write tile t+1's window into the other buffer and meet at a barrier. The fetch and
-// the stores have a whole tile's products to land before anything waits for them.
+// Workgroup barrier that waits only for this wave's LDS accesses and for all but its last N
+// vector memory operations (__syncthreads would wait for every load in flight, the window two
+// tiles ahead included). The memory clobber keeps the compiler's LDS accesses on their side.
+template <int N>
+__device__ __forceinline__ void fm_barrier() {
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    if constexpr (N == 0) asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" :: "n"(N) : "memory");
+}
+__device__ __forceinline__ void fm_barrier_lds() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// One step of the tile pipeline: store tile j - 3 (staged in outr), load tile j's window into
+// nxt, compute tile j - 2 from cur and stage it in outw, then meet at a barrier once tile j - 1's
+// window has landed. The four areas are distinct; their restrict qualifiers let the compiler see
+// that the window loads in flight (into nxt) do not touch what this step reads from LDS.
+template <int R, int MODE>
+__device__ __forceinline__ void fm_step(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A,
+                                        const uint8_t* __restrict__ in,
+                                        uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt,
+                                        uint8_t* __restrict__ outw, const uint8_t* __restrict__ outr,
+                                        bool st, int sx, int sy, bool ld, int lx, int ly,
+                                        bool cp, int cx, int cy) {
+    if (st) fm_store_out(a, sx, sy, outr);
+    if (ld) fm_dma<R>(a, in, lx, ly, nxt);
+    if (cp) {
+        const FilterView v = filter_view(a);
+        if (fm_needs_patch<R>(a, v, cx)) {
+            fm_patch<R>(a, cx, cy, cur);
+            fm_barrier_lds();
+        }
+        const int mis = (int)((uintptr_t)a.in & 15u);
+        const int q = fm_row_byte<R>(a, v, cx, cy, 0, mis) + 8;   // first B byte of row 0, wave 0
+        FmAcc acc;
+        if ((a.in_stride & 15u) == 0 && (q & 15) == 0) fm_compute<R, 16>(a, w, A, cx, cy, cur, acc);
+        else if ((a.in_stride & 7u) == 0 && (q & 7) == 0) fm_compute<R, 8>(a, w, A, cx, cy, cur, acc);
+        else fm_compute<R, 1>(a, w, A, cx, cy, cur, acc);
+        fm_stage_out<MODE>(a, acc, outw);
+    }
+    if (ld) fm_barrier<fm_dma_min<R>()>();
+    else fm_barrier<0>();
+}
+
+// Tiles l0 .. l1-1 of one item (all tiles in one item share its arguments and weights), as steps
+// j = 0 .. n + 2 of fm_step: the window of tile j goes to buf[j mod 3], so two windows are in
+// flight while a tile is computed, and tile j's bytes are staged in out[j mod 2].
 template <int R, int MODE>
 __device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const FilterWeights& w, uint32_t l0, uint32_t l1) {
     if (l0 >= l1) return;
     FmA<R> A;
     fm_build_a<R>(w, A);
-    const FilterView v = filter_view(a);
-    const int mis = (int)((uintptr_t)a.in & 15u);
-    int x0, y0;
-    fm_origin(a, l0, x0, y0);
-    FmFetch f;
-    fm_fetch<R>(a, x0, y0, f);
-    __syncthreads();              // a previous item's last tile may still be reading the buffers
-    fm_put<R>(f, sm.buf[0]);
-    __syncthreads();
-    int px = 0, py = 0;           // origin of the tile staged in sm.out[c ^ 1]
-    FmAcc acc;
-    for (uint32_t t = l0; t < l1; ++t) {
-        const uint32_t c = (t - l0) & 1u;
-        int x1 = 0, y1 = 0;
-        const bool more = t + 1 < l1;
-        if (more) {
-            fm_origin(a, t + 1, x1, y1);
-            fm_fetch<R>(a, x1, y1, f);
-        }
-        if (t > l0) fm_store_out(a, px, py, sm.out[c ^ 1u]);
-        if (fm_needs_patch<R>(a, v, x0)) {
-            fm_patch<R>(a, x0, y0, sm.buf[c]);
-            __syncthreads();
-        }
-        const bool al8 = (a.in_stride & 7u) == 0 && ((fm_row_byte<R>(a, v, x0, y0, 0, mis) + 4) & 7) == 0;
-        if (al8) fm_compute<R, true>(a, w, A, x0, y0, sm.buf[c], acc);
-        else fm_compute<R, false>(a, w, A, x0, y0, sm.buf[c], acc);
-        fm_stage_out<MODE>(a, acc, sm.out[c]);
-        if (more) fm_put<R>(f, sm.buf[c ^ 1u]);
-        __syncthreads();          // tile t+1's window and tile t's bytes are visible; every wave has left buf[c]
-        px = x0; py = y0;
-        x0 = x1; y0 = y1;
+    __syncthreads();              // a previous item's last step may still be reading the buffers
+    const int n = (int)(l1 - l0);
+    int x1 = 0, y1 = 0, x2 = 0, y2 = 0, x3 = 0, y3 = 0;   // origins of tiles j - 1, j - 2, j - 3
+    int r3 = 0;                   // j mod 3
+    for (int j = 0; j <= n + 2; ++j) {
+        int x0 = 0, y0 = 0;
+        const bool ld = j < n;
+        if (ld) fm_origin(a, l0 + (uint32_t)j, x0, y0);
+        const int rc = r3 == 2 ? 0 : r3 + 1;              // (j - 2) mod 3 = (j + 1) mod 3
+        fm_step<R, MODE>(a, w, A, a.in, sm.buf[rc], sm.buf[r3], sm.out[j & 1], sm.out[(j + 1) & 1],
+                         j >= 3, x3, y3, ld, x0, y0, j >= 2 && j - 2 < n, x2, y2);
+        x3 = x2; y3 = y2;
+        x2 = x1; y2 = y1;
+        x1 = x0; y1 = y0;
+        r3 = rc;
     }
-    fm_store_out(a, px, py, sm.out[(l1 - 1 - l0) & 1u]);
 }
 
 // Direct path for any radius <= 15: one output pixel per thread.
