@@ -476,29 +476,60 @@ __device__ __forceinline__ int fm_wave() {
 // read as zero, a chunk wholly before it is redirected to offset 0; see filter_dma) straight into
 // LDS: wave w's load i fills chunk slots 256 i + 64 w + lane (pad slots masked off), so no
 // register holds them and the ring can hold two tiles in flight.
+static constexpr int FM_DMA_MAX = (FmGeom<2>::CHUNKS + FT_THREADS - 1) / FT_THREADS;   // loads per window
+
+// This lane's part of the window loads, fixed for a work item: whether load i has a chunk for it
+// and, for rows that need no clamping at a 16-byte multiple stride, the chunk's offset from the
+// window's first row start (16 ch - lr stride for staged row lr, chunk ch).
+struct FmDmaLane {
+    int k[FM_DMA_MAX];
+    bool on[FM_DMA_MAX];
+};
+
 template <int R>
-__device__ __forceinline__ void fm_dma(const FilterArgs& a, const uint8_t* __restrict__ in, int x0, int y0,
-                                       uint8_t* __restrict__ buf) {
+__device__ __forceinline__ void fm_dma_lane(const FilterArgs& a, FmDmaLane& d) {
+    constexpr int CHUNKS = FmGeom<R>::CHUNKS;
+    const int wave = fm_wave(), lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < FM_DMA_MAX; ++i) {
+        const int c = i * FT_THREADS + wave * 64 + lane;
+        const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
+        d.on[i] = c < CHUNKS && ch < FM_ROW_CHUNKS;
+        d.k[i] = 16 * ch - (int)__umul24((uint32_t)lr, (uint32_t)a.in_stride);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void fm_dma(const FilterArgs& a, const uint8_t* __restrict__ in, const FmDmaLane& d,
+                                       int x0, int y0, uint8_t* __restrict__ buf) {
     constexpr int CHUNKS = FmGeom<R>::CHUNKS;
     const FilterView v = filter_view(a);
     const int mis = (int)((uintptr_t)in & 15u);
     const int extent = (int)__umul24((uint32_t)(a.view_last_row - v.first_row), (uint32_t)a.in_stride) + a.pad * 3;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(in - mis), 0, (mis + extent + 15) & ~15, 0x00020000);
-    const int wave = fm_wave(), lane = threadIdx.x & 63;
+    const int wave = fm_wave();
+    // rows y0 - R .. y0 - R + LROWS - 1 all inside the view: one offset per window plus the lane's
+    const bool inner = (a.in_stride & 15u) == 0 && y0 - R >= v.first_row &&
+                       y0 - R + FmGeom<R>::LROWS - 1 <= v.last_row;
+    const int base = (fm_row_start<R>(a, v, x0, y0, 0) + mis) & ~15;
 #pragma unroll
-    for (int i = 0; i < (CHUNKS + FT_THREADS - 1) / FT_THREADS; ++i) {
+    for (int i = 0; i < FM_DMA_MAX; ++i) {
         const int c0 = i * FT_THREADS + wave * 64;
         if (c0 >= CHUNKS) break;
-        const int c = c0 + lane;
-        const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
-        if (c < CHUNKS && ch < FM_ROW_CHUNKS) {
-            const int st = fm_row_start<R>(a, v, x0, y0, lr) + mis;
-            const int off = (st & ~15) + ch * 16;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rs, (__attribute__((address_space(3))) void*)(buf + c0 * 16), 16,
-                (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
+        if (!d.on[i]) continue;
+        int off;
+        if (inner) {
+            off = base + d.k[i];
+        } else {
+            int c = c0 + (int)(threadIdx.x & 63);
+            asm volatile("" : "+v"(c));               // recomputed here, not kept for every tile
+            const int lr = c / FM_PITCH_CHUNKS, ch = c - lr * FM_PITCH_CHUNKS;
+            off = ((fm_row_start<R>(a, v, x0, y0, lr) + mis) & ~15) + ch * 16;
         }
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rs, (__attribute__((address_space(3))) void*)(buf + c0 * 16), 16,
+            (uint32_t)(off < 0 ? 0 : off), 0, 0, 0);
     }
 }
 
@@ -605,31 +636,57 @@ __device__ __forceinline__ void fm_compute(const FilterArgs& a, const FilterWeig
     }
 }
 
+typedef short fm_i16x2 __attribute__((ext_vector_type(2)));
+
+// Four products as output bytes (byte 0 = e0). Mode 0 keeps the low bytes. Mode 1 takes
+// clamp(s >> shift, 0, 255); for shift <= 7 (NARROW) the sums are first saturated to 16 bits in
+// pairs, which gives the same bytes: a sum above 32767 shifts to at least 32767 >> 7 = 255, one
+// below -32768 to a negative value, so both clamp as before.
+template <int MODE, bool NARROW>
+__device__ __forceinline__ uint32_t fm_word(int e0, int e1, int e2, int e3, int shift) {
+    if constexpr (MODE == 0) {
+        return pack4((uint32_t)e0, (uint32_t)e1, (uint32_t)e2, (uint32_t)e3);
+    } else if constexpr (NARROW) {
+        const fm_i16x2 sh = {(short)shift, (short)shift};
+        const fm_i16x2 p01 = __builtin_amdgcn_cvt_pk_i16(e0, e1) >> sh;
+        const fm_i16x2 p23 = __builtin_amdgcn_cvt_pk_i16(e2, e3) >> sh;
+        uint32_t q01, q23;
+        asm("v_sat_pk_u8_i16 %0, %1" : "=v"(q01) : "v"(__builtin_bit_cast(uint32_t, p01)));
+        asm("v_sat_pk_u8_i16 %0, %1" : "=v"(q23) : "v"(__builtin_bit_cast(uint32_t, p23)));
+        return __builtin_amdgcn_perm(q23, q01, 0x05040100u);
+    } else {
+        return pack4((uint32_t)clampi(e0 >> shift, 0, 255), (uint32_t)clampi(e1 >> shift, 0, 255),
+                     (uint32_t)clampi(e2 >> shift, 0, 255), (uint32_t)clampi(e3 >> shift, 0, 255));
+    }
+}
+
 // A tile's products as bytes into an output staging area. D[m][n]: column n = lane % 32 (tile
 // row), row m = 8 (r / 4) + 4 (lane / 32) + r % 4 (byte of the block): each lane holds 4
-// consecutive bytes at 12 places of its row.
-template <int MODE>
-__device__ __forceinline__ void fm_stage_out(const FilterArgs& a, const FmAcc& acc, uint8_t* out) {
+// consecutive bytes at 12 places of its row, 8 (r / 4) bytes apart. The writes go one dword at a
+// time (the empty asm statements keep the compiler from pairing them, which would bring the bank
+// conflicts back) at immediate offsets from two bases, one per half of a 16-byte chunk.
+template <int MODE, bool NARROW>
+__device__ __forceinline__ void fm_stage_words(const FmAcc& acc, int shift, uint8_t* out) {
     const int lane = threadIdx.x & 63, wave = fm_wave();
     const int n = lane & 31, h = lane >> 5;
-    const int shift = a.shift;
-    uint8_t* row = out + n * FM_OUT_PITCH + 96 * wave + 4 * h;
     const int sw = (n >> 4) * 8;                  // half swap of rows 16-31
+    uint8_t* row = out + n * FM_OUT_PITCH + 96 * wave + 4 * h;
+    uint8_t* half[2] = {row + sw, row + (8 ^ sw)};
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            uint32_t o[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int s = acc.v[b][4 * i + e];
-                o[e] = MODE == 0 ? (uint32_t)s : (uint32_t)clampi(s >> shift, 0, 255);
-            }
-            // an opaque offset: merged into pairs, these writes would conflict again
-            int off = 32 * b + 16 * (i >> 1) + ((8 * (i & 1)) ^ sw);
-            asm volatile("" : "+v"(off));
-            *(uint32_t*)(row + off) = pack4(o[0], o[1], o[2], o[3]);
+            const uint32_t word = fm_word<MODE, NARROW>(acc.v[b][4 * i], acc.v[b][4 * i + 1], acc.v[b][4 * i + 2],
+                                                        acc.v[b][4 * i + 3], shift);
+            asm volatile("" ::: "memory");
+            *(uint32_t*)(half[i & 1] + 32 * b + 16 * (i >> 1)) = word;
         }
+}
+
+template <int MODE>
+__device__ __forceinline__ void fm_stage_out(const FilterArgs& a, const FmAcc& acc, uint8_t* out) {
+    if (MODE == 1 && a.shift <= 7) fm_stage_words<MODE, true>(acc, a.shift, out);
+    else fm_stage_words<MODE, false>(acc, a.shift, out);
 }
 
 // Stores of a staged tile: thread t moves row t / 8, bytes 16 (t % 8) + 128 k, k < 3, so a
@@ -643,14 +700,15 @@ __device__ __forceinline__ void fm_store_out(const FilterArgs& a, int x0, int y0
     const uint8_t* src = out + r * FM_OUT_PITCH + s16;
     UH_GLOBAL uint8_t* dst = as_global(a.out) + mul24_wide((uint32_t)(y - a.row0), (uint32_t)a.out_stride) + j0;
     const bool fast = j0 + 256 + 16 <= nbytes && (((uintptr_t)dst) & 15u) == 0;
-    const bool swapped = r >= 16;                 // waves 2 and 3: rows 16-31
+    const bool swapped = fm_wave() >= 2;          // rows 16-31 (a scalar test)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         u32x4 v = *(const u32x4*)(src + 128 * k);
-        if (swapped) v = u32x4{v.z, v.w, v.x, v.y};
         if (fast) {
-            *(UH_GLOBAL u32x4*)(dst + 128 * k) = v;
+            if (swapped) *(UH_GLOBAL u32x4*)(dst + 128 * k) = u32x4{v.z, v.w, v.x, v.y};
+            else *(UH_GLOBAL u32x4*)(dst + 128 * k) = v;
         } else {
+            if (swapped) v = u32x4{v.z, v.w, v.x, v.y};
             const int j = j0 + 128 * k;
             const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -679,13 +737,13 @@ __device__ __forceinline__ void fm_barrier_lds() {
 // that the window loads in flight (into nxt) do not touch what this step reads from LDS.
 template <int R, int MODE>
 __device__ __forceinline__ void fm_step(const FilterArgs& a, const FilterWeights& w, const FmA<R>& A,
-                                        const uint8_t* __restrict__ in,
+                                        const FmDmaLane& dl, const uint8_t* __restrict__ in,
                                         uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt,
                                         uint8_t* __restrict__ outw, const uint8_t* __restrict__ outr,
                                         bool st, int sx, int sy, bool ld, int lx, int ly,
                                         bool cp, int cx, int cy) {
     if (st) fm_store_out(a, sx, sy, outr);
-    if (ld) fm_dma<R>(a, in, lx, ly, nxt);
+    if (ld) fm_dma<R>(a, in, dl, lx, ly, nxt);
     if (cp) {
         const FilterView v = filter_view(a);
         if (fm_needs_patch<R>(a, v, cx)) {
@@ -712,6 +770,8 @@ __device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const Fil
     if (l0 >= l1) return;
     FmA<R> A;
     fm_build_a<R>(w, A);
+    FmDmaLane dl;
+    fm_dma_lane<R>(a, dl);
     __syncthreads();              // a previous item's last step may still be reading the buffers
     const int n = (int)(l1 - l0);
     int x1 = 0, y1 = 0, x2 = 0, y2 = 0, x3 = 0, y3 = 0;   // origins of tiles j - 1, j - 2, j - 3
@@ -721,7 +781,7 @@ __device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const Fil
         const bool ld = j < n;
         if (ld) fm_origin(a, l0 + (uint32_t)j, x0, y0);
         const int rc = r3 == 2 ? 0 : r3 + 1;              // (j - 2) mod 3 = (j + 1) mod 3
-        fm_step<R, MODE>(a, w, A, a.in, sm.buf[rc], sm.buf[r3], sm.out[j & 1], sm.out[(j + 1) & 1],
+        fm_step<R, MODE>(a, w, A, dl, a.in, sm.buf[rc], sm.buf[r3], sm.out[j & 1], sm.out[(j + 1) & 1],
                          j >= 3, x3, y3, ld, x0, y0, j >= 2 && j - 2 < n, x2, y2);
         x3 = x2; y3 = y2;
         x2 = x1; y2 = y1;
