@@ -705,8 +705,8 @@ __device__ __forceinline__ void fm_store_out(const FilterArgs& a, int x0, int y0
     for (int k = 0; k < 3; ++k) {
         u32x4 v = *(const u32x4*)(src + 128 * k);
         if (fast) {
-            if (swapped) *(UH_GLOBAL u32x4*)(dst + 128 * k) = u32x4{v.z, v.w, v.x, v.y};
-            else *(UH_GLOBAL u32x4*)(dst + 128 * k) = v;
+            if (swapped) __builtin_nontemporal_store(u32x4{v.z, v.w, v.x, v.y}, (UH_GLOBAL u32x4*)(dst + 128 * k));
+            else __builtin_nontemporal_store(v, (UH_GLOBAL u32x4*)(dst + 128 * k));
         } else {
             if (swapped) v = u32x4{v.z, v.w, v.x, v.y};
             const int j = j0 + 128 * k;
