@@ -18,7 +18,8 @@
 //   - Row sequences (elem_stride 1): one workgroup per row for n >= 4096 (256 threads, one
 //     radix-16 butterfly each per pass), 256 / (n/16) rows per workgroup below that.
 //   - Column sequences (seq_stride 1): lanes run across adjacent columns first, so a wave reads
-//     whole 32-128 byte row segments; S columns per workgroup (4 at n = 4096, 139 KB of LDS).
+//     whole 32-128 byte row segments; S columns per workgroup (4 at n = 4096, 139 KB of LDS), and
+//     the workgroups that share a line are placed on one XCD.
 // Pass twiddles exp(-2 pi i m / 8192) come from one 64 KB table per device, computed in double on
 // the host and rounded once to float (n = 8192 / 2^q reads every 2^q-th entry); the table is a
 // device global filled at the first call on each device.
@@ -206,6 +207,23 @@ __device__ __forceinline__ void fft_passes(float2* lds, int s, int t, bool live,
     if constexpr (!LAST) fft_passes<L, S, SEQ_FAST, PASS + 1, NS * R>(lds, s, t, live, in, in_es, out, out_es);
 }
 
+// Column groups whose S columns share 128-byte lines (K groups per line, S * 8 bytes each) run
+// on one XCD: workgroup b runs on XCD b mod 8, so in every run of 8K workgroups, group
+// 8K q + K r + u goes to workgroup 8K q + 8u + r. The K reads of a line then meet in that XCD's L2
+// instead of each coming from HBM. A grid tail shorter than 8K keeps the identity order.
+template <int S, bool SEQ_FAST>
+__device__ __forceinline__ uint32_t fft_group(uint32_t b, uint32_t grid) {
+    constexpr uint32_t K = (SEQ_FAST && S * 8 < 128) ? 128 / (S * 8) : 1;
+    if constexpr (K == 1) {
+        return b;
+    } else {
+        const uint32_t q = b / (8 * K);
+        if ((q + 1) * 8 * K > grid) return b;
+        const uint32_t w = b % (8 * K);
+        return q * 8 * K + (w % 8) * K + w / 8;
+    }
+}
+
 template <int L, int S, bool SEQ_FAST>
 __global__ void __launch_bounds__((FftCfg<L, S, SEQ_FAST>::THREADS))
 k_fft(const float2* in, uint64_t in_ss, uint64_t in_es, float2* out, uint64_t out_ss, uint64_t out_es,
@@ -215,7 +233,7 @@ k_fft(const float2* in, uint64_t in_ss, uint64_t in_es, float2* out, uint64_t ou
     const int tid = threadIdx.x;
     const int s = SEQ_FAST ? tid % S : tid / P::T;
     const int t = SEQ_FAST ? tid / S : tid % P::T;
-    const uint64_t seq = (uint64_t)blockIdx.x * S + s;
+    const uint64_t seq = (uint64_t)fft_group<S, SEQ_FAST>(blockIdx.x, gridDim.x) * S + s;
     const bool live = seq < count;
     const UH_GLOBAL f32x2_t* gi = (const UH_GLOBAL f32x2_t*)as_global(in) + (live ? seq * in_ss : 0);
     UH_GLOBAL f32x2_t* go = (UH_GLOBAL f32x2_t*)as_global(out) + (live ? seq * out_ss : 0);
