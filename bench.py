@@ -15,6 +15,8 @@ same clock, with its own roofline, CPU baseline and parity check:
   - matrixMultiply N=2048 float32, 16 subtasks of 512 x 512 (split-BF16 MFMA);
   - pageRank on a 10M-page power-law web graph, 20 iterations, 1M-page subtasks;
   - luDecomposition N=16384 (512 blocks, split-BF16 trailing update) + 8-RHS refined solve;
+  - the other apps (matrixTranspose, prefixSum, sparseSolver, fft, pageRankInvertedIndex at the
+    reference's default sizes), one record with a row per app;
   - the serving mix: seeded matrixMultiply / imageFiltering / pageRank requests served at
     saturation (all requests queued at once, `--serving-slots` clients) by the BATCHED and the
     PLAIN stub, then at two lower offered Poisson loads for latency.
@@ -27,14 +29,13 @@ CPU baseline: the same task on the pm:: runtime's CPU stubs (one thread per core
 share), which run the apps' CPU callbacks (restatements of the reference's CPU callbacks) over
 the same subtask decomposition; luDecomposition is timed at N=4096.
 
-`--workload apps` measures the other apps instead (matrixTranspose, prefixSum, sparseSolver, fft,
-pageRankInvertedIndex at the reference's default sizes), one line with a row per app.
+`--workload apps` measures only the other apps (the fft row as the line's headline).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                        [--workload all|imagefilter|matmul|pagerank|lu|apps]
                        [--no-cpu-baseline] [--serving-requests N] [--serving-slots S]
 --steps / --warmup apply to the headline (or the one workload named); defaults per workload:
-imagefilter and matmul 200 / 3, pagerank 10 / 1, lu 3 / 1.
+imagefilter and matmul 200 / 3, pagerank 10 / 1, lu 3 / 1, apps 8 / 1.
 """
 from __future__ import annotations
 
@@ -816,7 +817,7 @@ def main():
     line["host_cores"] = cores
     if args.workload == "all":
         rows = {}
-        for name in ("matmul", "pagerank", "lu"):
+        for name in ("matmul", "pagerank", "lu", "apps"):
             s, w = DEFAULTS[name]
             rows[name] = BENCHES[name](args, U, s, w, cores)
         line["rows"] = rows
