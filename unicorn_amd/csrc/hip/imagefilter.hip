@@ -776,10 +776,17 @@ __device__ __forceinline__ void fm_run(FmLds& sm, const FilterArgs& a, const Fil
     const int n = (int)(l1 - l0);
     int x1 = 0, y1 = 0, x2 = 0, y2 = 0, x3 = 0, y3 = 0;   // origins of tiles j - 1, j - 2, j - 3
     int r3 = 0;                   // j mod 3
+    int xn, yn;                   // origin of tile l0 + j, advanced down the column strips
+    fm_origin(a, l0, xn, yn);
+    const int yend = a.row0 + a.rows;
     for (int j = 0; j <= n + 2; ++j) {
         int x0 = 0, y0 = 0;
         const bool ld = j < n;
-        if (ld) fm_origin(a, l0 + (uint32_t)j, x0, y0);
+        if (ld) {
+            x0 = xn; y0 = yn;
+            yn += FM_H;
+            if (yn >= yend) { yn = a.row0; xn += FM_W; }
+        }
         const int rc = r3 == 2 ? 0 : r3 + 1;              // (j - 2) mod 3 = (j + 1) mod 3
         fm_step<R, MODE>(a, w, A, dl, a.in, sm.buf[rc], sm.buf[r3], sm.out[j & 1], sm.out[(j + 1) & 1],
                          j >= 3, x3, y3, ld, x0, y0, j >= 2 && j - 2 < n, x2, y2);
