@@ -57,6 +57,7 @@ typedef struct pmHipStubStats
     uint64_t ma_copies;         /* multi-assigned copies of CPU subtasks run by the HIP stub */
     uint64_t ma_wins;           /* of those, copies that finished first and committed */
     uint64_t oom_waits;         /* device allocations retried after retiring the oldest in-flight subtask */
+    uint64_t sentinel_compressed; /* PLAIN: reduction outputs sent back in sentinel-compressed form */
 } pmHipStubStats;
 
 #ifdef __cplusplus
@@ -102,6 +103,12 @@ size_t pmGetHbmResident();
 /* Number of CPU stub threads (0 = one per host core). Takes effect at pmInitialize. */
 pmStatus pmSetCpuStubCount(int pCount);
 pmStatus pmGetHipStubStats(pmHipStubStats* pStats);
+/* Size above which a PLAIN-mode reduction subtask that writes its task's only address space
+ * whole sends the output back sentinel-compressed (its non-zero elements and their indices) when
+ * fewer than 40 % of the elements are non-zero. The default is the reference's 1 GiB
+ * (CUDA_SENTINEL_COMPRESSION_THRESHOLD, pmInternalDefinitions.h:223); a smaller value lets tests
+ * and smaller outputs use the path. pPrevious (optional) receives the value replaced. */
+pmStatus pmSetSentinelCompressionThreshold(size_t pBytes, size_t* pPrevious);
 pmStatus pmResetHipStubStats();
 /* Make the HBM mirror of an address space current and keep it (prefetch). */
 pmStatus pmPrefetchToDevice(pmMemHandle pMemHandle);

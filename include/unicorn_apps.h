@@ -28,7 +28,7 @@ typedef struct ua_stats {
     uint64_t subtasks, batches, h2d_bytes, d2h_bytes, d2d_bytes, cache_hits, evictions;
     uint64_t plain_launches, persistent_launches, queue_items;
     uint64_t ns_queue, ns_stage, ns_flush, ns_wait, ns_retire, ns_wake;   /* see pmHipStubStats */
-    uint64_t ma_copies, ma_wins, oom_waits;
+    uint64_t ma_copies, ma_wins, oom_waits, sentinel_compressed;
 } ua_stats;
 
 int ua_init(int cpu_stubs);                   /* pmInitialize; cpu_stubs 0 = one per core */
@@ -40,6 +40,7 @@ int ua_set_hip_graphs(int on);
 int ua_set_hip_depth(int depth);
 int ua_set_batch_limit(int subtasks);
 int ua_set_hbm_budget(uint64_t bytes);
+int ua_set_sentinel_threshold(uint64_t bytes, uint64_t* previous);   /* pmSetSentinelCompressionThreshold */
 /* canSplitCpuSubtasks / canSplitGpuSubtasks of the imageFiltering and matrixMultiply tasks
  * submitted afterwards (default 1 / 0, as the reference apps) and the pieces per split subtask
  * (pmSetSplitFactors; 0 = one per stub of the splitting device type). */

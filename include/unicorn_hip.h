@@ -126,6 +126,17 @@ int uh_invert_index(const uint32_t* dump, uint32_t pages, uint32_t max_out, uint
 int uh_prii_rank(const void* entries, uint32_t n, const float* in_ranks, float initial, float damping,
                  uint32_t first_page, float* out, uint32_t out_pages, uint32_t total_pages, void* stream);
 
+/* ---- sentinel compression (replaces pmCudaInterface::CompressForSentinel,
+ *      pmCudaInterface.cu:380-472, called from pmExecutionStub.cpp:3850-3881) ----
+ * Counts the non-zero elements of data[0 .. elems) (type: pmReductionDataType, 0 int .. 5 double;
+ * an element is zero when it compares equal to zero of its type) into *nonzero, waiting for the
+ * stream. When *nonzero <= max_nonzero, also writes, in stream order, the compressed form to out
+ * (8-byte aligned, at least 8 + max_nonzero * (element size + 4) bytes): a uint64 count, the
+ * non-zero elements in index order, then their uint32 indices; otherwise out is not written.
+ * elems must be below 2^32. */
+int uh_sentinel_compress(const void* data, uint64_t elems, int type, void* out, uint64_t max_nonzero,
+                         uint64_t* nonzero, void* stream);
+
 /* ---- subscription staging (replaces the pinned-buffer host memcpy + per-record H2D of
  *      pmExecutionStub.cpp:3655-3847 for HBM-resident address spaces) ----
  * Copies n_records byte ranges: record i moves len[i] bytes from src + src_off[i] to

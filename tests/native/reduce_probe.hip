@@ -27,7 +27,7 @@ namespace {
 struct RConf {
     uint32_t n;      // elements
     int op, type;
-    uint32_t pad;
+    uint32_t sparse; // nonzero: each subtask sets only about one element in 16 (the rest 0)
 };
 
 __host__ __device__ inline uint32_t mix(uint32_t a, uint32_t b) {
@@ -40,7 +40,8 @@ __host__ __device__ inline uint32_t mix(uint32_t a, uint32_t b) {
 
 // small integers that every element type holds exactly: products of at most 8 subtasks stay
 // below 2^8, sums below 8 * 3
-__host__ __device__ inline long long gen(int op, int type, uint32_t sid, uint32_t i) {
+__host__ __device__ inline long long gen(int op, int type, uint32_t sid, uint32_t i, uint32_t sparse) {
+    if (sparse && (mix(sid + 101, i) & 15u) != 0) return 0;
     const uint32_t h = mix(sid + 1, i);
     const bool is_unsigned = type == REDUCE_UNSIGNED_INTS || type == REDUCE_UNSIGNED_LONGS;
     switch (op) {
@@ -57,12 +58,12 @@ __global__ void rp_kernel(pmTaskInfo ti, pmDeviceInfo*, pmSubtaskInfo si, pmStat
     const RConf* c = (const RConf*)ti.taskConf;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c->n) return;
-    ((T*)si.memInfo[0].writePtr)[i] = (T)gen(c->op, c->type, (uint32_t)si.subtaskId, i);
+    ((T*)si.memInfo[0].writePtr)[i] = (T)gen(c->op, c->type, (uint32_t)si.subtaskId, i, c->sparse);
 }
 
 template <typename T>
 void fill_cpu(const RConf* c, uint32_t sid, void* p) {
-    for (uint32_t i = 0; i < c->n; ++i) ((T*)p)[i] = (T)gen(c->op, c->type, sid, i);
+    for (uint32_t i = 0; i < c->n; ++i) ((T*)p)[i] = (T)gen(c->op, c->type, sid, i, c->sparse);
 }
 
 size_t type_size(int type) { return type == REDUCE_INTS || type == REDUCE_UNSIGNED_INTS || type == REDUCE_FLOATS ? 4 : 8; }
@@ -182,7 +183,8 @@ pmCallbackHandle g_dcb[3];
 
 // device: 0 CPU stubs, 1 the HIP stub, 2 both. Returns the task's pmStatus; out receives the
 // reduced space (n elements of the type).
-extern "C" int rp_reduce(int op, int type, uint32_t subtasks, uint32_t n, int device, void* out) {
+// sparse: nonzero makes every subtask set only about one element in 16 (the others 0).
+extern "C" int rp_reduce_ex(int op, int type, uint32_t subtasks, uint32_t n, int device, uint32_t sparse, void* out) {
     if (op < 0 || op >= MAX_REDUCTION_OP_TYPES || type < 0 || type >= MAX_REDUCTION_DATA_TYPES || device < 0 || device > 2)
         return pmInvalidIndex;
     pmCallbackHandle cbh;
@@ -207,7 +209,7 @@ extern "C" int rp_reduce(int op, int type, uint32_t subtasks, uint32_t n, int de
     pmRawMemPtr p;
     pmGetRawMemPtr(mh, &p);
     memset(p, 0, bytes);
-    RConf c{n, op, type, 0};
+    RConf c{n, op, type, sparse};
     pmTaskMem mem[1] = {pmTaskMem(mh, WRITE_ONLY, SUBSCRIPTION_NATURAL)};
     pmTaskDetails d(&c, sizeof(c), mem, 1, cbh, subtasks);
     pmTaskHandle th = nullptr;
@@ -221,6 +223,10 @@ extern "C" int rp_reduce(int op, int type, uint32_t subtasks, uint32_t n, int de
     memcpy(out, p, bytes);
     pmReleaseMemory(mh);
     return (int)s;
+}
+
+extern "C" int rp_reduce(int op, int type, uint32_t subtasks, uint32_t n, int device, void* out) {
+    return rp_reduce_ex(op, type, subtasks, n, device, 0, out);
 }
 
 // out: subtasks * n words, the redistributed space; meta: up to orders (order, bytes) pairs;

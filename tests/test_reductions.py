@@ -32,6 +32,8 @@ def load(runtime):
     L = C.CDLL(str(PROBE))
     L.rp_reduce.restype = C.c_int
     L.rp_reduce.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]
+    L.rp_reduce_ex.restype = C.c_int
+    L.rp_reduce_ex.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p]
     L.rp_redistribute.restype = C.c_int
     L.rp_redistribute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
                                   C.POINTER(C.c_uint32)]
@@ -53,7 +55,8 @@ def _mix(a, b):
     return h
 
 
-def gen(op, t, sid, n):
+def gen(op, t, sid, n, sparse=False):
+    """Subtask sid's values (reduce_probe.hip gen); sparse keeps about one element in 16."""
     h = _mix(sid + 1, np.arange(n))
     uns = t in (np.uint32, np.uint64)
     name = OPS[op]
@@ -65,6 +68,8 @@ def gen(op, t, sid, n):
         v = (h % 2).astype(np.int64)
     else:
         v = (h % 5).astype(np.int64)
+    if sparse:
+        v = np.where((_mix(sid + 101, np.arange(n)) & 15) != 0, 0, v)
     return v.astype(t)
 
 
@@ -161,3 +166,18 @@ def test_redistribution_cpu(lib, subtasks, n, orders):
     np.testing.assert_array_equal(out[:body.size], body)
     assert np.all(out[body.size:] == 0)
     assert meta == want_meta
+
+
+@pytest.mark.parametrize("ti", range(6))
+def test_sparse_reduction_cpu(lib, ti):
+    """The sparse variant of the probe (about one element in 16 set per subtask), used by the GPU
+    sentinel-compression tests, folds as numpy does on the CPU stubs."""
+    n, subtasks = 1000, 4
+    t = TYPES[ti]
+    for op in range(10):
+        out = np.zeros(n, dtype=t)
+        rc = lib.rp_reduce_ex(op, ti, subtasks, n, 0, 1, out.ctypes.data)
+        assert rc == 0, (op, ti)
+        want = fold(op, t, [gen(op, t, s, n, sparse=True) for s in range(subtasks)])
+        np.testing.assert_array_equal(out, want, err_msg=f"op {op} type {ti}")
+    assert 0 < np.count_nonzero(gen(0, t, 0, n, sparse=True)) < n // 8

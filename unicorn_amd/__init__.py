@@ -60,7 +60,7 @@ class ua_stats(C.Structure):
         "subtasks", "batches", "h2d_bytes", "d2h_bytes", "d2d_bytes", "cache_hits", "evictions",
         "plain_launches", "persistent_launches", "queue_items",
         "ns_queue", "ns_stage", "ns_flush", "ns_wait", "ns_retire", "ns_wake", "ma_copies", "ma_wins",
-        "oom_waits")]
+        "oom_waits", "sentinel_compressed")]
 
 
 class ua_if_config(C.Structure):
@@ -82,6 +82,7 @@ def _declare_apps(lib: C.CDLL) -> None:
         "ua_set_hip_mode": (i32, [i32]), "ua_set_hip_compact": (i32, [i32]),
         "ua_set_hip_graphs": (i32, [i32]), "ua_set_hip_depth": (i32, [i32]),
         "ua_set_batch_limit": (i32, [i32]), "ua_set_hbm_budget": (i32, [C.c_uint64]),
+        "ua_set_sentinel_threshold": (i32, [C.c_uint64, C.POINTER(C.c_uint64)]),
         "ua_set_split": (i32, [i32, i32, i32, i32]), "ua_set_multi_assign": (i32, [i32]),
         "ua_get_stats": (i32, [C.POINTER(ua_stats)]), "ua_reset_stats": (i32, []),
         "ua_last_error": (C.c_char_p, []),
@@ -188,6 +189,15 @@ def set_hip_mode(mode: int, compact: bool = False, graphs: bool = True) -> None:
 def set_hbm_budget(nbytes: int) -> None:
     """HBM budget of the runtime in bytes (0 = the default)."""
     _check(apps_lib().ua_set_hbm_budget(nbytes), "ua_set_hbm_budget")
+
+
+def set_sentinel_threshold(nbytes: int) -> int:
+    """Size above which PLAIN-mode reduction outputs go back sentinel-compressed
+    (pmSetSentinelCompressionThreshold; the default is 1 GiB). Returns the value replaced."""
+    init()
+    prev = C.c_uint64(0)
+    _check(apps_lib().ua_set_sentinel_threshold(nbytes, C.byref(prev)), "pmSetSentinelCompressionThreshold")
+    return int(prev.value)
 
 
 def set_split(cpu: bool = True, gpu: bool = False, cpu_pieces: int = 0, gpu_pieces: int = 0) -> None:

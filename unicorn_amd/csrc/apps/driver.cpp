@@ -55,6 +55,12 @@ int ua_set_multi_assign(int on) {
     return 0;
 }
 int ua_set_hbm_budget(uint64_t bytes) { return (int)pmSetHbmBudget((size_t)bytes); }
+int ua_set_sentinel_threshold(uint64_t bytes, uint64_t* previous) {
+    size_t prev = 0;
+    const int rc = (int)pmSetSentinelCompressionThreshold((size_t)bytes, &prev);
+    if (previous) *previous = prev;
+    return rc;
+}
 
 int ua_get_stats(ua_stats* s) {
     pmHipStubStats h;
@@ -76,6 +82,7 @@ int ua_get_stats(ua_stats* s) {
     s->ma_copies = h.ma_copies;
     s->ma_wins = h.ma_wins;
     s->oom_waits = h.oom_waits;
+    s->sentinel_compressed = h.sentinel_compressed;
     s->plain_launches = s->persistent_launches = s->queue_items = 0;
     if (pmHasHipDevice()) uh_launch_counters(&s->plain_launches, &s->persistent_launches, &s->queue_items);
     return 0;

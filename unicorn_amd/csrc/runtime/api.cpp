@@ -585,6 +585,11 @@ pmStatus pmGetHipStubStats(pmHipStubStats* s) {
     *s = r->stats;
     return pmSuccess;
 }
+pmStatus pmSetSentinelCompressionThreshold(size_t bytes, size_t* previous) {
+    const size_t prev = g_sentinel_threshold.exchange(bytes);
+    if (previous) *previous = prev;
+    return pmSuccess;
+}
 pmStatus pmResetHipStubStats() {
     Runtime* r = Runtime::get();
     if (!r) return pmInitializationFailure;

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <functional>
 #include <unordered_map>
@@ -27,6 +28,8 @@ namespace pm {
 namespace rt {
 
 Runtime* Runtime::inst_ = nullptr;
+
+std::atomic<size_t> g_sentinel_threshold{(size_t)1 << 30};   // CUDA_SENTINEL_COMPRESSION_THRESHOLD
 
 double now_secs() {
     using namespace std::chrono;
@@ -775,7 +778,9 @@ struct GpuSlot {
     // device buffers owned by this subtask (freed at completion)
     std::vector<std::pair<void*, size_t>> dev_bufs;
     // PLAIN: written records to scatter from the pinned buffer at completion
-    struct Scatter { uint mem; size_t src_off, len, pinned_off; };
+    // sent >= 0: the pinned bytes hold the record sentinel-compressed, `sent` non-zero elements
+    // of sent_elem bytes (uint64 count, the elements, their uint32 indices)
+    struct Scatter { uint mem; size_t src_off, len, pinned_off; int64_t sent = -1; uint32_t sent_elem = 0; };
     std::vector<Scatter> scatters;
     uint8_t* pinned = nullptr;
     size_t pinned_cap = 0;
@@ -1242,7 +1247,23 @@ static void retire_slot(Runtime* r, GpuSlot& s, bool plain) {
                         s.st->shadow_len[sc.mem] = u.length;
                         s.st->shadow_on_device[sc.mem] = false;
                     }
-                    memcpy((uint8_t*)s.st->shadow[sc.mem] + (sc.src_off - u.offset), s.pinned + sc.pinned_off, sc.len);
+                    uint8_t* dst = (uint8_t*)s.st->shadow[sc.mem] + (sc.src_off - u.offset);
+                    if (sc.sent >= 0) {
+                        // expand the sentinel-compressed record (reference
+                        // UncompressSentinelCompressedData, pmExecutionStub.cpp:3883-3950)
+                        const uint8_t* p = s.pinned + sc.pinned_off;
+                        const uint64_t cnt = (uint64_t)sc.sent;
+                        const size_t es = sc.sent_elem;
+                        const uint8_t* vals = p + 8;
+                        const uint32_t* idx = (const uint32_t*)(vals + cnt * es);
+                        memset(dst, 0, sc.len);
+                        for (uint64_t j = 0; j < cnt; ++j) {
+                            if ((size_t)idx[j] * es + es > sc.len) raise(pmFatalError, "sentinel index out of range");
+                            memcpy(dst + (size_t)idx[j] * es, vals + j * es, es);
+                        }
+                    } else {
+                        memcpy(dst, s.pinned + sc.pinned_off, sc.len);
+                    }
                 } else if (t->deferred(sc.mem)) {
                     t->defer_write(sid, sc.mem, sc.src_off, s.pinned + sc.pinned_off, sc.len);
                 } else {
@@ -1625,7 +1646,45 @@ void Runtime::gpu_loop() {
                     if (plain) {
                         // D2H of written records into the pinned buffer (pmCudaInterface.cu:510-527)
                         uint64_t bytes = 0;
+                        // sentinel compression (reference CheckSentinelCompression,
+                        // pmExecutionStub.cpp:3850-3881): a built-in reduction over the task's only
+                        // address space, written whole by one record, above the threshold
+                        pmReductionOpType sop;
+                        pmReductionDataType sdt = MAX_REDUCTION_DATA_TYPES;
+                        const bool sentinel_task =
+                            t->spaces.size() == 1 && t->writable(0) && t->is_reduced(0) &&
+                            is_builtin_reduction(t->callbacks->cb.dataReduction, &sop, &sdt) &&
+                            t->spaces[0]->length > g_sentinel_threshold.load() && s.scatters.size() == 1 &&
+                            s.scatters[0].mem == 0 && s.scatters[0].src_off == 0 &&
+                            s.scatters[0].len == t->spaces[0]->length;
+                        if (sentinel_task) {
+                            static const size_t kElem[] = {4, 4, 8, 8, 4, 8};   // pmReductionDataType sizes
+                            const size_t es = kElem[sdt];
+                            const size_t len = s.scatters[0].len;
+                            const uint64_t elems = len / es;
+                            // compressed when fewer than 40 % of the elements are non-zero
+                            // (CUDA_SENTINEL_COMPRESSION_MAX_NON_SENTINELS, pmInternalDefinitions.h:224)
+                            const uint64_t lim = (uint64_t)std::ceil((double)elems * 0.4);
+                            const uint64_t max_nz = lim ? lim - 1 : 0;
+                            const size_t cbytes = 8 + (size_t)max_nz * (es + 4);
+                            if (elems * es == len && cbytes <= len) {
+                                void* comp = slot_device_alloc(s, cbytes);
+                                uint64_t nz = 0;
+                                hipchk(uh_sentinel_compress(s.dev_scatters[0].buf, elems, (int)sdt, comp, max_nz, &nz,
+                                                            s.stream), "sentinel compression");
+                                if (nz <= max_nz) {
+                                    const size_t got = 8 + (size_t)nz * (es + 4);
+                                    hipchk(uh_memcpy_d2h(s.pinned + s.scatters[0].pinned_off, comp, got, s.stream),
+                                           "D2H staging");
+                                    s.scatters[0].sent = (int64_t)nz;
+                                    s.scatters[0].sent_elem = (uint32_t)es;
+                                    bytes += got;
+                                    stat_add(&pmHipStubStats::sentinel_compressed, 1);
+                                }
+                            }
+                        }
                         for (size_t k = 0; k < s.scatters.size();) {
+                            if (s.scatters[k].sent >= 0) { ++k; continue; }
                             size_t v = k + 1, len = s.scatters[k].len;
                             while (v < s.scatters.size() &&
                                    s.scatters[v].mem == s.scatters[k].mem &&

@@ -396,6 +396,10 @@ void commit_redistribution(Task* t);
 // private_writes: writable NATURAL views point into shadows (multi-assigned CPU subtasks).
 pmSubtaskInfo host_subtask_info(Task* t, Subtask& st, std::vector<std::vector<uint8_t>>& compact_bufs,
                                 bool private_writes = false);
+// Size above which PLAIN-mode reduction outputs go back sentinel-compressed
+// (pmSetSentinelCompressionThreshold; the reference's 1 GiB by default).
+extern std::atomic<size_t> g_sentinel_threshold;
+
 // Reduction helpers
 bool is_builtin_reduction(pmDataReductionCallback cb, pmReductionOpType* op, pmReductionDataType* dt);
 void reduce_host(void* dst, const void* src, size_t bytes, pmReductionOpType op, pmReductionDataType dt);

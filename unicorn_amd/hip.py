@@ -73,6 +73,7 @@ def lib() -> C.CDLL:
             "uh_fft_c2c_f32": (i32, [vp, u64, u64, vp, u64, u64, u32, u32, vp]),
             "uh_invert_index": (i32, [vp, u32, u32, u32, u32, u32, u32, vp, vp, vp]),
             "uh_prii_rank": (i32, [vp, u32, vp, f32, f32, u32, vp, u32, u32, vp]),
+            "uh_sentinel_compress": (i32, [vp, u64, i32, vp, u64, C.POINTER(u64), vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
