@@ -753,9 +753,13 @@ __device__ __forceinline__ void fm_step(const FilterArgs& a, const FilterWeights
         const int mis = (int)((uintptr_t)a.in & 15u);
         const int q = fm_row_byte<R>(a, v, cx, cy, 0, mis) + 8;   // first B byte of row 0, wave 0
         FmAcc acc;
+        // products at raised wave priority: the other workgroup's wave on this SIMD issues its
+        // staging, stores and loads in the MFMAs' shadow instead of ahead of them
+        __builtin_amdgcn_s_setprio(1);
         if ((a.in_stride & 15u) == 0 && (q & 15) == 0) fm_compute<R, 16>(a, w, A, cx, cy, cur, acc);
         else if ((a.in_stride & 7u) == 0 && (q & 7) == 0) fm_compute<R, 8>(a, w, A, cx, cy, cur, acc);
         else fm_compute<R, 1>(a, w, A, cx, cy, cur, acc);
+        __builtin_amdgcn_s_setprio(0);
         fm_stage_out<MODE>(a, acc, outw);
     }
     if (ld) fm_barrier<fm_dma_min<R>()>();
