@@ -172,8 +172,8 @@ typedef struct uh_imagefilter_args {
     int8_t weights[32 * 32];                    /* row-major (2r+1)x(2r+1), r <= 15 */
 } uh_imagefilter_args;
 int uh_imagefilter(const uh_imagefilter_args* args, void* stream);
-/* Engine of radius 1 and 2 for launches and work items recorded afterwards: 0 = v_dot4
- * (default), 1 = int8 MFMA (UH_FILTER_ENGINE=mfma at start); -1 only reports. Both give the same
+/* Engine of radius 1 and 2 for launches and work items recorded afterwards: 1 = int8 MFMA
+ * (default), 0 = v_dot4 (UH_FILTER_ENGINE=dot4 at start); -1 only reports. Both give the same
  * bytes. *previous (if not NULL) receives the engine before the call. */
 int uh_imagefilter_engine(int engine, int* previous);
 

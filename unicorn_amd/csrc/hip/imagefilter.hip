@@ -7,8 +7,11 @@
 // imageFiltering.cpp:353-361); rows and columns are clamped to the image edges.
 //
 // Design. The reference stages every tile through a D2D copy into pitched texture memory and
-// reads three separate texels per pixel. Here each workgroup computes 256 x 16 output tiles
-// straight from the (HBM-resident) view, several tiles per workgroup, as a pipeline:
+// reads three separate texels per pixel. Here the tiles are computed straight from the
+// (HBM-resident) view by one of two engines. The default int8 MFMA engine (the fm_* functions
+// below) computes 128 x 32 tiles as banded products over the raw interleaved bytes, through a
+// three-buffer direct-to-LDS window ring and an LDS output staging area. The v_dot4 engine
+// computes 256 x 16 tiles, several per workgroup, as a pipeline:
 //   1. stage: the (16+2r)-row input window (264 pixels from x0 - 4) moves from HBM to LDS with
 //      16-byte direct-to-LDS buffer loads, one tile ahead of the compute, holding no registers;
 //   2. de-interleave: v_alignbyte extracts each 4-pixel group's 12 bytes, v_perm splits them into
