@@ -49,6 +49,7 @@ def engine(gpu_runtime, request):
     (257, 1030, 3, 0, 256),    # radius 3: direct kernel
     (1030, 2100, 2, 1, 512),
     (333, 1001, 1, 1, 300),    # radius 1, odd widths: staged rows at every byte alignment
+    (200, 1000, 2, 1, 200),    # 3000-byte rows: subtask columns alternate 16- and 8-byte aligned
 ])
 def test_imagefilter_gpu_matches_oracle(gpu_runtime, engine, name, hip_mode, compact, graphs, H, W, r, mode, tile):
     U = gpu_runtime
