@@ -607,10 +607,11 @@ def app_row(name, workload, run_gpu, steps, parity, cpu_run, cores, kernel, dtyp
             "roofline": kernel, "cpu_baseline": cpu}
 
 
-def hbm_roofline(kernel_name, nbytes, ms, what):
+def hbm_roofline(kernel_name, nbytes, ms, what, profile=None):
+    """`profile` names the profiles/<profile>_traffic.json holding the kernel's measured DRAM bytes."""
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kernel_name),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(profile) if profile else None,
             "kernel": kernel_name, "kernel_ms": round(ms, 5), "algorithmic_bytes_per_launch": int(nbytes),
             "per_launch": what}
 
@@ -648,8 +649,9 @@ def bench_apps(args, U, steps, warmup, cores):
     ms_col = kernel_ms(lambda s: L.uh_fft_c2c_f32(d_in.ptr, 1, n, d_out.ptr, 1, n, n, 512, s))
     d_in.free()
     d_out.free()
-    kern = hbm_roofline("k_fft (columns)", 512 * n * 16, ms_col, "512 columns of 4096 points (NATURAL view)")
-    kern["rows_kernel"] = hbm_roofline("k_fft (rows)", 512 * n * 16, ms_row, "512 rows of 4096 points")
+    kern = hbm_roofline("k_fft (columns)", 512 * n * 16, ms_col, "512 columns of 4096 points (NATURAL view)",
+                        "fft_4096_columns")
+    kern["rows_kernel"] = hbm_roofline("k_fft (rows)", 512 * n * 16, ms_row, "512 rows of 4096 points", "fft_4096_rows")
     head = app_row("fft 4096x4096", "fft 4096x4096 complex float, 512 rows / columns per subtask",
                    lambda: f.run(U.DEVICE_GPU), steps, parity, (lambda: f.run(U.DEVICE_CPU)) if use_cpu else None,
                    cores, kern, "c64 (f32 pairs)")
@@ -671,7 +673,7 @@ def bench_apps(args, U, steps, warmup, cores):
         "matrixTranspose 16384x4096", "matrixTranspose 16384x4096 int32 out of place, 2048-square blocks",
         lambda: mt.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "M.T"},
         (lambda: mt.run(U.DEVICE_CPU)) if use_cpu else None, cores,
-        hbm_roofline("k_transpose32", 2 * 2048 * 2048 * 4, ms, "one 2048x2048 block"), "i32")
+        hbm_roofline("k_transpose32", 2 * 2048 * 2048 * 4, ms, "one 2048x2048 block", "transpose_2048"), "i32")
     mt.close()
 
     log("apps: prefixSum 10M")
@@ -690,7 +692,8 @@ def bench_apps(args, U, steps, warmup, cores):
         "prefixSum 10M", "prefixSum 10,000,000 int32, 1,000,000 per subtask (scan task, host aux scan, add task)",
         lambda: ps.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "oracle serialPrefixSum"},
         (lambda: ps.run(U.DEVICE_CPU)) if use_cpu else None, cores,
-        hbm_roofline("k_scan_totals + k_scan_blocks", 8 * 1_000_000, ms, "one 1,000,000-element chunk (both passes)"),
+        hbm_roofline("k_scan_totals + k_scan_blocks", 8 * 1_000_000, ms, "one 1,000,000-element chunk (both passes)",
+                     "prefix_sum_1m"),
         "i32")
     ps.close()
 
@@ -715,7 +718,7 @@ def bench_apps(args, U, steps, warmup, cores):
         lambda: sm.run(U.DEVICE_GPU), steps, {"checked": True, "bit_exact": ok, "against": "oracle COO loop (float)"},
         (lambda: sm.run(U.DEVICE_CPU)) if use_cpu else None, cores,
         hbm_roofline("k_spmm_coo", 12 * r_.size + 4 * r_.size * n + 4 * n * n, ms,
-                     "one 1024-row block: entries, the B rows they select, C"), "f32")
+                     "one 1024-row block: entries, the B rows they select, C", "sparse_1024"), "f32")
     sm.close()
 
     log("apps: pageRankInvertedIndex 1M pages")
@@ -746,9 +749,11 @@ def bench_apps(args, U, steps, warmup, cores):
         ms_rank = kernel_ms(lambda s: L.uh_prii_rank(d_e.ptr, n0, d_x.ptr, 1.0, 0.85, 0, d_y.ptr, P, pages, s))
         d_e.free(); d_x.free(); d_y.free()
         kern = hbm_roofline("uh_invert_index (4 kernels)", 4 * P + 4 * links + 12 * links, ms_inv,
-                            f"one 250K-page subtask: counts and {links} links read, {links} entries written")
+                            f"one 250K-page subtask: counts and {links} links read, {links} entries written",
+                            "prii_invert_250k")
         kern["rank_kernel"] = hbm_roofline("k_prii_rank", 12 * n0 + 4 * n0 + 4 * P, ms_rank,
-                                           f"one subtask's {n0} inlinks: entries, source ranks, output ranks")
+                                           f"one subtask's {n0} inlinks: entries, source ranks, output ranks",
+                                           "prii_rank_250k")
         rows["pageRankInvertedIndex"] = app_row(
             "pageRankInvertedIndex 1M pages", f"pageRankInvertedIndex {pages} pages ({col.size} links) from a web "
             f"dump, 250K pages per subtask, inversion + 2 iterations", lambda: a.run(U.DEVICE_GPU, 2), max(2, steps // 4),
